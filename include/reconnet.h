@@ -1,0 +1,202 @@
+/*
+ * C-ABI of libReconNet4D_no_tf.so for AMD Instinct MI355X (gfx950).
+ *
+ * Part 1 reproduces, symbol for symbol and argument for argument, the 17
+ * extern "C" exports of the reference library (wudufan/ReconNet,
+ * ReconNet4D_no_tf/), so that python/ReconNet_no_tf.py keeps working over
+ * ctypes.  Their contract is the reference's: host pointers, caller-owned and
+ * pre-allocated outputs, C-contiguous float32 arrays
+ *   images    [nBatches][nx][ny][nz][nChannels]
+ *   sinograms [nBatches][nu][nview][nv][nChannels]
+ *   float3    arrays [nview][3]
+ * and results on the host when the call returns.  Unlike the reference no C++
+ * exception crosses the boundary: a failure prints "<function> failed: <reason>"
+ * on stderr and is readable through rn_last_error() (part 2); outputs are then
+ * left unspecified.
+ *
+ * Part 2 is additive: the error query, device-pointer entry points on
+ * caller-owned HIP streams (used by the Python package and by torch), a
+ * batched device-resident OS-SQS engine, and the BF16 convolution kernels of
+ * the learned prior.  No torch types appear in any signature.
+ */
+#ifndef RECONNET_H
+#define RECONNET_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================== Part 1: reference exports ===================== */
+
+/* Projector.cu:12 */
+int SetDevice(int device);
+
+/* DistanceDriven.cu:386-389.  typeProjector is accepted and ignored, as in the reference FP. */
+void cDistanceDrivenFanProjection(float* prj, const float* img, const float* deg,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeProjector);
+
+/* DistanceDriven.cu:793-796.  typeProjector 1 = FBP weighting, otherwise length pre-weight. */
+void cDistanceDrivenFanBackprojection(float* img, const float* prj, const float* deg,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeProjector);
+
+/* DistanceDriven3D.cu:346-350 (detU = (1,0,0), detV = (0,1,0) implied) */
+void cDistanceDrivenTomoProjection(float* prj, const float* img,
+        const float* detCenter, const float* src,
+        int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v);
+
+/* DistanceDriven3D.cu:666-670 */
+void cDistanceDrivenTomoBackprojection(float* img, const float* prj,
+        const float* detCenter, const float* src,
+        int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v);
+
+/* Filter.cu:432-435.  typeFilter 0 = Ram-Lak, 1 = Hamming, 2 = Hann, 3 = cosine. */
+void cFilterFanFilter(float* pFPrj, const float* pPrj,
+        int nBatches, int nChannels, int nu, int nview,
+        int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeFilter, int typeProjector);
+
+/* Filter.cu:476-479 */
+void cFilterParallelFilter(float* pFPrj, const float* pPrj,
+        int nBatches, int nChannels, int nu, int nview,
+        int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeFilter, int typeProjector);
+
+/* PixelDriven.cu:100-103 (always FBP-weighted, PixelDriven.cu:93) */
+void cPixelDrivenFanBackprojection(float* pImg, const float* pPrj, const float* pDeg,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeProjector);
+
+/* Siddon.cu:424-427 */
+void cSiddonFanProjection(float* prj, const float* img, const float* deg,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeProjector);
+
+/* Siddon.cu:481-484 */
+void cSiddonFanBackprojection(float* img, const float* prj, const float* deg,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeProjector);
+
+/* SiddonParallel.cu:152-156 (output zeroed first; the reference leaves it uninitialised) */
+void cSiddonParallelProjection(float* prj, const float* img,
+        const float* detCenter, const float* detU, const float* detV, const float* invRayDir,
+        int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v);
+
+/* SiddonParallel.cu:230-234 (runs over every ray; the reference's launch grid misses rays) */
+void cSiddonParallelBackprojection(float* img, const float* prj,
+        const float* detCenter, const float* detU, const float* detV, const float* invRayDir,
+        int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v);
+
+/* SiddonCone.cu:117-121 */
+void cSiddonConeProjectionAbitrary(float* prj, const float* img,
+        const float* detCenter, const float* detU, const float* detV, const float* src,
+        int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v);
+
+/* SiddonCone.cu:196-200 */
+void cSiddonConeBackprojectionAbitrary(float* img, const float* prj,
+        const float* detCenter, const float* detU, const float* detV, const float* src,
+        int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v);
+
+/* ossqs.cu:12-17.  A short last subset uses nview - startView views. */
+void cOSSQS(float* res, const float* img, const float* prj,
+        const float* normImg, const float* wprj,
+        const float* degs, float lb, float ub, int nviewPerSubset, int nSubsets,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, int typeProjector, float lambda);
+
+/* ossqs.cu:163-170 */
+void cOSSQSConeAbitrary(float* res, const float* img, const float* prj,
+        const float* normImg, const float* mask,
+        const float* detCenter, const float* detU, const float* detV, const float* src,
+        float lb, float ub,
+        int nviewPerSubset, int nSubsets, int nBatches, int nChannels,
+        int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v,
+        int typeProjector, float lambda);
+
+/* TotalVariation.cu:70.  The returned cost is reduced in FP64 (the reference sums FP32 on the host). */
+float cTVSQS2D(float* grad, float* norm, const float* img, int nx, int ny);
+
+/* Recon3D/HelicalBP.cu:132-135 (exported by the reference's separate libRecon3D) */
+void cHelicalBPFromParallelConebeamBackprojection(float* pImg, const float* pPrj,
+        int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v, float dsd, float dso,
+        int nviewPerPI, float theta0, float volZ0, float zrot, int mPI, float Q);
+
+/* ===================== Part 2: additive API (no reference counterpart) ===================== */
+
+/* Message of the last failure on the calling thread ("" if the last call succeeded). */
+const char* rn_last_error(void);
+
+/* Fan geometry plan: trig tables, the float32 X/Y view split and workspaces
+ * for one (geometry, angle set).  deg is a host array of nview angles.  A plan
+ * is used by one host thread at a time.  Returns NULL on failure. */
+void* rn_fan_plan_create(int nx, int ny, int nz, float dx, float dy, float dz,
+        int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
+        float dsd, float dso, const float* deg);
+void rn_fan_plan_destroy(void* plan);
+
+/* Operator kinds for rn_fan_forward / rn_fan_backward */
+enum {
+    RN_FP_DD = 0,          /* DDFPFanKernelX/Y */
+    RN_FP_SIDDON = 1,      /* SiddonFanProjectionKernel */
+    RN_BP_DD = 0,          /* DDBPFanKernelX/Y, iterative (length pre-weight) */
+    RN_BP_DD_FBP = 1,      /* DDBPFanKernelX/Y, FBP weighting */
+    RN_BP_DD_ADJOINT = 2,  /* exact transpose of RN_FP_DD */
+    RN_BP_SIDDON = 3,      /* exact transpose of RN_FP_SIDDON (gather, no atomics) */
+    RN_BP_PIXEL = 4        /* bpFanKernel3D (FBP weighting) */
+};
+
+/* Device pointers in the reference layouts, enqueued on `stream` (hipStream_t,
+ * NULL = default stream).  Outputs are overwritten.  0 on success, -1 on error. */
+int rn_fan_forward(void* plan, int kind, const float* img, float* prj, int nBatches, int nChannels, void* stream);
+int rn_fan_backward(void* plan, int kind, const float* prj, float* img, int nBatches, int nChannels, void* stream);
+/* Ramp filter (cFilterFanFilter when fan != 0, else cFilterParallelFilter) on device sinograms. */
+int rn_fan_filter(void* plan, const float* prj, float* fprj, int nBatches, int nChannels,
+        int typeFilter, int fan, void* stream);
+
+/* Batched OS-SQS(-TV) engine over nslices 2D fan slices sharing one plan
+ * geometry (nz = nv = 1).  Images are [nslices][nx][ny], sinograms
+ * [nslices][nu][nview], all device pointers. */
+void* rn_sqs_create(void* plan, int nslices, int nviewPerSubset, int nSubsets);
+void rn_sqs_destroy(void* sqs);
+int rn_sqs_set_data(void* sqs, const float* prj, const float* wprj, void* stream);
+int rn_sqs_compute_norm(void* sqs, void* stream);        /* D = A'(w A 1), FP64 accumulation */
+int rn_sqs_set_norm(void* sqs, const float* norm, void* stream);
+int rn_sqs_get_norm(void* sqs, float* norm, void* stream);
+int rn_sqs_set_image(void* sqs, const float* img, void* stream);
+int rn_sqs_get_image(void* sqs, float* img, void* stream);
+/* One pass over all subsets: x <- clamp(x - (lambda M g + 2 beta gTV) / (D + 2 beta nTV + 1e-6)).
+ * beta = 0 reproduces cOSSQS.  If track != NULL it receives, per slice, the FP64 sums
+ * |x_new - x_old|^2 and |x_new|^2 over the pass (2 * nslices doubles). */
+int rn_sqs_pass(void* sqs, float lambda, float beta, float eps, float lb, float ub, double* track, void* stream);
+
+/* TV surrogate on nslices device images [nslices][nx][ny]; tv (host, may be NULL) receives
+ * the FP64 per-slice costs. */
+int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
+        double* tv, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RECONNET_H */

@@ -1,0 +1,51 @@
+// Arbitrary-geometry (parallel / cone) Siddon, DD tomosynthesis, cone OS-SQS
+// and helical BP on reference-layout device buffers.
+#pragma once
+
+#include "rn_common.h"
+
+namespace rn {
+
+struct ArbGeo {
+    int nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+    int nu, nview, nv;
+    float du, dv, off_u, off_v;
+};
+
+// Siddon with per-view detCenter / detU / detV and either invRayDir
+// (parallel, SiddonParallel.cu) or the source position (cone, SiddonCone.cu).
+// All float3 arrays are device [nview][3].
+void siddon_arb_fp(const ArbGeo& g, bool parallel, int nB, int nC, const float* img, float* prj,
+                   const float* detCenter, const float* detU, const float* detV, const float* srcOrDir,
+                   DevBuf& work, hipStream_t st);
+void siddon_arb_bp(const ArbGeo& g, bool parallel, int nB, int nC, const float* prj, float* img,
+                   const float* detCenter, const float* detU, const float* detV, const float* srcOrDir,
+                   DevBuf& work, hipStream_t st);
+
+// DistanceDriven3D.cu tomosynthesis projector pair (detector axes = x, y)
+void dd_tomo_fp(const ArbGeo& g, int nB, int nC, const float* img, float* prj,
+                const float* detCenter, const float* src, DevBuf& work, hipStream_t st);
+void dd_tomo_bp(const ArbGeo& g, int nB, int nC, const float* prj, float* img,
+                const float* detCenter, const float* src, DevBuf& work, hipStream_t st);
+
+// cOSSQSConeAbitrary (ossqs.cu:163-333); x is updated in place
+void os_sqs_cone(const ArbGeo& g, int nB, int nC, float* x, const float* prj, const float* norm, const float* mask,
+                 const float* detCenter, const float* detU, const float* detV, const float* src,
+                 float lb, float ub, int nviewPerSubset, int nSubsets, float lambda, DevBuf& work, hipStream_t st);
+
+struct HelicalGeo {
+    int nx, ny, nz;
+    float dx, dy, dz;
+    int nu, nview, nv;
+    float du, dv, off_u, off_v, dsd, dso;
+    int nviewPerPI;
+    float theta0, volZ0, zrot;
+    int mPI;
+    float Q;
+};
+
+// bpHelicalParallelConebeam (Recon3D/HelicalBP.cu:10-83)
+void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st);
+
+} // namespace rn

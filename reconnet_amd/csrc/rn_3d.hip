@@ -1,0 +1,475 @@
+// 3D projectors for gfx950: arbitrary-geometry Siddon (parallel / cone),
+// DD tomosynthesis, the cone OS-SQS data step and the helical BP.
+//
+// Reference counterparts: Siddon.cu:11-351 (tracing), SiddonParallel.cu:11-105,
+// SiddonCone.cu:11-68, DistanceDriven3D.cu:139-227 / 409-559, ossqs.cu:163-333,
+// Recon3D/HelicalBP.cu:10-83.
+//
+// Siddon tracing keeps the reference's voxel walk, but each crossing
+// parameter is computed directly from the boundary index, (b - s) / d, instead
+// of by repeated FP32 addition, so rounding does not accumulate along a ray.
+// The arbitrary-geometry BP is the exact transpose of that walk; with detector
+// axes and sources free per view there is no cheap footprint inversion, so it
+// scatters with hardware float atomics (global_atomic_add_f32).  The DD
+// tomosynthesis pair integrates footprints by direct 2D overlap sums in FP32
+// instead of the reference's FP64 integral images.
+
+#include "rn_3d.h"
+#include "rn_fan.h"
+
+#include <cmath>
+#include <vector>
+
+namespace rn {
+
+namespace {
+
+struct Grid3K {
+    int nx, ny, nz;
+    float dx, dy, dz, cx, cy, cz;
+};
+
+struct F3 { float x, y, z; };
+
+__device__ __forceinline__ F3 ld3(const float* p, int i) { return F3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// MoveSourceDstNearGrid (SiddonParallel.cu:35-55, cudaMath.h:25-60)
+__device__ void move_near_grid(F3& s, F3& t, const Grid3K& g)
+{
+    const float ex = g.nx * g.dx, ey = g.ny * g.dy, ez = g.nz * g.dz;
+    const float r = sqrtf(ex * ex + ey * ey + ez * ez);
+    float dx = t.x - s.x, dy = t.y - s.y, dz = t.z - s.z;
+    const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (len >= 1e-6f) { dx /= len; dy /= len; dz /= len; }
+    const float t0 = (g.cx - s.x) * dx + (g.cy - s.y) * dy + (g.cz - s.z) * dz;
+    const float t1 = (t.x - g.cx) * dx + (t.y - g.cy) * dy + (t.z - g.cz) * dz;
+    if (t0 > r) { s.x += (t0 - r) * dx; s.y += (t0 - r) * dy; s.z += (t0 - r) * dz; }
+    if (t1 > r) { t.x -= (t1 - r) * dx; t.y -= (t1 - r) * dy; t.z -= (t1 - r) * dz; }
+}
+
+__device__ __forceinline__ void arb_ray(const ArbGeo& a, const Grid3K& g, bool parallel, const float* C,
+                                        const float* U, const float* V, const float* S, int view, int iu, int iv,
+                                        F3& src, F3& dst)
+{
+    const float u = (iu - a.off_u - (a.nu - 1) / 2.0f) * a.du;
+    const float v = (iv - a.off_v - (a.nv - 1) / 2.0f) * a.dv;
+    const F3 c = ld3(C, view), du = ld3(U, view), dv = ld3(V, view), sd = ld3(S, view);
+    dst = F3{c.x + du.x * u + dv.x * v, c.y + du.y * u + dv.y * v, c.z + du.z * u + dv.z * v};
+    if (parallel) {
+        // GetSrcDstForParallel (SiddonParallel.cu:11-32)
+        const float m1 = sqrtf((g.cx - c.x) * (g.cx - c.x) + (g.cy - c.y) * (g.cy - c.y) + (g.cz - c.z) * (g.cz - c.z));
+        const float ex = g.nx * g.dx, ey = g.ny * g.dy, ez = g.nz * g.dz;
+        const float m2 = sqrtf(ex * ex + ey * ey + ez * ez) / 2;
+        const float m = fmaxf(m1, m2) * 1.1f;
+        src = F3{c.x + sd.x * m + du.x * u + dv.x * v, c.y + sd.y * m + du.y * u + dv.y * v,
+                 c.z + sd.z * m + du.z * u + dv.z * v};
+    } else {
+        src = sd;
+    }
+    move_near_grid(src, dst, g);
+}
+
+// Siddon walk (Siddon.cu:90-238).  f(voxel linear index x*ny*nz + y*nz + z, length)
+template <class F>
+__device__ void siddon_walk(F3 src, F3 dst, const Grid3K& g, F f)
+{
+    const float eps = 1e-6f;
+    const float dist = sqrtf((src.x - dst.x) * (src.x - dst.x) + (src.y - dst.y) * (src.y - dst.y) +
+                             (src.z - dst.z) * (src.z - dst.z));
+    const float sx = (src.x - g.cx) / g.dx + g.nx / 2.0f, sy = (src.y - g.cy) / g.dy + g.ny / 2.0f,
+                sz = (src.z - g.cz) / g.dz + g.nz / 2.0f;
+    const float tx = (dst.x - g.cx) / g.dx + g.nx / 2.0f, ty = (dst.y - g.cy) / g.dy + g.ny / 2.0f,
+                tz = (dst.z - g.cz) / g.dz + g.nz / 2.0f;
+    const float ddx = tx - sx, ddy = ty - sy, ddz = tz - sz;
+    float a0x = 1, a1x = 0, a0y = 1, a1y = 0, a0z = 1, a1z = 0;
+    if (fabsf(ddx) > eps) { a0x = -sx / ddx; a1x = (g.nx - sx) / ddx; } else if (sx < 0 || sx > g.nx) return;
+    if (fabsf(ddy) > eps) { a0y = -sy / ddy; a1y = (g.ny - sy) / ddy; } else if (sy < 0 || sy > g.ny) return;
+    if (fabsf(ddz) > eps) { a0z = -sz / ddz; a1z = (g.nz - sz) / ddz; } else if (sz < 0 || sz > g.nz) return;
+    const float amin = fmaxf(0.f, fmaxf(fminf(a0x, a1x), fmaxf(fminf(a0y, a1y), fminf(a0z, a1z))));
+    const float amax = fminf(1.f, fminf(fmaxf(a0x, a1x), fminf(fmaxf(a0y, a1y), fmaxf(a0z, a1z))));
+    if (amax <= amin) return;
+    int gx = (int)(sx + amin * ddx), gy = (int)(sy + amin * ddy), gz = (int)(sz + amin * ddz);
+    if (fabsf(ddx) > eps) gx = min(max(gx, 0), g.nx - 1);
+    if (fabsf(ddy) > eps) gy = min(max(gy, 0), g.ny - 1);
+    if (fabsf(ddz) > eps) gz = min(max(gz, 0), g.nz - 1);
+    if (gx < 0 || gx >= g.nx || gy < 0 || gy >= g.ny || gz < 0 || gz >= g.nz) return;
+    const int px = ddx > 0, py = ddy > 0, pz = ddz > 0;
+    // next boundary index per axis (OutboundFirstVoxel) and its crossing parameter
+    float bx = (ddx > eps) ? gx + 1 : gx, by = (ddy > eps) ? gy + 1 : gy, bz = (ddz > eps) ? gz + 1 : gz;
+    const bool mx = fabsf(ddx) > eps, my = fabsf(ddy) > eps, mz = fabsf(ddz) > eps;
+    const float ix = mx ? dist / ddx : 0.f, iy = my ? dist / ddy : 0.f, iz = mz ? dist / ddz : 0.f;
+    const float stx = px ? 1.f : -1.f, sty = py ? 1.f : -1.f, stz = pz ? 1.f : -1.f;
+    float ax = mx ? (bx - sx) * ix : 2.f * dist;
+    float ay = my ? (by - sy) * iy : 2.f * dist;
+    float az = mz ? (bz - sz) * iz : 2.f * dist;
+    // with no motion along an axis the reference keeps alpha = 2*dist + k*0
+    float prev = amin * dist;
+    const int maxit = g.nx + g.ny + g.nz + 4;
+    for (int it = 0; it < maxit; ++it) {
+        const int cx = gx, cy = gy, cz = gz;
+        float now;
+        bool done = false;
+        if (ax < ay && ax < az) {
+            now = ax;
+            bx += stx; ax = mx ? (bx - sx) * ix : ax;
+            gx += px ? 1 : -1; done = (gx >= g.nx || gx < 0);
+        } else if (ay < az) {
+            now = ay;
+            by += sty; ay = my ? (by - sy) * iy : ay;
+            gy += py ? 1 : -1; done = (gy >= g.ny || gy < 0);
+        } else {
+            now = az;
+            bz += stz; az = mz ? (bz - sz) * iz : az;
+            gz += pz ? 1 : -1; done = (gz >= g.nz || gz < 0);
+        }
+        f(((long)cx * g.ny + cy) * g.nz + cz, now - prev);
+        prev = now;
+        if (done) break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_siddon_arb_fp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ img,
+                                                       float* __restrict__ prj, const float* C, const float* U,
+                                                       const float* V, const float* S, int nC)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int view = blockIdx.y;
+    const int bc = blockIdx.z;   // b * nv * nC + iv * nC + c  (iv folded in)
+    const int c = bc % nC, biv = bc / nC, iv = biv % a.nv, b = biv / a.nv;
+    if (iu >= a.nu) return;
+    F3 s, t;
+    arb_ray(a, g, parallel != 0, C, U, V, S, view, iu, iv, s, t);
+    const float* base = img + (long)b * g.nx * g.ny * g.nz * nC + c;
+    float val = 0.f;
+    siddon_walk(s, t, g, [&](long vox, float len) { val += base[vox * nC] * len; });
+    prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c] = val;
+}
+
+__global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ prj,
+                                                       float* __restrict__ img, const float* C, const float* U,
+                                                       const float* V, const float* S, int nC)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int view = blockIdx.y;
+    const int bc = blockIdx.z;
+    const int c = bc % nC, biv = bc / nC, iv = biv % a.nv, b = biv / a.nv;
+    if (iu >= a.nu) return;
+    const float p = prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c];
+    if (p == 0.f) return;
+    F3 s, t;
+    arb_ray(a, g, parallel != 0, C, U, V, S, view, iu, iv, s, t);
+    float* base = img + (long)b * g.nx * g.ny * g.nz * nC + c;
+    siddon_walk(s, t, g, [&](long vox, float len) { unsafeAtomicAdd(base + vox * nC, p * len); });
+}
+
+// ---- DD tomosynthesis ----
+
+// average over [x1,x2] x [y1,y2] of a [ny][nx] plane (zero outside)
+__device__ __forceinline__ float box_avg(const float* __restrict__ P, int nx, int ny, float x1, float x2, float y1,
+                                         float y2)
+{
+    const float area = (x2 - x1) * (y2 - y1);
+    const float lx = fmaxf(x1, 0.f), hx = fminf(x2, (float)nx);
+    const float ly = fmaxf(y1, 0.f), hy = fminf(y2, (float)ny);
+    if (!(hx > lx) || !(hy > ly)) return 0.f;
+    const int m0 = (int)lx, m1 = min((int)hx, nx - 1), n0 = (int)ly, n1 = min((int)hy, ny - 1);
+    float s = 0.f;
+    for (int n = n0; n <= n1; ++n) {
+        const float oy = fminf(hy, (float)(n + 1)) - fmaxf(ly, (float)n);
+        if (oy <= 0.f) continue;
+        float r = 0.f;
+        for (int m = m0; m <= m1; ++m) {
+            const float ox = fminf(hx, (float)(m + 1)) - fmaxf(lx, (float)m);
+            r = fmaf(P[(long)n * nx + m], fmaxf(ox, 0.f), r);
+        }
+        s = fmaf(r, oy, s);
+    }
+    return s / area;
+}
+
+// vol_zyx: [slice][nz][ny][nx]
+__global__ __launch_bounds__(128) void k_dd_tomo_fp(ArbGeo a, Grid3K g, const float* __restrict__ vol_zyx,
+                                                    float* __restrict__ prj, const float* C, const float* S, int nC)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int view = blockIdx.y;
+    const int bc = blockIdx.z;
+    const int c = bc % nC, biv = bc / nC, iv = biv % a.nv, b = biv / a.nv;
+    if (iu >= a.nu) return;
+    const F3 cc = ld3(C, view), s0 = ld3(S, view);
+    const float u = (iu - a.off_u - (a.nu - 1) / 2.0f) * a.du;
+    const float v = (iv - a.off_v - (a.nv - 1) / 2.0f) * a.dv;
+    auto tix = [&](float px) { return (px - g.cx) / g.dx + g.nx / 2.0f; };
+    auto tiy = [&](float py) { return (py - g.cy) / g.dy + g.ny / 2.0f; };
+    const float zc = (cc.z - g.cz) / g.dz + g.nz / 2.0f;
+    const float sx = tix(s0.x), sy = tiy(s0.y), sz = (s0.z - g.cz) / g.dz + g.nz / 2.0f;
+    float x1 = tix(cc.x + u - a.du / 2), x2 = tix(cc.x + u + a.du / 2);
+    float y1 = tiy(cc.y + v - a.dv / 2), y2 = tiy(cc.y + v + a.dv / 2);
+    if (x1 > x2) { const float t = x1; x1 = x2; x2 = t; }
+    if (y1 > y2) { const float t = y1; y1 = y2; y2 = t; }
+    const float dzs = zc - sz;
+    const float rx1 = (x1 - sx) / dzs, rx2 = (x2 - sx) / dzs, ry1 = (y1 - sy) / dzs, ry2 = (y2 - sy) / dzs;
+    const long plane = (long)g.nx * g.ny;
+    const float* vol = vol_zyx + (long)(b * nC + c) * plane * g.nz;
+    float val = 0.f;
+    for (int iz = 0; iz < g.nz; ++iz) {
+        const float h = (float)iz - sz;
+        val += box_avg(vol + iz * plane, g.nx, g.ny, fmaf(rx1, h, sx), fmaf(rx2, h, sx), fmaf(ry1, h, sy),
+                       fmaf(ry2, h, sy));
+    }
+    const float ddx = s0.x - (cc.x + u), ddy = s0.y - (cc.y + v), ddz = s0.z - cc.z;
+    val *= g.dz / fabsf(ddz) * sqrtf(ddz * ddz + ddy * ddy + ddx * ddx);
+    prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c] = val;
+}
+
+// wprj: [slice][nview][nv][nu] pre-weighted; out img reference layout
+__global__ void k_dd_tomo_preweight(ArbGeo a, const float* __restrict__ prj_ref, float* __restrict__ w,
+                                    const float* C, const float* S, int nC)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int view = blockIdx.y;
+    const int bc = blockIdx.z;   // (b*nv + iv)*nC + c
+    const int c = bc % nC, biv = bc / nC, iv = biv % a.nv, b = biv / a.nv;
+    if (iu >= a.nu) return;
+    const F3 cc = ld3(C, view), s0 = ld3(S, view);
+    const float u = (iu - a.off_u - (a.nu - 1) / 2.0f) * a.du;
+    const float v = (iv - a.off_v - (a.nv - 1) / 2.0f) * a.dv;
+    const float ddx = s0.x - (cc.x + u), ddy = s0.y - (cc.y + v), ddz = s0.z - cc.z;
+    // PreweightBPCartKernelXY (DistanceDriven3D.cu:429-430)
+    const float wgt = a.dz / fabsf(ddz) * sqrtf(ddz * ddz + ddy * ddy + ddx * ddx);
+    const float pv = prj_ref[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c];
+    w[((((long)b * nC + c) * a.nview + view) * a.nv + iv) * a.nu + iu] = pv * wgt;
+}
+
+__global__ __launch_bounds__(256) void k_dd_tomo_bp(ArbGeo a, Grid3K g, const float* __restrict__ w,
+                                                    float* __restrict__ img, const float* C, const float* S, int nC)
+{
+    const int iy = blockIdx.x * 16 + threadIdx.x;
+    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int bz = blockIdx.z;   // (b*nC + c)*nz + iz
+    const int iz = bz % g.nz, bc = bz / g.nz, c = bc % nC, b = bc / nC;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float x = (ix - (g.nx - 1) / 2.0f) * g.dx + g.cx;
+    const float y = (iy - (g.ny - 1) / 2.0f) * g.dy + g.cy;
+    const float z = (iz - (g.nz - 1) / 2.0f) * g.dz + g.cz;
+    const float cu = (a.nu - 1) / 2.0f + a.off_u, cv = (a.nv - 1) / 2.0f + a.off_v;
+    const float* wb = w + (long)(b * nC + c) * a.nview * a.nv * a.nu;
+    float val = 0.f;
+    for (int view = 0; view < a.nview; ++view) {
+        const F3 cc = ld3(C, view), s0 = ld3(S, view);
+        // ProjectConeToDetCart (DistanceDriven3D.cu:478-489); its output is a bin-centre
+        // index used directly as an integral-image coordinate (DistanceDriven3D.cu:544-553)
+        const float r = (cc.z - s0.z) / (z - s0.z);
+        float u1 = (s0.x - cc.x + r * (x - g.dx / 2 - s0.x)) / a.du + cu;
+        float u2 = (s0.x - cc.x + r * (x + g.dx / 2 - s0.x)) / a.du + cu;
+        float v1 = (s0.y - cc.y + r * (y - g.dy / 2 - s0.y)) / a.dv + cv;
+        float v2 = (s0.y - cc.y + r * (y + g.dy / 2 - s0.y)) / a.dv + cv;
+        if (u1 > u2) { const float t = u1; u1 = u2; u2 = t; }
+        if (v1 > v2) { const float t = v1; v1 = v2; v2 = t; }
+        val += box_avg(wb + (long)view * a.nv * a.nu, a.nu, a.nv, u1, u2, v1, v2);
+    }
+    img[((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + c] = val;
+}
+
+// ---- cone OS-SQS helpers ----
+
+__global__ void k_cone_residual(const float* __restrict__ fp, const float* __restrict__ prj, float* __restrict__ r,
+                                int nB, int nu, int nview, int cur, int start, int nvc)
+{
+    // fp, r: [b][nu][cur][nv*nC]; prj: [b][nu][nview][nv*nC]
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)nu * cur * nvc;
+    if (i >= per * nB) return;
+    const long b = i / per, rem = i % per;
+    const long iu = rem / ((long)cur * nvc), rem2 = rem % ((long)cur * nvc);
+    const long vi = rem2 / nvc, k = rem2 % nvc;
+    r[i] = fp[i] - prj[((b * nu + iu) * nview + start + vi) * nvc + k];
+}
+
+__global__ void k_cone_update(float* __restrict__ x, const float* __restrict__ bp, const float* __restrict__ D,
+                              const float* __restrict__ mask, float step, float lb, float ub, int crop, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = (x[i] - step * (bp[i] / (D[i] + 1e-6f))) * mask[i];
+    if (crop) v = v < lb ? lb : (v > ub ? ub : v);
+    x[i] = v;
+}
+
+// ---- helical ----
+
+// prj_k: [slice][nview][nv][nu]
+__global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* __restrict__ prj, float* __restrict__ img,
+                                                    int nC)
+{
+    const int iy = blockIdx.x * 16 + threadIdx.x;
+    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int bz = blockIdx.z;   // (b*nC + c)*nz + iz
+    const int iz = bz % h.nz, bc = bz / h.nz, c = bc % nC, b = bc / nC;
+    if (ix >= h.nx || iy >= h.ny) return;
+    const float PI = 3.141592657f;
+    const float x = (ix - (h.nx - 1) / 2.0f) * h.dx, y = (iy - (h.ny - 1) / 2.0f) * h.dy;
+    const float z = iz * h.dz + h.volZ0;
+    const int iStart = (int)((z / h.zrot * 2 * PI - PI / 4) * h.nviewPerPI / PI);
+    const float* pb = prj + (long)(b * nC + c) * h.nview * h.nv * h.nu;
+    const int period = 2 * h.nviewPerPI;     // views per turn
+    const float dth = PI / h.nviewPerPI;
+    float imgVal = 0.f;
+    for (int i = iStart; i < iStart + h.nviewPerPI; ++i) {
+        float totalW = 0.f, val = 0.f;
+        for (int k = i - h.mPI * h.nviewPerPI; k <= i + h.mPI * h.nviewPerPI; k += h.nviewPerPI) {
+            if (k < 0 || k >= h.nview) continue;
+            // theta = theta0 + k pi / nviewPerPI, reduced by whole turns before sincos
+            const int kr = k % period;
+            float st, ct;
+            sincosf(h.theta0 + kr * dth, &st, &ct);
+            const float u = x * st - y * ct;
+            const float turnsz = h.zrot * ((float)k / (float)period);
+            const float v = (z - turnsz + h.zrot * asinf(u / h.dso) / (2 * PI)) * h.dsd /
+                            (sqrtf(h.dso * h.dso - u * u) - (x * ct + y * st));
+            const float q = fabsf(v / h.dv / h.nv * 2.0f);
+            float w;
+            if (q <= h.Q) w = 1.f;
+            else if (q < 1.f) { w = cosf((q - h.Q) / (1 - h.Q) * PI / 2); w *= w; }
+            else w = 0.f;
+            const float pu = u / h.du + (h.nu - 1.0f) / 2.0f + h.off_u;
+            const float pv = v / h.dv + (h.nv - 1.0f) / 2.0f + h.off_v;
+            // bilinear fetch with border 0 (texture at (pv+.5, k+.5, pu+.5))
+            const float fu = floorf(pu), fv = floorf(pv);
+            const int u0 = (int)fu, v0 = (int)fv;
+            const float au = pu - fu, av = pv - fv;
+            const float* pk = pb + (long)k * h.nv * h.nu;
+            float smp = 0.f;
+            for (int dv = 0; dv < 2; ++dv) {
+                const int vv = v0 + dv;
+                if (vv < 0 || vv >= h.nv) continue;
+                const float wv = dv ? av : 1.f - av;
+                for (int du = 0; du < 2; ++du) {
+                    const int uu = u0 + du;
+                    if (uu < 0 || uu >= h.nu) continue;
+                    smp += wv * (du ? au : 1.f - au) * pk[(long)vv * h.nu + uu];
+                }
+            }
+            val += smp * w;
+            totalW += w;
+        }
+        imgVal += val / (totalW + 1e-6f);
+    }
+    img[((((long)b * h.nx + ix) * h.ny + iy) * h.nz + iz) * nC + c] = imgVal;
+}
+
+Grid3K grid_of(const ArbGeo& g) { return Grid3K{g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.cx, g.cy, g.cz}; }
+
+} // namespace
+
+void siddon_arb_fp(const ArbGeo& g, bool parallel, int nB, int nC, const float* img, float* prj,
+                   const float* detCenter, const float* detU, const float* detV, const float* srcOrDir,
+                   DevBuf& work, hipStream_t st)
+{
+    (void)work;
+    RN_REQUIRE(detCenter && detU && detV && srcOrDir, "null geometry array");
+    dim3 blk(256), grd(cdiv(g.nu, 256), g.nview, nB * g.nv * nC);
+    hipLaunchKernelGGL(k_siddon_arb_fp, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, img, prj, detCenter, detU,
+                       detV, srcOrDir, nC);
+    check_launch("siddon_arb_fp");
+}
+
+void siddon_arb_bp(const ArbGeo& g, bool parallel, int nB, int nC, const float* prj, float* img,
+                   const float* detCenter, const float* detU, const float* detV, const float* srcOrDir,
+                   DevBuf& work, hipStream_t st)
+{
+    (void)work;
+    RN_REQUIRE(detCenter && detU && detV && srcOrDir, "null geometry array");
+    const size_t nimg = (size_t)nB * g.nx * g.ny * g.nz * nC;
+    RN_HIP_CHECK(hipMemsetAsync(img, 0, nimg * sizeof(float), st));
+    dim3 blk(256), grd(cdiv(g.nu, 256), g.nview, nB * g.nv * nC);
+    hipLaunchKernelGGL(k_siddon_arb_bp, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, prj, img, detCenter, detU,
+                       detV, srcOrDir, nC);
+    check_launch("siddon_arb_bp");
+}
+
+void dd_tomo_fp(const ArbGeo& g, int nB, int nC, const float* img, float* prj,
+                const float* detCenter, const float* src, DevBuf& work, hipStream_t st)
+{
+    RN_REQUIRE(detCenter && src, "null geometry array");
+    const size_t nvol = (size_t)g.nx * g.ny * g.nz;
+    const int ns = nB * nC;
+    work.ensure(2 * ns * nvol * sizeof(float));
+    float* t1 = work.as<float>();
+    float* t2 = t1 + ns * nvol;
+    // [b][x][y][z][c] -> [(b,c)][x][y][z] -> [(b,c)][z][x][y] -> [(b,c)][z][y][x]
+    const float* base = img;
+    if (nC > 1) {
+        launch_transpose(img, t2, (int)nvol, nC, nB, st);
+        base = t2;
+    }
+    launch_transpose(base, t1, g.nx * g.ny, g.nz, ns, st);
+    launch_transpose(t1, t2, g.nx, g.ny, ns * g.nz, st);
+    dim3 blk(128), grd(cdiv(g.nu, 128), g.nview, nB * g.nv * nC);
+    hipLaunchKernelGGL(k_dd_tomo_fp, grd, blk, 0, st, g, grid_of(g), t2, prj, detCenter, src, nC);
+    check_launch("dd_tomo_fp");
+}
+
+void dd_tomo_bp(const ArbGeo& g, int nB, int nC, const float* prj, float* img,
+                const float* detCenter, const float* src, DevBuf& work, hipStream_t st)
+{
+    RN_REQUIRE(detCenter && src, "null geometry array");
+    const size_t nprj = (size_t)nB * nC * g.nu * g.nview * g.nv;
+    work.ensure(nprj * sizeof(float));
+    dim3 b1(256), g1(cdiv(g.nu, 256), g.nview, nB * g.nv * nC);
+    hipLaunchKernelGGL(k_dd_tomo_preweight, g1, b1, 0, st, g, prj, work.as<float>(), detCenter, src, nC);
+    check_launch("dd_tomo_preweight");
+    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nB * nC * g.nz);
+    hipLaunchKernelGGL(k_dd_tomo_bp, grd, blk, 0, st, g, grid_of(g), work.as<float>(), img, detCenter, src, nC);
+    check_launch("dd_tomo_bp");
+}
+
+void os_sqs_cone(const ArbGeo& g, int nB, int nC, float* x, const float* prj, const float* norm, const float* mask,
+                 const float* detCenter, const float* detU, const float* detV, const float* src,
+                 float lb, float ub, int nviewPerSubset, int nSubsets, float lambda, DevBuf& work, hipStream_t st)
+{
+    const size_t nimg = (size_t)nB * g.nx * g.ny * g.nz * nC;
+    const int nvc = g.nv * nC;
+    const size_t nsub = (size_t)nB * g.nu * nviewPerSubset * nvc;
+    DevBuf fp(nsub * sizeof(float)), r(nsub * sizeof(float)), bp(nimg * sizeof(float));
+    int start = 0;
+    for (int k = 0; k < nSubsets && start < g.nview; k++) {
+        const int cur = std::min(nviewPerSubset, g.nview - start);
+        ArbGeo gs = g;
+        gs.nview = cur;
+        siddon_arb_fp(gs, false, nB, nC, x, fp.as<float>(), detCenter + 3 * start, detU + 3 * start,
+                      detV + 3 * start, src + 3 * start, work, st);
+        const long n = (long)nB * g.nu * cur * nvc;
+        hipLaunchKernelGGL(k_cone_residual, dim3(cdiv(n, 256)), dim3(256), 0, st, fp.as<float>(), prj, r.as<float>(),
+                           nB, g.nu, g.nview, cur, start, nvc);
+        check_launch("cone_residual");
+        siddon_arb_bp(gs, false, nB, nC, r.as<float>(), bp.as<float>(), detCenter + 3 * start, detU + 3 * start,
+                      detV + 3 * start, src + 3 * start, work, st);
+        hipLaunchKernelGGL(k_cone_update, dim3(cdiv((long)nimg, 256)), dim3(256), 0, st, x, bp.as<float>(), norm, mask,
+                           lambda * nSubsets, lb, ub, lb < ub ? 1 : 0, nimg);
+        check_launch("cone_update");
+        start += cur;
+    }
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st)
+{
+    const size_t per = (size_t)h.nu * h.nview * h.nv;
+    work.ensure(2 * (size_t)nB * nC * per * sizeof(float));
+    float* t1 = work.as<float>();
+    float* t2 = t1 + (size_t)nB * nC * per;
+    // [b][u][view][v][c] -> [(b, c)][u][view*v] -> [(b, c)][view][v][u]
+    const float* base = prj;
+    if (nC > 1) {
+        // [b][u*view*v][c] -> [b][c][u*view*v]
+        launch_transpose(prj, t2, (int)per, nC, nB, st);
+        base = t2;
+    }
+    launch_transpose(base, t1, h.nu, h.nview * h.nv, nB * nC, st);
+    dim3 blk(16, 16), grd(cdiv(h.ny, 16), cdiv(h.nx, 16), nB * nC * h.nz);
+    hipLaunchKernelGGL(k_helical_bp, grd, blk, 0, st, h, t1, img, nC);
+    check_launch("helical_bp");
+}
+
+} // namespace rn

@@ -1,0 +1,642 @@
+// 2D fan-beam projector kernels for gfx950 (wave64).
+//
+// Forward projectors are line driven: one lane per detector bin, a loop over
+// the image lines (rows for X views, columns for Y views) that integrates a
+// contiguous image line over the ray's footprint by direct overlap sums.
+// There are no prefix sums (no FP32 cancellation) and no texture hardware.
+// Backprojectors are pixel driven gathers: one lane per pixel, a loop over
+// views, a register accumulator and a single store; no atomics.
+//
+// Reference counterparts: DDFPFanKernelX/Y (DistanceDriven.cu:64-196),
+// DDBPFanKernelX/Y (:500-614), PreweightBPKernelX/Y (:441-477),
+// SiddonFanProjectionKernel / SiddonFanBackprojectionKernel (Siddon.cu:240-382),
+// bpFanKernel3D (PixelDriven.cu:11-75).
+
+#include "rn_fan.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace rn {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// geometry helpers
+// ---------------------------------------------------------------------------
+
+// Ray of detector angle a (sin sa, cos ca) in view t (cos ct, sin st):
+// X views:  X(Y) = A + B (Y - ny/2)   (absolute image coordinates)
+__device__ __forceinline__ void xline(const FanK& g, float ct, float st, float sa, float ca,
+                                      float& A, float& B, float& cu)
+{
+    cu = ca * ct + sa * st;            // cos(a - t)
+    const float su = sa * ct - ca * st;  // sin(a - t)
+    const float inv = 1.0f / cu;
+    A = 0.5f * g.nx + (g.dso / g.dx) * sa * inv;
+    B = (g.dy / g.dx) * su * inv;
+}
+
+// Y views:  Y(X) = A + B (X - nx/2)
+__device__ __forceinline__ void yline(const FanK& g, float ct, float st, float sa, float ca,
+                                      float& A, float& B, float& su)
+{
+    const float cu = ca * ct + sa * st;
+    su = sa * ct - ca * st;
+    const float inv = 1.0f / su;
+    A = 0.5f * g.ny - (g.dso / g.dy) * sa * inv;
+    B = (g.dx / g.dy) * cu * inv;
+}
+
+// Average over [lo, hi] of a piecewise-constant line of n cells (zero outside
+// [0, n)).  For hi == lo the value at lo is returned (a ray exactly along an
+// image line).  Numerator and denominator use the same rounded end points, so
+// a constant line is reproduced exactly.
+template <class T>
+__device__ __forceinline__ T seg_avg(const T* __restrict__ line, int n, float lo, float hi)
+{
+    const float w = hi - lo;
+    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
+    if (!(h > l)) {
+        if (w > 0.0f) return T(0);
+        return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : T(0);
+    }
+    const int m0 = (int)l;
+    const int m1 = min((int)h, n - 1);
+    T s = T(0);
+    for (int m = m0; m <= m1; ++m) {
+        const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
+        s += line[m] * (T)(ph - pl);
+    }
+    return s / (T)w;
+}
+
+// Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).
+__device__ __forceinline__ float seg_weight(int m, float lo, float hi)
+{
+    const float w = hi - lo;
+    const float pl = fmaxf(lo, (float)m), ph = fminf(hi, (float)(m + 1));
+    if (w > 0.0f) return ph > pl ? (ph - pl) / w : 0.0f;
+    return (lo >= (float)m && lo < (float)(m + 1)) ? 1.0f : 0.0f;
+}
+
+// conservative half width (in bins) of the set of rays through a disc of
+// radius r at distance dist from the source:  asin(r/d) <= r/(d-r)
+__device__ __forceinline__ float half_width_bins(float r, float dist, float inv_da)
+{
+    return r * inv_da / fmaxf(dist - r, 1e-3f) + 0.02f;
+}
+
+// ---------------------------------------------------------------------------
+// forward projectors (line driven)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(128) void k_dd_fan_fp(FanK g, const float* __restrict__ RX,
+                                                   const float* __restrict__ RY, float* __restrict__ prj,
+                                                   int view0, int nviews)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int vi = blockIdx.y;
+    const int s = blockIdx.z;
+    if (iu >= g.nu) return;
+    const int view = view0 + vi;
+    const float ct = g.vcos[view], st = g.vsin[view];
+    const long plane = (long)g.nx * g.ny;
+    float acc = 0.0f;
+    float A0, B0, A1, B1, c0, c1;
+    if (g.vcls[view] == 0) {
+        xline(g, ct, st, g.esin[iu], g.ecos[iu], A0, B0, c0);
+        xline(g, ct, st, g.esin[iu + 1], g.ecos[iu + 1], A1, B1, c1);
+        const float* R = RX + s * plane;
+        const float y0 = -0.5f * g.ny;
+        for (int iy = 0; iy < g.ny; ++iy) {
+            const float yr = y0 + (float)iy;
+            const float xa = fmaf(B0, yr, A0), xb = fmaf(B1, yr, A1);
+            acc += seg_avg(R + (long)iy * g.nx, g.nx, fminf(xa, xb), fmaxf(xa, xb));
+        }
+        const float cuc = g.bcos[iu] * ct + g.bsin[iu] * st;
+        acc *= g.dy / fabsf(cuc);
+    } else {
+        yline(g, ct, st, g.esin[iu], g.ecos[iu], A0, B0, c0);
+        yline(g, ct, st, g.esin[iu + 1], g.ecos[iu + 1], A1, B1, c1);
+        const float* R = RY + s * plane;
+        const float x0 = -0.5f * g.nx;
+        for (int ix = 0; ix < g.nx; ++ix) {
+            const float xr = x0 + (float)ix;
+            const float ya = fmaf(B0, xr, A0), yb = fmaf(B1, xr, A1);
+            acc += seg_avg(R + (long)ix * g.ny, g.ny, fminf(ya, yb), fmaxf(ya, yb));
+        }
+        const float suc = g.bsin[iu] * ct - g.bcos[iu] * st;
+        acc *= g.dx / fabsf(suc);
+    }
+    prj[((long)s * nviews + vi) * g.nu + iu] = acc;
+}
+
+// Siddon FP (exact ray / pixel intersection lengths), one line per image row
+// (X views) or column (Y views): the ray crosses row j over Y in [j, j+1],
+// i.e. over X in [X(j), X(j+1)], with path length dy / |cos(a - t)|; the
+// length inside each pixel is that times the pixel's share of the X interval.
+// With meas != nullptr the epilogue writes the weighted residual
+// wgt * (Ax - meas) of the OS-SQS data step (ossqs.cu:101-122) instead of Ax.
+template <class T>
+__global__ __launch_bounds__(128) void k_siddon_fan_fp(FanK g, const T* __restrict__ Pyx,
+                                                       const T* __restrict__ Pxy, T* __restrict__ prj,
+                                                       int view0, int nviews,
+                                                       const float* __restrict__ meas, const float* __restrict__ wgt,
+                                                       int meas_nviews)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int vi = blockIdx.y;
+    const int s = blockIdx.z;
+    if (iu >= g.nu) return;
+    const int view = view0 + vi;
+    const float ct = g.vcos[view], st = g.vsin[view];
+    const long plane = (long)g.nx * g.ny;
+    T acc = T(0);
+    float A, B, c;
+    if (g.vcls[view] == 0) {
+        xline(g, ct, st, g.bsin[iu], g.bcos[iu], A, B, c);
+        const T* P = Pyx + s * plane;
+        const float y0 = -0.5f * g.ny;
+        for (int j = 0; j < g.ny; ++j) {
+            const float yr = y0 + (float)j;
+            const float xa = fmaf(B, yr, A), xb = fmaf(B, yr + 1.0f, A);
+            acc += seg_avg<T>(P + (long)j * g.nx, g.nx, fminf(xa, xb), fmaxf(xa, xb));
+        }
+        acc *= (T)(g.dy / fabsf(c));
+    } else {
+        yline(g, ct, st, g.bsin[iu], g.bcos[iu], A, B, c);
+        const T* P = Pxy + s * plane;
+        const float x0 = -0.5f * g.nx;
+        for (int i = 0; i < g.nx; ++i) {
+            const float xr = x0 + (float)i;
+            const float ya = fmaf(B, xr, A), yb = fmaf(B, xr + 1.0f, A);
+            acc += seg_avg<T>(P + (long)i * g.ny, g.ny, fminf(ya, yb), fmaxf(ya, yb));
+        }
+        acc *= (T)(g.dx / fabsf(c));
+    }
+    if (meas != nullptr) {
+        const long mi = ((long)s * meas_nviews + view) * g.nu + iu;
+        acc = (T)wgt[mi] * (acc - (T)meas[mi]);
+    }
+    prj[((long)s * nviews + vi) * g.nu + iu] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// backprojectors (pixel driven gathers)
+// ---------------------------------------------------------------------------
+
+// Exact transpose of k_siddon_fan_fp.
+template <class T>
+__global__ __launch_bounds__(256) void k_siddon_fan_bp(FanK g, const T* __restrict__ prj,
+                                                       T* __restrict__ img, int view0, int nviews, int accumulate)
+{
+    const int iy = blockIdx.x * 16 + threadIdx.x;
+    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int s = blockIdx.z;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float xc = (ix - 0.5f * (g.nx - 1)) * g.dx, yc = (iy - 0.5f * (g.ny - 1)) * g.dy;
+    const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
+    const float inv_da = 1.0f / g.da;
+    const float yr = (float)iy - 0.5f * g.ny;
+    const float xr = (float)ix - 0.5f * g.nx;
+    T acc = T(0);
+    for (int vi = 0; vi < nviews; ++vi) {
+        const int view = view0 + vi;
+        const float ct = g.vcos[view], st = g.vsin[view];
+        const float rx = xc * ct + yc * st, ry = -xc * st + yc * ct;
+        const float d0 = ry + g.dso;
+        const float uc = g.cu - g.off_a - atanf(rx / d0) * inv_da;
+        const float hw = half_width_bins(rpix, sqrtf(rx * rx + d0 * d0), inv_da);
+        const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
+        const T* p = prj + ((long)s * nviews + vi) * g.nu;
+        if (g.vcls[view] == 0) {
+            for (int k = k0; k <= k1; ++k) {
+                float A, B, c;
+                xline(g, ct, st, g.bsin[k], g.bcos[k], A, B, c);
+                const float xa = fmaf(B, yr, A), xb = fmaf(B, yr + 1.0f, A);
+                const float wgt = seg_weight(ix, fminf(xa, xb), fmaxf(xa, xb));
+                acc += p[k] * (T)(g.dy / fabsf(c) * wgt);
+            }
+        } else {
+            for (int k = k0; k <= k1; ++k) {
+                float A, B, c;
+                yline(g, ct, st, g.bsin[k], g.bcos[k], A, B, c);
+                const float ya = fmaf(B, xr, A), yb = fmaf(B, xr + 1.0f, A);
+                const float wgt = seg_weight(iy, fminf(ya, yb), fmaxf(ya, yb));
+                acc += p[k] * (T)(g.dx / fabsf(c) * wgt);
+            }
+        }
+    }
+    T* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
+    *o = accumulate ? *o + acc : acc;
+}
+
+// Exact transpose of k_dd_fan_fp (no reference counterpart: the reference DD
+// BP is not the adjoint of its FP).
+__global__ __launch_bounds__(256) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
+                                                    float* __restrict__ img, int view0, int nviews, int accumulate)
+{
+    const int iy = blockIdx.x * 16 + threadIdx.x;
+    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int s = blockIdx.z;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float inv_da = 1.0f / g.da;
+    float acc = 0.0f;
+    for (int vi = 0; vi < nviews; ++vi) {
+        const int view = view0 + vi;
+        const float ct = g.vcos[view], st = g.vsin[view];
+        const float* p = prj + ((long)s * nviews + vi) * g.nu;
+        const bool isx = g.vcls[view] == 0;
+        // the pixel enters line (iy or ix) and line (iy+1 or ix+1) with weight 1/2
+        const int l0 = isx ? iy : ix;
+        const int nl = isx ? g.ny : g.nx;
+        const int cell = isx ? ix : iy;
+        for (int l = l0; l <= l0 + 1 && l < nl; ++l) {
+            // centre of the pixel's segment on this line, physical coordinates
+            float px, py, r;
+            if (isx) { px = (ix + 0.5f - 0.5f * g.nx) * g.dx; py = (l - 0.5f * g.ny) * g.dy; r = 0.5f * g.dx; }
+            else     { px = (l - 0.5f * g.nx) * g.dx; py = (iy + 0.5f - 0.5f * g.ny) * g.dy; r = 0.5f * g.dy; }
+            const float rx = px * ct + py * st, ry = -px * st + py * ct;
+            const float d0 = ry + g.dso;
+            const float uc = g.cu - g.off_a - atanf(rx / d0) * inv_da;
+            // bins whose footprint (centre +- half a bin) meets the segment
+            const float hw = half_width_bins(r, sqrtf(rx * rx + d0 * d0), inv_da) + 0.5f;
+            const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
+            if (k1 < k0) continue;
+            const float lr = isx ? (float)l - 0.5f * g.ny : (float)l - 0.5f * g.nx;
+            float A, B, c;
+            float e_prev;
+            if (isx) { xline(g, ct, st, g.esin[k0], g.ecos[k0], A, B, c); }
+            else     { yline(g, ct, st, g.esin[k0], g.ecos[k0], A, B, c); }
+            e_prev = fmaf(B, lr, A);
+            for (int k = k0; k <= k1; ++k) {
+                if (isx) xline(g, ct, st, g.esin[k + 1], g.ecos[k + 1], A, B, c);
+                else     yline(g, ct, st, g.esin[k + 1], g.ecos[k + 1], A, B, c);
+                const float e_next = fmaf(B, lr, A);
+                const float wgt = seg_weight(cell, fminf(e_prev, e_next), fmaxf(e_prev, e_next));
+                float L;
+                if (isx) L = g.dy / fabsf(g.bcos[k] * ct + g.bsin[k] * st);
+                else     L = g.dx / fabsf(g.bsin[k] * ct - g.bcos[k] * st);
+                acc = fmaf(0.5f * L * p[k], wgt, acc);
+                e_prev = e_next;
+            }
+        }
+    }
+    float* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
+    *o = accumulate ? *o + acc : acc;
+}
+
+// PreweightBPKernelX/Y: length factor for the iterative (non-FBP) DD BP.
+__global__ void k_dd_preweight(FanK g, const float* __restrict__ prj, float* __restrict__ wprj)
+{
+    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
+    const int view = blockIdx.y;
+    const int s = blockIdx.z;
+    if (iu >= g.nu) return;
+    const float ct = g.vcos[view], st = g.vsin[view];
+    float f;
+    if (g.vcls[view] == 0) f = g.dy / fabsf(ct * g.bcos[iu] + st * g.bsin[iu]);   // cos(t - a)
+    else f = g.dx / fabsf(st * g.bcos[iu] - ct * g.bsin[iu]);                      // sin(t - a)
+    const long i = ((long)s * g.nview + view) * g.nu + iu;
+    wprj[i] = prj[i] * f;
+}
+
+// Reference-parity DD BP (DDBPFanKernelX/Y): pixel edges projected through
+// atan, average of the (pre-weighted) detector row over the footprint.
+__global__ __launch_bounds__(256) void k_dd_fan_bp(FanK g, const float* __restrict__ wprj,
+                                                   float* __restrict__ img, int fbp)
+{
+    const int iy = blockIdx.x * 16 + threadIdx.x;
+    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int s = blockIdx.z;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
+    const float inv_da = 1.0f / g.da;
+    const float boff = 0.5f * g.nu - g.off_a;
+    float acc = 0.0f;
+    for (int view = 0; view < g.nview; ++view) {
+        const float ct = g.vcos[view], st = g.vsin[view];
+        const float rx = x * ct + y * st, ry = -x * st + y * ct;
+        float ex, ey;   // edge offset (rotated frame) of +-half a pixel
+        if (g.vcls[view] == 0) { ex = 0.5f * g.dx * ct; ey = -0.5f * g.dx * st; }
+        else                   { ex = 0.5f * g.dy * st; ey = 0.5f * g.dy * ct; }
+        const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
+        const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
+        float val = seg_avg(wprj + ((long)s * g.nview + view) * g.nu, g.nu, fminf(b1, b2), fmaxf(b1, b2));
+        if (fbp) {
+            const float d = g.dso + ry;
+            val *= g.dso * g.dso / (rx * rx + d * d);
+        }
+        acc += val;
+    }
+    img[(long)s * g.nx * g.ny + (long)ix * g.ny + iy] = acc;
+}
+
+// bpFanKernel3D with isFBP = true (the only mode the reference launches).
+__global__ __launch_bounds__(256) void k_pixel_fan_bp(FanK g, const float* __restrict__ prj,
+                                                      float* __restrict__ img)
+{
+    const int iy = blockIdx.x * 16 + threadIdx.x;
+    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int s = blockIdx.z;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
+    const float inv_da = 1.0f / g.da;
+    float acc = 0.0f;
+    for (int view = 0; view < g.nview; ++view) {
+        const float ct = g.vcos[view], st = g.vsin[view];
+        const float rx = x * ct + y * st, ry = -x * st + y * ct;
+        const float d = ry + g.dso;
+        const float pu = g.cu - g.off_a - atanf(rx / d) * inv_da;
+        const float f0 = floorf(pu);
+        const int i0 = (int)f0;
+        const float f = pu - f0;
+        const float* p = prj + ((long)s * g.nview + view) * g.nu;
+        const float v0 = (i0 >= 0 && i0 < g.nu) ? p[i0] : 0.0f;
+        const float v1 = (i0 + 1 >= 0 && i0 + 1 < g.nu) ? p[i0 + 1] : 0.0f;
+        acc += ((1.0f - f) * v0 + f * v1) * (g.dso * g.dso / (rx * rx + d * d));
+    }
+    img[(long)s * g.nx * g.ny + (long)ix * g.ny + iy] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// layout helpers
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, float* __restrict__ out,
+                                                   int rows, int cols)
+{
+    __shared__ float tile[32][33];
+    const long base = (long)blockIdx.z * rows * cols;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[k][threadIdx.x] = in[base + (long)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[base + (long)c * rows + r] = tile[threadIdx.x][k];
+    }
+}
+
+__global__ void k_avg_prev(const float* __restrict__ in, float* __restrict__ out, int n0, int n1, int axis)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = (long)n0 * n1;
+    const long base = (long)blockIdx.y * per;
+    if (i >= per) return;
+    const int a = (int)(i / n1), b = (int)(i % n1);
+    const float cur = in[base + i];
+    float prev = 0.0f;
+    if (axis == 0) { if (a > 0) prev = in[base + i - n1]; }
+    else           { if (b > 0) prev = in[base + i - 1]; }
+    out[base + i] = 0.5f * (prev + cur);
+}
+
+__global__ void k_mix_gather(const float* __restrict__ src, float* __restrict__ dst, const float* __restrict__ M,
+                             int n, int np, int nc, int nout)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int sd = blockIdx.y;  // (b*nout + o)*nc + c
+    if (i >= n) return;
+    const int c = sd % nc, bo = sd / nc, o = bo % nout, b = bo / nout;
+    float v = 0.0f;
+    const float* sp = src + (((long)b * n + i) * np) * nc + c;
+    for (int p = 0; p < np; ++p) {
+        const float m = M[o * np + p];
+        if (m != 0.0f) v = fmaf(m, sp[(long)p * nc], v);
+    }
+    dst[(long)sd * n + i] = v;
+}
+
+__global__ void k_mix_scatter(const float* __restrict__ src, float* __restrict__ dst, const float* __restrict__ M,
+                              int n, int np, int nc, int nout, int accumulate)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int dd = blockIdx.y;  // (b*nout + o)*nc + c
+    if (i >= n) return;
+    const int c = dd % nc, bo = dd / nc, o = bo % nout, b = bo / nout;
+    float v = 0.0f;
+    for (int p = 0; p < np; ++p) {
+        const float m = M[o * np + p];
+        if (m != 0.0f) v = fmaf(m, src[((long)(b * np + p) * nc + c) * n + i], v);
+    }
+    float* d = dst + (((long)b * n + i) * nout + o) * nc + c;
+    *d = accumulate ? *d + v : v;
+}
+
+__global__ void k_fill(float* __restrict__ p, float v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_axpy(float* __restrict__ y, const float* __restrict__ x, float a, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = fmaf(a, x[i], y[i]);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+void FanTables::build(const FanGeo& g, const float* deg_host, hipStream_t st)
+{
+    RN_REQUIRE(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.nu > 0 && g.nview > 0 && g.nv > 0, "fan dimensions must be positive");
+    RN_REQUIRE(g.dx > 0 && g.dy > 0 && g.dz > 0 && g.da != 0 && g.dv > 0 && g.dso > 0, "fan spacings must be positive");
+    RN_REQUIRE(deg_host != nullptr, "angle array is null");
+    geo = g;
+    deg.assign(deg_host, deg_host + g.nview);
+    cls.resize(g.nview);
+    const int nu = g.nu, nview = g.nview;
+    // layout: vcos[nview] vsin[nview] vcls[nview] esin[nu+1] ecos[nu+1] bsin[nu] bcos[nu]
+    const size_t nf = 3 * (size_t)nview + 2 * (size_t)(nu + 1) + 2 * (size_t)nu;
+    host_.assign(nf, 0.0f);
+    float* vcos = host_.data();
+    float* vsin = vcos + nview;
+    int* vcls = reinterpret_cast<int*>(vsin + nview);
+    float* esin = vsin + 2 * nview;
+    float* ecos = esin + (nu + 1);
+    float* bsin = ecos + (nu + 1);
+    float* bcos = bsin + nu;
+    for (int v = 0; v < nview; v++) {
+        const float d = deg_host[v];
+        vcos[v] = (float)std::cos((double)d);
+        vsin[v] = (float)std::sin((double)d);
+        // the float32 host decision of DistanceDriven.cu:302 (glibc cosf/sinf)
+        cls[v] = std::fabs(cosf(d)) > std::fabs(sinf(d)) ? 0 : 1;
+        vcls[v] = cls[v];
+    }
+    const double da = (double)g.da, cu = (nu - 1) / 2.0, off = (double)g.off_a;
+    for (int e = 0; e <= nu; e++) {
+        const double a = (-(e - cu) - off) * da + da / 2.0;
+        esin[e] = (float)std::sin(a);
+        ecos[e] = (float)std::cos(a);
+    }
+    for (int k = 0; k < nu; k++) {
+        const double a = (-(k - cu) - off) * da;
+        bsin[k] = (float)std::sin(a);
+        bcos[k] = (float)std::cos(a);
+    }
+    buf_.alloc(nf * sizeof(float));
+    RN_HIP_CHECK(hipMemcpyAsync(buf_.get(), host_.data(), nf * sizeof(float), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+    float* d = buf_.as<float>();
+    k.nx = g.nx; k.ny = g.ny; k.nu = nu; k.nview = nview;
+    k.dx = g.dx; k.dy = g.dy; k.da = g.da; k.off_a = g.off_a; k.dso = g.dso; k.dsd = g.dsd;
+    k.cu = 0.5f * (nu - 1);
+    k.vcos = d; k.vsin = d + nview; k.vcls = reinterpret_cast<const int*>(d + 2 * nview);
+    k.esin = d + 3 * nview; k.ecos = k.esin + (nu + 1);
+    k.bsin = k.ecos + (nu + 1); k.bcos = k.bsin + nu;
+}
+
+bool FanTables::matches(const FanGeo& g, const float* deg_host) const
+{
+    if (buf_.get() == nullptr) return false;
+    if (std::memcmp(&g, &geo, sizeof(FanGeo)) != 0) return false;
+    return std::memcmp(deg.data(), deg_host, sizeof(float) * g.nview) == 0;
+}
+
+void launch_dd_fan_fp(const FanK& g, const float* RX, const float* RY, float* prj_vu,
+                      int nslices, int view0, int nviews, hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
+    hipLaunchKernelGGL(k_dd_fan_fp, grd, blk, 0, st, g, RX, RY, prj_vu, view0, nviews);
+    check_launch("dd_fan_fp");
+}
+
+void launch_siddon_fan_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj_vu,
+                          int nslices, int view0, int nviews, hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
+    hipLaunchKernelGGL(k_siddon_fan_fp<float>, grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
+                       (const float*)nullptr, (const float*)nullptr, 0);
+    check_launch("siddon_fan_fp");
+}
+
+void launch_siddon_fan_fp_residual(const FanK& g, const float* Pyx, const float* Pxy, float* res_vu,
+                                   int nslices, int view0, int nviews, const float* meas_vu, const float* w_vu,
+                                   hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
+    hipLaunchKernelGGL(k_siddon_fan_fp<float>, grd, blk, 0, st, g, Pyx, Pxy, res_vu, view0, nviews,
+                       meas_vu, w_vu, g.nview);
+    check_launch("siddon_fan_fp_residual");
+}
+
+void launch_siddon_fan_fp_f64(const FanK& g, const double* Pyx, const double* Pxy, double* prj_vu,
+                              int nslices, int view0, int nviews, hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
+    hipLaunchKernelGGL(k_siddon_fan_fp<double>, grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
+                       (const float*)nullptr, (const float*)nullptr, 0);
+    check_launch("siddon_fan_fp_f64");
+}
+
+void launch_siddon_fan_bp_f64(const FanK& g, const double* prj_vu, double* img_xy, int nslices,
+                              int view0, int nviews, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
+    hipLaunchKernelGGL(k_siddon_fan_bp<double>, grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, 0);
+    check_launch("siddon_fan_bp_f64");
+}
+
+void launch_siddon_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int nslices,
+                          int view0, int nviews, bool accumulate, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
+    hipLaunchKernelGGL(k_siddon_fan_bp<float>, grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, accumulate ? 1 : 0);
+    check_launch("siddon_fan_bp");
+}
+
+void launch_dd_fan_adj(const FanK& g, const float* prj_vu, float* img_xy, int nslices,
+                       int view0, int nviews, bool accumulate, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
+    hipLaunchKernelGGL(k_dd_fan_adj, grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, accumulate ? 1 : 0);
+    check_launch("dd_fan_adj");
+}
+
+void launch_dd_preweight(const FanK& g, const float* prj_vu, float* wprj_vu, int nslices, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    dim3 blk(256), grd(cdiv(g.nu, 256), g.nview, nslices);
+    hipLaunchKernelGGL(k_dd_preweight, grd, blk, 0, st, g, prj_vu, wprj_vu);
+    check_launch("dd_preweight");
+}
+
+void launch_dd_fan_bp(const FanK& g, const float* wprj_vu, float* img_xy, int nslices, bool fbp, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
+    hipLaunchKernelGGL(k_dd_fan_bp, grd, blk, 0, st, g, wprj_vu, img_xy, fbp ? 1 : 0);
+    check_launch("dd_fan_bp");
+}
+
+void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int nslices, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
+    hipLaunchKernelGGL(k_pixel_fan_bp, grd, blk, 0, st, g, prj_vu, img_xy);
+    check_launch("pixel_fan_bp");
+}
+
+void launch_transpose(const float* in, float* out, int rows, int cols, int batch, hipStream_t st)
+{
+    if (batch <= 0 || rows <= 0 || cols <= 0) return;
+    dim3 blk(32, 8), grd(cdiv(cols, 32), cdiv(rows, 32), batch);
+    hipLaunchKernelGGL(k_transpose, grd, blk, 0, st, in, out, rows, cols);
+    check_launch("transpose");
+}
+
+void launch_avg_prev(const float* in, float* out, int n0, int n1, int batch, int axis, hipStream_t st)
+{
+    if (batch <= 0) return;
+    dim3 blk(256), grd(cdiv((long)n0 * n1, 256), batch);
+    hipLaunchKernelGGL(k_avg_prev, grd, blk, 0, st, in, out, n0, n1, axis);
+    check_launch("avg_prev");
+}
+
+void launch_mix_gather(const float* src, float* dst, const float* M, int nB, int n, int np, int nc, int nout,
+                       hipStream_t st)
+{
+    dim3 blk(256), grd(cdiv(n, 256), nB * nout * nc);
+    hipLaunchKernelGGL(k_mix_gather, grd, blk, 0, st, src, dst, M, n, np, nc, nout);
+    check_launch("mix_gather");
+}
+
+void launch_mix_scatter(const float* src, float* dst, const float* M, int nB, int n, int np, int nc, int nout,
+                        bool accumulate, hipStream_t st)
+{
+    dim3 blk(256), grd(cdiv(n, 256), nB * nout * nc);
+    hipLaunchKernelGGL(k_mix_scatter, grd, blk, 0, st, src, dst, M, n, np, nc, nout, accumulate ? 1 : 0);
+    check_launch("mix_scatter");
+}
+
+void launch_fill(float* p, float v, size_t n, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill, dim3(cdiv((long)n, 256)), dim3(256), 0, st, p, v, n);
+    check_launch("fill");
+}
+
+void launch_axpy(float* y, const float* x, float a, size_t n, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_axpy, dim3(cdiv((long)n, 256)), dim3(256), 0, st, y, x, a, n);
+    check_launch("axpy");
+}
+
+} // namespace rn

@@ -1,0 +1,53 @@
+// OS-SQS data step (ossqs.cu), TV surrogate (TotalVariation.cu) and the
+// combined SQS-TV iteration, device resident.
+#pragma once
+
+#include "rn_fan.h"
+
+#include <vector>
+
+namespace rn {
+
+// TVSQSKernel over nslices 2D images [s][nx][ny]: grad, norm (FP32) and the
+// per-slice sums of var reduced in FP64 (host vector, length nslices).
+void tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
+            std::vector<double>* tv, DevBuf& scratch, hipStream_t st);
+
+struct SqsStep {
+    float lambda = 1.0f;    // step scale of the data term (ossqs.cu:139)
+    float beta = 0.0f;      // TV weight (0 = data term only, as cOSSQS)
+    float eps = 1e-6f;      // TV smoothing delta^2 (TotalVariation.cu:59)
+    float lb = -3.4028235e38f, ub = 3.4028235e38f;
+};
+
+// Device-resident OS-SQS state for a batch of 2D fan slices sharing one
+// geometry.  Images are [s][nx][ny]; sinograms are held view-major [s][nview][nu].
+class OsSqs {
+public:
+    void setup(const FanGeo& g, const float* deg_host, int nslices, int nviewPerSubset, int nSubsets,
+               hipStream_t st);
+    // measured sinogram and weights in reference layout [s][nu][nview] (device); w may be null (= 1)
+    void set_data(const float* prj_ref, const float* w_ref, hipStream_t st);
+    // D = A'(w A 1) with FP64 accumulation (NormImg of ReconNet_no_tf.py:359-394)
+    void compute_norm(hipStream_t st);
+    void set_norm(const float* norm_xy, hipStream_t st);
+    const float* norm() const { return D_.as<float>(); }
+    void set_image(const float* x_xy, hipStream_t st);
+    void get_image(float* x_xy, hipStream_t st) const;
+    float* image() { return X_[cur_].as<float>(); }
+    // one pass over all subsets; if track != nullptr, per-slice FP64 sums of
+    // (x_new - x_old)^2 and x_new^2 over the pass are returned in it (2*nslices)
+    void pass(const SqsStep& o, hipStream_t st, std::vector<double>* track = nullptr);
+    // refresh the transposed copy after the image was changed externally
+    void sync_transposed(hipStream_t st);
+    const FanTables& tables() const { return t_; }
+    int nslices() const { return ns_; }
+private:
+    FanTables t_;
+    int ns_ = 0, npers_ = 0, nsub_ = 0;
+    DevBuf B_, W_, D_, X_[2], Xyx_, R_, G_, red_;
+    int cur_ = 0;
+    bool have_w_ = false;
+};
+
+} // namespace rn

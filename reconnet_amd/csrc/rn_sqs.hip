@@ -1,0 +1,299 @@
+// OS-SQS with the Siddon fan projector (ossqs.cu:12-161) and the TV surrogate
+// (TotalVariation.cu:9-142), device resident.
+//
+// One subset step is three launches over the whole slice batch:
+//   1. Siddon FP of the subset views with the residual w (Ax - b) fused into
+//      the epilogue (replaces Minus2D + Multiply2D, ossqs.cu:105-122);
+//   2. pixel-driven gather BP of the residual (no atomics);
+//   3. a fused update: TV gradient / curvature stencil, the SQS step
+//      x - (lambda M g + 2 beta gTV) / (D + 2 beta nTV + 1e-6), the crop to
+//      [lb, ub], and the transposed copy that the next FP's X views read.
+// The normaliser D = A'(w A 1) is accumulated in FP64; so are the TV cost and
+// the convergence sums.
+
+#include "rn_sqs.h"
+
+#include <cstring>
+
+namespace rn {
+
+namespace {
+
+constexpr int TB = 16;   // 16 x 16 pixel tiles
+
+__device__ __forceinline__ void tv_terms(const float* __restrict__ X, int ix, int iy, int nx, int ny, float eps,
+                                         float& gtv, float& ntv, float& var)
+{
+    const long ind = (long)ix * ny + iy;
+    const float x = X[ind];
+    const float dx0 = ix == 0 ? x : x - X[ind - ny];
+    const float dx1 = ix == nx - 1 ? x : x - X[ind + ny];
+    const float dy0 = iy == 0 ? x : x - X[ind - 1];
+    const float dy1 = iy == ny - 1 ? x : x - X[ind + 1];
+    const float t0 = sqrtf(dx0 * dx0 + eps), t1 = sqrtf(dx1 * dx1 + eps);
+    const float t2 = sqrtf(dy0 * dy0 + eps), t3 = sqrtf(dy1 * dy1 + eps);
+    gtv = (dx0 / t0 + dx1 / t1 + dy0 / t2 + dy1 / t3) / 2;
+    ntv = 1 / t0 + 1 / t1 + 1 / t2 + 1 / t3;
+    var = (t0 + t1 + t2 + t3) / 2;
+}
+
+// block-wide FP64 sum of up to two values; thread 0 writes the partials
+__device__ __forceinline__ void block_sum2(double a, double b, double* out)
+{
+    __shared__ double sa[TB * TB], sb[TB * TB];
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    sa[t] = a; sb[t] = b;
+    __syncthreads();
+    for (int k = (blockDim.x * blockDim.y) / 2; k > 0; k >>= 1) {
+        if (t < k) { sa[t] += sa[t + k]; sb[t] += sb[t + k]; }
+        __syncthreads();
+    }
+    if (t == 0) { out[0] = sa[0]; out[1] = sb[0]; }
+}
+
+__global__ __launch_bounds__(256) void k_tv_sqs(const float* __restrict__ img, float* __restrict__ grad,
+                                                float* __restrict__ norm, int nx, int ny, float eps,
+                                                double* __restrict__ partial)
+{
+    const int iy = blockIdx.x * TB + threadIdx.x;
+    const int ix = blockIdx.y * TB + threadIdx.y;
+    const int s = blockIdx.z;
+    const long plane = (long)nx * ny;
+    double v = 0.0;
+    if (ix < nx && iy < ny) {
+        float g, n, var;
+        tv_terms(img + s * plane, ix, iy, nx, ny, eps, g, n, var);
+        grad[s * plane + (long)ix * ny + iy] = g;
+        norm[s * plane + (long)ix * ny + iy] = n;
+        v = (double)var;
+    }
+    const long nb = (long)gridDim.x * gridDim.y;
+    block_sum2(v, 0.0, partial + 2 * (s * nb + (long)blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+__global__ __launch_bounds__(256) void k_sqs_update(const float* __restrict__ X, float* __restrict__ Xn,
+                                                    const float* __restrict__ G, const float* __restrict__ D,
+                                                    int nx, int ny, float step, float beta2, float eps,
+                                                    float lb, float ub, int crop)
+{
+    const int iy = blockIdx.x * TB + threadIdx.x;
+    const int ix = blockIdx.y * TB + threadIdx.y;
+    const int s = blockIdx.z;
+    const long plane = (long)nx * ny;
+    if (ix >= nx || iy >= ny) return;
+    const long i = s * plane + (long)ix * ny + iy;
+    const float x = X[i];
+    float num = step * G[i];
+    float den = D[i];
+    if (beta2 != 0.0f) {
+        float g, n, var;
+        tv_terms(X + s * plane, ix, iy, nx, ny, eps, g, n, var);
+        num = fmaf(beta2, g, num);
+        den = fmaf(beta2, n, den);
+    }
+    float xn = x - num / (den + 1e-6f);
+    if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
+    Xn[i] = xn;
+}
+
+// per-block FP64 partials of sum (b - a)^2 and sum b^2
+__global__ __launch_bounds__(256) void k_diff_sums(const float* __restrict__ a, const float* __restrict__ b,
+                                                   int nx, int ny, double* __restrict__ partial)
+{
+    const int iy = blockIdx.x * TB + threadIdx.x;
+    const int ix = blockIdx.y * TB + threadIdx.y;
+    const int s = blockIdx.z;
+    const long plane = (long)nx * ny;
+    double dd = 0.0, bb = 0.0;
+    if (ix < nx && iy < ny) {
+        const long i = s * plane + (long)ix * ny + iy;
+        const double d = (double)b[i] - (double)a[i];
+        dd = d * d;
+        bb = (double)b[i] * (double)b[i];
+    }
+    const long nb = (long)gridDim.x * gridDim.y;
+    block_sum2(dd, bb, partial + 2 * (s * nb + (long)blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+// sum the per-block partials of each slice (FP64), accumulate into out[2*s..]
+__global__ void k_reduce_partials(const double* __restrict__ partial, long nb, double* __restrict__ out, int accumulate)
+{
+    __shared__ double sa[256], sb[256];
+    const int s = blockIdx.x;
+    double a = 0, b = 0;
+    for (long i = threadIdx.x; i < nb; i += blockDim.x) {
+        a += partial[2 * (s * nb + i)];
+        b += partial[2 * (s * nb + i) + 1];
+    }
+    sa[threadIdx.x] = a; sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) { sa[threadIdx.x] += sa[threadIdx.x + k]; sb[threadIdx.x] += sb[threadIdx.x + k]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (accumulate) { out[2 * s] += sa[0]; out[2 * s + 1] += sb[0]; }
+        else { out[2 * s] = sa[0]; out[2 * s + 1] = sb[0]; }
+    }
+}
+
+__global__ void k_fill_f64(double* p, double v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_weight_rows_f64(const double* __restrict__ fp1, const float* __restrict__ w, double* __restrict__ y,
+                                  size_t per_slice)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s = blockIdx.y;
+    if (i < per_slice) y[s * per_slice + i] = fp1[i] * (double)w[s * per_slice + i];
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+} // namespace
+
+void tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
+            std::vector<double>* tv, DevBuf& scratch, hipStream_t st)
+{
+    RN_REQUIRE(nx > 0 && ny > 0 && nslices > 0, "TV dimensions must be positive");
+    dim3 blk(TB, TB), grd(cdiv(ny, TB), cdiv(nx, TB), nslices);
+    const long nb = (long)grd.x * grd.y;
+    scratch.ensure((2 * nb * nslices + 2 * (size_t)nslices) * sizeof(double));
+    double* partial = scratch.as<double>();
+    double* out = partial + 2 * nb * nslices;
+    hipLaunchKernelGGL(k_tv_sqs, grd, blk, 0, st, img, grad, norm, nx, ny, eps, partial);
+    check_launch("tv_sqs");
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nslices), dim3(256), 0, st, partial, nb, out, 0);
+    check_launch("reduce_partials");
+    if (tv) {
+        std::vector<double> h(2 * (size_t)nslices);
+        RN_HIP_CHECK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        RN_HIP_CHECK(hipStreamSynchronize(st));
+        tv->resize(nslices);
+        for (int s = 0; s < nslices; s++) (*tv)[s] = h[2 * s];
+    }
+}
+
+void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nviewPerSubset, int nSubsets,
+                  hipStream_t st)
+{
+    RN_REQUIRE(nslices > 0, "number of slices must be positive");
+    RN_REQUIRE(nviewPerSubset > 0 && nSubsets > 0, "subset sizes must be positive");
+    RN_REQUIRE(g.nz == 1 && g.nv == 1, "OS-SQS engine handles 2D slices (nz = nv = 1)");
+    if (!t_.matches(g, deg_host)) t_.build(g, deg_host, st);
+    ns_ = nslices; npers_ = nviewPerSubset; nsub_ = nSubsets;
+    const size_t plane = (size_t)g.nx * g.ny, sino = (size_t)g.nu * g.nview;
+    B_.ensure(ns_ * sino * sizeof(float));
+    W_.ensure(ns_ * sino * sizeof(float));
+    D_.ensure(ns_ * plane * sizeof(float));
+    X_[0].ensure(ns_ * plane * sizeof(float));
+    X_[1].ensure(ns_ * plane * sizeof(float));
+    Xyx_.ensure(ns_ * plane * sizeof(float));
+    R_.ensure(ns_ * (size_t)g.nu * npers_ * sizeof(float));
+    G_.ensure(ns_ * plane * sizeof(float));
+}
+
+void OsSqs::set_data(const float* prj_ref, const float* w_ref, hipStream_t st)
+{
+    const FanGeo& g = t_.geo;
+    launch_transpose(prj_ref, B_.as<float>(), g.nu, g.nview, ns_, st);
+    if (w_ref) launch_transpose(w_ref, W_.as<float>(), g.nu, g.nview, ns_, st);
+    else launch_fill(W_.as<float>(), 1.0f, (size_t)ns_ * g.nu * g.nview, st);
+    have_w_ = true;
+}
+
+void OsSqs::compute_norm(hipStream_t st)
+{
+    const FanGeo& g = t_.geo;
+    const size_t plane = (size_t)g.nx * g.ny, sino = (size_t)g.nu * g.nview;
+    // FP64 scratch: ones image (1 slice), its projection, the weighted
+    // projections of every slice and the FP64 normaliser
+    DevBuf ones(plane * sizeof(double)), fp1(sino * sizeof(double));
+    DevBuf y((size_t)ns_ * sino * sizeof(double)), d64((size_t)ns_ * plane * sizeof(double));
+    hipLaunchKernelGGL(k_fill_f64, dim3(cdiv((long)plane, 256)), dim3(256), 0, st, ones.as<double>(), 1.0, plane);
+    check_launch("fill_f64");
+    launch_siddon_fan_fp_f64(t_.k, ones.as<double>(), ones.as<double>(), fp1.as<double>(), 1, 0, g.nview, st);
+    hipLaunchKernelGGL(k_weight_rows_f64, dim3(cdiv((long)sino, 256), ns_), dim3(256), 0, st,
+                       fp1.as<double>(), W_.as<float>(), y.as<double>(), sino);
+    check_launch("weight_rows_f64");
+    launch_siddon_fan_bp_f64(t_.k, y.as<double>(), d64.as<double>(), ns_, 0, g.nview, st);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(cdiv((long)(ns_ * plane), 256)), dim3(256), 0, st,
+                       d64.as<double>(), D_.as<float>(), ns_ * plane);
+    check_launch("f64_to_f32");
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void OsSqs::set_norm(const float* norm_xy, hipStream_t st)
+{
+    const size_t n = (size_t)ns_ * t_.geo.nx * t_.geo.ny;
+    RN_HIP_CHECK(hipMemcpyAsync(D_.get(), norm_xy, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+}
+
+void OsSqs::sync_transposed(hipStream_t st)
+{
+    launch_transpose(X_[cur_].as<float>(), Xyx_.as<float>(), t_.geo.nx, t_.geo.ny, ns_, st);
+}
+
+void OsSqs::set_image(const float* x_xy, hipStream_t st)
+{
+    const size_t n = (size_t)ns_ * t_.geo.nx * t_.geo.ny;
+    RN_HIP_CHECK(hipMemcpyAsync(X_[cur_].get(), x_xy, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    sync_transposed(st);
+}
+
+void OsSqs::get_image(float* x_xy, hipStream_t st) const
+{
+    const size_t n = (size_t)ns_ * t_.geo.nx * t_.geo.ny;
+    RN_HIP_CHECK(hipMemcpyAsync(x_xy, X_[cur_].get(), n * sizeof(float), hipMemcpyDeviceToDevice, st));
+}
+
+void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
+{
+    RN_REQUIRE(have_w_, "OS-SQS data not set");
+    const FanGeo& g = t_.geo;
+    const size_t plane = (size_t)g.nx * g.ny;
+    dim3 blk(TB, TB), grd(cdiv(g.ny, TB), cdiv(g.nx, TB), ns_);
+    const long nb = (long)grd.x * grd.y;
+    if (track) {
+        red_.ensure((2 * nb * ns_ + 2 * (size_t)ns_) * sizeof(double) + ns_ * plane * sizeof(float));
+        float* x0 = reinterpret_cast<float*>(red_.as<double>() + 2 * nb * ns_ + 2 * (size_t)ns_);
+        RN_HIP_CHECK(hipMemcpyAsync(x0, X_[cur_].get(), ns_ * plane * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    const bool crop = o.lb < o.ub;
+    int start = 0;
+    for (int k = 0; k < nsub_ && start < g.nview; k++) {
+        const int cur = std::min(npers_, g.nview - start);
+        float* X = X_[cur_].as<float>();
+        float* Xn = X_[1 - cur_].as<float>();
+        launch_siddon_fan_fp_residual(t_.k, Xyx_.as<float>(), X, R_.as<float>(), ns_, start, cur,
+                                      B_.as<float>(), W_.as<float>(), st);
+        launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
+        hipLaunchKernelGGL(k_sqs_update, grd, blk, 0, st, X, Xn, G_.as<float>(), D_.as<float>(), g.nx, g.ny,
+                           o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
+        check_launch("sqs_update");
+        launch_transpose(Xn, Xyx_.as<float>(), g.nx, g.ny, ns_, st);
+        cur_ = 1 - cur_;
+        start += cur;
+    }
+    if (track) {
+        double* partial = red_.as<double>();
+        double* sums = partial + 2 * nb * ns_;
+        const float* x0 = reinterpret_cast<const float*>(sums + 2 * (size_t)ns_);
+        hipLaunchKernelGGL(k_diff_sums, grd, blk, 0, st, x0, X_[cur_].as<float>(), g.nx, g.ny, partial);
+        check_launch("diff_sums");
+        hipLaunchKernelGGL(k_reduce_partials, dim3(ns_), dim3(256), 0, st, partial, nb, sums, 0);
+        check_launch("reduce_partials");
+        track->resize(2 * (size_t)ns_);
+        RN_HIP_CHECK(hipMemcpyAsync(track->data(), sums, track->size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        RN_HIP_CHECK(hipStreamSynchronize(st));
+    }
+}
+
+} // namespace rn

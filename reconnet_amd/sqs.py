@@ -1,0 +1,101 @@
+"""Device-resident OS-SQS(-TV) over a batch of 2D fan slices.
+
+Wraps the C-ABI engine rn_sqs_* (ossqs.cu semantics with the Siddon fan
+projector, plus the TotalVariation.cu surrogate as prior).  Subsets are
+contiguous blocks of ceil(nview / M) views of the plan's angle order; with
+``order="interleaved"`` the angles and sinograms are permuted so that those
+blocks are the interleaved subsets {k, k+M, ...} (the reference's `order`
+argument, ReconNet_no_tf.py:419-422).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .geometry import FanGeometry, interleaved_order
+from .torch_ops import FanPlan, _ptr, _stream
+
+__all__ = ["OsSqsTV"]
+
+
+class OsSqsTV:
+    def __init__(self, geo: FanGeometry, nsubsets, nslices, order="interleaved"):
+        self.geo = geo
+        self.nsubsets = int(nsubsets)
+        self.nslices = int(nslices)
+        if order == "interleaved":
+            self.order = interleaved_order(geo.nview, self.nsubsets)
+        elif order is None:
+            self.order = np.arange(geo.nview)
+        else:
+            self.order = np.asarray(order)
+        self.plan = FanPlan(geo, geo.angles()[self.order])
+        self.npers = int(math.ceil(geo.nview / float(self.nsubsets)))
+        self._h = lib().rn_sqs_create(self.plan.handle, self.nslices, self.npers, self.nsubsets)
+        if not self._h:
+            check(-1, "rn_sqs_create")
+        self._order_t = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rn_sqs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _reorder(self, sino):
+        if self._order_t is None or self._order_t.device != sino.device:
+            self._order_t = torch.as_tensor(self.order, dtype=torch.long, device=sino.device)
+        return sino.index_select(2, self._order_t).contiguous()
+
+    def set_data(self, sino, weights=None):
+        """sino, weights: [nslices, nu, nview] float32 CUDA, natural view order."""
+        s = self._reorder(sino)
+        w = self._reorder(weights) if weights is not None else None
+        self._keep = (s, w)
+        check(lib().rn_sqs_set_data(self._h, _ptr(s), _ptr(w) if w is not None else None, _stream()),
+              "rn_sqs_set_data")
+
+    def compute_norm(self):
+        check(lib().rn_sqs_compute_norm(self._h, _stream()), "rn_sqs_compute_norm")
+
+    def set_norm(self, norm):
+        check(lib().rn_sqs_set_norm(self._h, _ptr(norm.contiguous()), _stream()), "rn_sqs_set_norm")
+
+    def norm(self):
+        g = self.geo
+        out = torch.empty((self.nslices, g.nx, g.ny), dtype=torch.float32, device="cuda")
+        check(lib().rn_sqs_get_norm(self._h, _ptr(out), _stream()), "rn_sqs_get_norm")
+        return out
+
+    def set_image(self, x):
+        x = x.contiguous()
+        check(lib().rn_sqs_set_image(self._h, _ptr(x), _stream()), "rn_sqs_set_image")
+
+    def image(self, out=None):
+        g = self.geo
+        if out is None:
+            out = torch.empty((self.nslices, g.nx, g.ny), dtype=torch.float32, device="cuda")
+        check(lib().rn_sqs_get_image(self._h, _ptr(out), _stream()), "rn_sqs_get_image")
+        return out
+
+    def run_pass(self, beta=0.0, eps=1e-6, lb=None, ub=None, lam=1.0, track=False):
+        """One pass over all subsets; returns per-slice (|dx|^2, |x|^2) when track."""
+        lb = float(np.finfo(np.float32).min) if lb is None else float(lb)
+        ub = float(np.finfo(np.float32).max) if ub is None else float(ub)
+        buf = None
+        if track:
+            buf = (ctypes.c_double * (2 * self.nslices))()
+        check(lib().rn_sqs_pass(self._h, float(lam), float(beta), float(eps), lb, ub,
+                                buf if track else None, _stream()), "rn_sqs_pass")
+        if track:
+            a = np.frombuffer(buf, dtype=np.float64).reshape(self.nslices, 2)
+            return a[:, 0].copy(), a[:, 1].copy()
+        return None

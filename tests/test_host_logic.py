@@ -1,0 +1,106 @@
+"""Host-side logic that needs no GPU: parameter files, angle and subset
+recipes, phantoms, noise, short-scan weights and argument plumbing."""
+
+import numpy as np
+import pytest
+
+from reconnet_amd.geometry import WORKLOADS, fan_angles, interleaved_order
+from reconnet_amd.phantom import add_poisson_noise, ellipse_params, phantom
+from reconnet_amd.recon import ParkerWeighting, ReconNet
+
+
+def test_defaults_match_reference_class():
+    r = ReconNet()
+    assert (r.rotview, r.nu, r.nv, r.nx, r.ny, r.nz) == (720, 736, 1, 512, 512, 1)
+    assert (r.dsd, r.dso, r.du) == (1085.6, 595, 1.2858)
+    assert r.da == pytest.approx(1.2858 / 1085.6)
+    assert np.array_equal(np.asarray(r.angles, np.float32), fan_angles(720))
+
+
+def test_fromfile_semantics(tmp_path):
+    p = tmp_path / "param.txt"
+    p.write_text("NX 256\nny\t128\nDU 1.0\ndsd 1000\nROTVIEW 90\noff_a 1.5\ndx 0.7\n")
+    r = ReconNet()
+    r.FromFile(str(p))
+    assert r.nx == 256 and r.ny == 128 and isinstance(r.nx, int)
+    assert r.rotview == 90 and len(r.angles) == 90
+    assert r.off_a == pytest.approx(-1.5)            # negated
+    assert r.da == pytest.approx(1.0 / 1000.0)        # recomputed
+    assert r.dx == pytest.approx(0.7) and isinstance(r.dx, np.float32)
+
+
+def test_tvsqs_shape_check():
+    with pytest.raises(ValueError):
+        ReconNet().cTVSQS2d(np.zeros((2, 8, 8, 1), np.float32))
+
+
+def test_interleaved_order_is_permutation():
+    for n, m in [(1152, 12), (360, 7), (100, 3)]:
+        o = interleaved_order(n, m)
+        assert np.array_equal(np.sort(o), np.arange(n))
+
+
+def test_phantom_is_seeded_and_in_range():
+    a = phantom(7, 64, 64, 1.0, 1.0)
+    b = phantom(7, 64, 64, 1.0, 1.0)
+    assert np.array_equal(a, b) and a.dtype == np.float32
+    p = ellipse_params(7)
+    assert p.shape == (10, 6)
+    assert np.all(np.hypot(p[:, 0], p[:, 1]) <= 110 + 1e-9)
+    assert np.all((p[:, 2] >= 5) & (p[:, 2] <= 80) & (p[:, 5] >= 0) & (p[:, 5] <= 0.04))
+    assert not np.array_equal(a, phantom(8, 64, 64, 1.0, 1.0))
+
+
+def test_phantom_area_matches_ellipse_sum():
+    p = ellipse_params(3, n_ellipses=3, axis_range=(5, 20), r_max=40)
+    img = phantom(3, 256, 256, 0.5, 0.5, n_ellipses=3)
+    # cut off (r_max=40 inside a 128 mm image): integral = sum of mu * pi a b
+    total = float(img.sum()) * 0.25
+    ref = float(np.sum(p[:, 5] * np.pi * p[:, 2] * p[:, 3]))
+    assert total == pytest.approx(ref, rel=2e-3)
+
+
+def test_poisson_noise_statistics():
+    p = np.full((200000,), 2.0, np.float32)
+    noisy, w = add_poisson_noise(p, 1e5, seed=1)
+    counts = w * 1e5
+    assert abs(counts.mean() - 1e5 * np.exp(-2.0)) < 3 * np.sqrt(1e5 * np.exp(-2.0) / 200000) * 3
+    assert noisy.mean() == pytest.approx(2.0, abs=2e-3)
+    n2, _ = add_poisson_noise(p, 1e5, seed=1)
+    assert np.array_equal(noisy, n2)
+
+
+def test_parker_weights_sum_to_one_on_conjugate_rays():
+    # for a short scan pi + 2 gamma, Parker weights of a ray and of its
+    # conjugate (beta + pi + 2 alpha, -alpha) add up to 1
+    r = ReconNet()
+    r.nu, r.rotview = 64, 400
+    r.da = 0.006
+    gamma = r.nu * r.da / 2
+    r.angles = list(np.linspace(0, np.pi + 2 * gamma, r.rotview).astype(np.float32))
+    w = ParkerWeighting(r)[0, :, :, 0]
+    alphas = ((np.arange(r.nu) - (r.nu - 1) / 2) - r.off_a) * r.da
+    betas = np.asarray(r.angles, np.float64)
+    gam = (betas[-1] - np.pi) / 2
+
+    def weight(beta, alpha):
+        a1 = np.sin(np.pi / 4 * beta / (gam - alpha)) ** 2
+        if beta < 2 * gam - 2 * alpha:
+            return a1
+        if beta <= np.pi - 2 * alpha:
+            return 1.0
+        return np.sin(np.pi / 4 * (np.pi + 2 * gam - beta) / (gam + alpha)) ** 2
+
+    iu, iv = 20, 50
+    assert w[iu, iv] == pytest.approx(weight(betas[iv], alphas[iu]), rel=1e-6)
+    b2 = betas[iv] + np.pi + 2 * alphas[iu]
+    assert weight(betas[iv], alphas[iu]) + weight(b2, -alphas[iu]) == pytest.approx(1.0, abs=1e-9)
+
+
+def test_workloads_match_benchmark_configs():
+    c0, c1 = WORKLOADS["fan256"], WORKLOADS["fan512"]
+    assert (c0.nx, c0.nu, c0.nview, c0.dx) == (256, 512, 360, 1.0)
+    assert c0.da == pytest.approx(1.2 / 1085.6)
+    assert (c1.nx, c1.nu, c1.nview, c1.dx) == (512, 736, 1152, 0.7)
+    assert c1.da == pytest.approx(1.0 / 1085.6)
+    assert c1.fov_radius() == pytest.approx(179.2, abs=0.1)
