@@ -190,10 +190,41 @@ int rn_sqs_get_image(void* sqs, float* img, void* stream);
  * |x_new - x_old|^2 and |x_new|^2 over the pass (2 * nslices doubles). */
 int rn_sqs_pass(void* sqs, float lambda, float beta, float eps, float lb, float ub, double* track, void* stream);
 
+/* Per-kernel HIP-event timing inside rn_sqs_pass (events recorded on the pass's stream
+ * around the FP+residual, BP and update launches of every subset).  Enabling resets the
+ * totals; rn_sqs_kernel_times returns total ms per kernel and the number of subset steps. */
+int rn_sqs_set_profiling(void* sqs, int on);
+int rn_sqs_kernel_times(void* sqs, double* ms_fp, double* ms_bp, double* ms_update, long* steps);
+
 /* TV surrogate on nslices device images [nslices][nx][ny]; tv (host, may be NULL) receives
  * the FP64 per-slice costs. */
 int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
         double* tv, void* stream);
+
+/* Learned-prior 3x3 convolutions (no reference counterpart).  Images fp32
+ * [B][H][W]; activations bf16 bit patterns (uint16) NHWC [B][H][W][32]; weights
+ * w1, w5 fp32 [9][32], wt bf16 [9][32 out][32 in]; tap t = 3 (dy+1) + (dx+1);
+ * zero padding.  All pointers are device pointers; 0 on success, -1 on error. */
+int rn_cnn_conv_in(const float* x, const float* w1, const float* b1, unsigned short* a1,
+        int B, int H, int W, void* stream);
+/* mask == NULL: out = relu?(conv(in) + bias) (MFMA forward);
+ * mask != NULL: out = conv(in) * (mask > 0) (data gradient with flipped, transposed weights) */
+int rn_cnn_conv_mid(const unsigned short* in, const unsigned short* wt, const float* bias,
+        const unsigned short* mask, unsigned short* out, int B, int H, int W, int relu, void* stream);
+/* y = xres + b5 + conv(a4; w5) (xres may be NULL) */
+int rn_cnn_conv_out(const unsigned short* a4, const float* w5, float b5, const float* xres, float* y,
+        int B, int H, int W, void* stream);
+/* dz4 = (a4 > 0) * conv^T(dy; w5) */
+int rn_cnn_out_dgrad(const float* dy, const float* w5, const unsigned short* a4, unsigned short* dz4,
+        int B, int H, int W, void* stream);
+/* mode 0: grad[9*32 + 32] of layer 1 (weights, bias) from x and dz1;
+ * mode 1: grad[9*32 + 1] of layer 5 from dy and a4 */
+int rn_cnn_wgrad_narrow(const float* img, const unsigned short* act, float* grad, int B, int H, int W,
+        int mode, void* stream);
+int rn_cnn_bias_grad(const unsigned short* dz, float* grad, long npx, void* stream);
+/* grad[9][32 out][32 in] = sum_p dz[p][out] in[p + tap][in]  (MFMA) */
+int rn_cnn_wgrad_mid(const unsigned short* in, const unsigned short* dz, float* grad, int B, int H, int W,
+        void* stream);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,8 @@ def lib():
     _sig(h, "rn_sqs_set_image", c_int, [_P, _P, _P])
     _sig(h, "rn_sqs_get_image", c_int, [_P, _P, _P])
     _sig(h, "rn_sqs_pass", c_int, [_P, c_float, c_float, c_float, c_float, c_float, _P, _P])
+    _sig(h, "rn_sqs_set_profiling", c_int, [_P, c_int])
+    _sig(h, "rn_sqs_kernel_times", c_int, [_P, _P, _P, _P, _P])
     _sig(h, "rn_tv_sqs", c_int, [_P, _P, _P, c_int, c_int, c_int, c_float, _P, _P])
     h.cTVSQS2D.restype = c_float
     _lib = h

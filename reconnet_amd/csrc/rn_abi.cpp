@@ -626,6 +626,18 @@ extern "C" int rn_sqs_pass(void* sqs, float lambda, float beta, float eps, float
     });
 }
 
+extern "C" int rn_sqs_set_profiling(void* sqs, int on)
+{
+    void* stream = nullptr;
+    SQS_CALL("rn_sqs_set_profiling", { (void)st; e.set_profiling(on != 0); });
+}
+
+extern "C" int rn_sqs_kernel_times(void* sqs, double* ms_fp, double* ms_bp, double* ms_update, long* steps)
+{
+    void* stream = nullptr;
+    SQS_CALL("rn_sqs_kernel_times", { (void)st; e.kernel_times(ms_fp, ms_bp, ms_update, steps); });
+}
+
 extern "C" int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
                          double* tv, void* stream)
 {
@@ -636,5 +648,79 @@ extern "C" int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int
         tv_sqs(img, grad, norm, nx, ny, nslices, eps, tv ? &t : nullptr, scratch, static_cast<hipStream_t>(stream));
         if (tv) std::copy(t.begin(), t.end(), tv);
         else RN_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+// ----- learned-prior convolutions (device pointers; bf16 tensors as raw uint16) -----
+
+#include "rn_conv.h"
+
+namespace {
+DevBuf& conv_work()
+{
+    thread_local DevBuf w;
+    return w;
+}
+} // namespace
+
+extern "C" int rn_cnn_conv_in(const float* x, const float* w1, const float* b1, uint16_t* a1, int B, int H, int W,
+                              void* stream)
+{
+    return guarded_status("rn_cnn_conv_in", [&] {
+        RN_REQUIRE(x && w1 && b1 && a1 && B > 0 && H > 0 && W > 0, "invalid argument");
+        conv_in(x, w1, b1, a1, B, H, W, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_cnn_conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const uint16_t* mask,
+                               uint16_t* out, int B, int H, int W, int relu, void* stream)
+{
+    return guarded_status("rn_cnn_conv_mid", [&] {
+        RN_REQUIRE(in && wt && out && B > 0 && H > 0 && W > 0, "invalid argument");
+        conv_mid(in, wt, bias, mask, out, B, H, W, relu != 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_cnn_conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, float* y, int B,
+                               int H, int W, void* stream)
+{
+    return guarded_status("rn_cnn_conv_out", [&] {
+        RN_REQUIRE(a4 && w5 && y && B > 0 && H > 0 && W > 0, "invalid argument");
+        conv_out(a4, w5, b5, xres, y, B, H, W, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_cnn_out_dgrad(const float* dy, const float* w5, const uint16_t* a4, uint16_t* dz4, int B, int H,
+                                int W, void* stream)
+{
+    return guarded_status("rn_cnn_out_dgrad", [&] {
+        RN_REQUIRE(dy && w5 && a4 && dz4 && B > 0 && H > 0 && W > 0, "invalid argument");
+        conv_out_dgrad(dy, w5, a4, dz4, B, H, W, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_cnn_wgrad_narrow(const float* img, const uint16_t* act, float* grad, int B, int H, int W, int mode,
+                                   void* stream)
+{
+    return guarded_status("rn_cnn_wgrad_narrow", [&] {
+        RN_REQUIRE(img && act && grad && B > 0 && H > 0 && W > 0 && (mode == 0 || mode == 1), "invalid argument");
+        wgrad_narrow(img, act, grad, B, H, W, mode, conv_work(), static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_cnn_bias_grad(const uint16_t* dz, float* grad, long npx, void* stream)
+{
+    return guarded_status("rn_cnn_bias_grad", [&] {
+        RN_REQUIRE(dz && grad && npx > 0, "invalid argument");
+        bias_grad(dz, grad, npx, conv_work(), static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_cnn_wgrad_mid(const uint16_t* in, const uint16_t* dz, float* grad, int B, int H, int W,
+                                void* stream)
+{
+    return guarded_status("rn_cnn_wgrad_mid", [&] {
+        RN_REQUIRE(in && dz && grad && B > 0 && H > 0 && W > 0, "invalid argument");
+        wgrad_mid(in, dz, grad, B, H, W, conv_work(), static_cast<hipStream_t>(stream));
     });
 }

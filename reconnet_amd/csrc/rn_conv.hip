@@ -1,0 +1,476 @@
+// 3x3 "same" convolutions of the learned prior (no reference counterpart).
+//
+// Network: 1 -> 32 -> 32 -> 32 -> 32 -> 1 channels, 3x3 kernels, ReLU after
+// the first four layers, residual output y = x + conv5(...).
+// Activations are BF16, NHWC ([B][H][W][32], 64 bytes per pixel); every
+// product accumulates in FP32.
+//
+// The 32 -> 32 layers (forward and data gradient) run on gfx950 BF16 matrix
+// cores: v_mfma_f32_16x16x32_bf16 with A = weights (16 output channels x 32
+// input channels of one tap) and B = 16 consecutive pixels x 32 input
+// channels; 9 taps x 2 channel tiles = 18 MFMAs per 16-pixel segment.  The
+// weight gradient of those layers is an MFMA GEMM over pixels with LDS
+// transposes.  The 1 -> 32 and 32 -> 1 layers (K = 9, N = 1) are VALU kernels.
+
+#include "rn_conv.h"
+
+namespace rn {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
+{
+    bf16x8 r;
+    __builtin_memcpy(&r, &v, sizeof(r));
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// layer 1: x (fp32 [B][H][W]) -> relu(conv(x) + b) (bf16 NHWC), w [9][32]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
+                                                 const float* __restrict__ b, uint16_t* __restrict__ out,
+                                                 int B, int H, int W)
+{
+    __shared__ float sw[9 * 32 + 32];
+    for (int i = threadIdx.x; i < 9 * 32 + 32; i += blockDim.x) sw[i] = i < 288 ? w[i] : b[i - 288];
+    __syncthreads();
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long npx = (long)B * H * W;
+    if (p >= npx) return;
+    const int xw = (int)(p % W), yh = (int)((p / W) % H);
+    const long base = p - (long)yh * W - xw;
+    float v[9];
+    #pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int yy = yh + t / 3 - 1, xx = xw + t % 3 - 1;
+        v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[base + (long)yy * W + xx] : 0.0f;
+    }
+    uint32_t packed[16];
+    #pragma unroll
+    for (int c = 0; c < 32; c += 2) {
+        float z0 = sw[288 + c], z1 = sw[288 + c + 1];
+        #pragma unroll
+        for (int t = 0; t < 9; ++t) { z0 = fmaf(sw[t * 32 + c], v[t], z0); z1 = fmaf(sw[t * 32 + c + 1], v[t], z1); }
+        packed[c / 2] = (uint32_t)f2bf(fmaxf(z0, 0.0f)) | ((uint32_t)f2bf(fmaxf(z1, 0.0f)) << 16);
+    }
+    uint4* o = reinterpret_cast<uint4*>(out + p * 32);
+    #pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// 32 -> 32 layer on MFMA.  wt: [9][32 co][32 ci] bf16 (tap t = 3*dy + dx).
+// mode 0: out = relu?(acc + bias); mode 1: out = acc * (mask > 0)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+                                                  const float* __restrict__ bias, const uint16_t* __restrict__ mask,
+                                                  uint16_t* __restrict__ out, int B, int H, int W, int relu,
+                                                  int mode)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    // weight fragments: A[m = co][k = ci], lane holds co = r16 (+16 mt), ci = 8 g4 .. 8 g4 + 7
+    bf16x8 wa[9][2];
+    #pragma unroll
+    for (int t = 0; t < 9; ++t)
+        #pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+            wa[t][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((t * 32 + 16 * mt + r16) * 32 + 8 * g4)));
+    float bv[2][4];
+    #pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) bv[mt][i] = (mode == 0) ? bias[16 * mt + 4 * g4 + i] : 0.0f;
+    const int segs_per_row = (W + 15) / 16;
+    const long nseg = (long)B * H * segs_per_row;
+    for (long sg = wave; sg < nseg; sg += nwaves) {
+        const int sx = (int)(sg % segs_per_row);
+        const long by = sg / segs_per_row;
+        const int yh = (int)(by % H);
+        const long bimg = by / H;
+        const int xpix = sx * 16 + r16;                // this lane's B column (pixel)
+        f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        #pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int yy = yh + t / 3 - 1, xx = xpix + t % 3 - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *reinterpret_cast<const uint4*>(in + (((bimg * H + yy) * W + xx) * 32 + 8 * g4));
+            const bf16x8 bfrag = as_bf16x8(v);
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], bfrag, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], bfrag, acc[1], 0, 0, 0);
+        }
+        // D[m = co][n = pixel]: lane holds co = 16 mt + 4 g4 + i for pixel r16
+        if (xpix < W) {
+            const long pbase = ((bimg * H + yh) * W + xpix) * 32;
+            #pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int c0 = 16 * mt + 4 * g4;
+                float z[4];
+                if (mode == 0) {
+                    #pragma unroll
+                    for (int i = 0; i < 4; ++i) { z[i] = acc[mt][i] + bv[mt][i]; if (relu) z[i] = fmaxf(z[i], 0.0f); }
+                } else {
+                    const uint2 m = *reinterpret_cast<const uint2*>(mask + pbase + c0);
+                    const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16),
+                                            (uint16_t)(m.y & 0xffff), (uint16_t)(m.y >> 16)};
+                    #pragma unroll
+                    for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
+                }
+                uint2 o;
+                o.x = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+                o.y = (uint32_t)f2bf(z[2]) | ((uint32_t)f2bf(z[3]) << 16);
+                *reinterpret_cast<uint2*>(out + pbase + c0) = o;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// layer 5: y = xres + b + sum_t sum_ci a4[p + t][ci] w5[t][ci]   (fp32 out)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a, const float* __restrict__ w,
+                                                  float b5, const float* __restrict__ xres, float* __restrict__ y,
+                                                  int B, int H, int W)
+{
+    __shared__ float sw[288];
+    for (int i = threadIdx.x; i < 288; i += blockDim.x) sw[i] = w[i];
+    __syncthreads();
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long npx = (long)B * H * W;
+    if (p >= npx) return;
+    const int xw = (int)(p % W), yh = (int)((p / W) % H);
+    const long base = p - (long)yh * W - xw;
+    float s = b5;
+    for (int t = 0; t < 9; ++t) {
+        const int yy = yh + t / 3 - 1, xx = xw + t % 3 - 1;
+        if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
+        const uint4* src = reinterpret_cast<const uint4*>(a + (base + (long)yy * W + xx) * 32);
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = src[q];
+            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+            #pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = 8 * q + 2 * e;
+                s = fmaf(sw[t * 32 + c], bf2f((uint16_t)(u[e] & 0xffff)), s);
+                s = fmaf(sw[t * 32 + c + 1], bf2f((uint16_t)(u[e] >> 16)), s);
+            }
+        }
+    }
+    y[p] = (xres ? xres[p] : 0.0f) + s;
+}
+
+// layer-5 data gradient: dz4[p][ci] = (a4[p][ci] > 0) * sum_t dy[p - off_t] w5[t][ci]
+__global__ __launch_bounds__(256) void k_out_dgrad(const float* __restrict__ dy, const float* __restrict__ w,
+                                                   const uint16_t* __restrict__ a4, uint16_t* __restrict__ dz,
+                                                   int B, int H, int W)
+{
+    __shared__ float sw[288];
+    for (int i = threadIdx.x; i < 288; i += blockDim.x) sw[i] = w[i];
+    __syncthreads();
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long npx = (long)B * H * W;
+    if (p >= npx) return;
+    const int xw = (int)(p % W), yh = (int)((p / W) % H);
+    const long base = p - (long)yh * W - xw;
+    float g[9];
+    #pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        // output pixel q = p - off_t receives a4[p] through tap t
+        const int yy = yh - (t / 3 - 1), xx = xw - (t % 3 - 1);
+        g[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? dy[base + (long)yy * W + xx] : 0.0f;
+    }
+    const uint4* am = reinterpret_cast<const uint4*>(a4 + p * 32);
+    uint4* o = reinterpret_cast<uint4*>(dz + p * 32);
+    #pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 m = am[q];
+        const uint32_t mu[4] = {m.x, m.y, m.z, m.w};
+        uint32_t r[4];
+        #pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float z[2];
+            #pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = 8 * q + 2 * e + h;
+                float s = 0.0f;
+                #pragma unroll
+                for (int t = 0; t < 9; ++t) s = fmaf(sw[t * 32 + c], g[t], s);
+                const uint16_t mk = h ? (uint16_t)(mu[e] >> 16) : (uint16_t)(mu[e] & 0xffff);
+                z[h] = bf2f(mk) > 0.0f ? s : 0.0f;
+            }
+            r[e] = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+        }
+        o[q] = make_uint4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradients of the narrow layers (reduction over pixels, VALU):
+// mode 0 (layer 1): part[t][c] = sum_p x[p + off_t] dz1[p][c], part[9][c] = sum_p dz1[p][c]
+// mode 1 (layer 5): part[t][c] = sum_p dy[p] a4[p + off_t][c], part[9][0] = sum_p dy[p]
+// one workgroup: 8 groups of 32 lanes (lane = channel c), partials per workgroup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_wgrad_narrow(const float* __restrict__ img, const uint16_t* __restrict__ act,
+                                                      float* __restrict__ part, int B, int H, int W, int mode)
+{
+    __shared__ float red[8][10 * 32];
+    const int c = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const long npx = (long)B * H * W;
+    float acc[10];
+    #pragma unroll
+    for (int t = 0; t < 10; ++t) acc[t] = 0.0f;
+    for (long p = (long)blockIdx.x * 8 + grp; p < npx; p += (long)gridDim.x * 8) {
+        const int xw = (int)(p % W), yh = (int)((p / W) % H);
+        const long base = p - (long)yh * W - xw;
+        if (mode == 0) {
+            const float d = bf2f(act[p * 32 + c]);
+            #pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int yy = yh + t / 3 - 1, xx = xw + t % 3 - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) acc[t] = fmaf(img[base + (long)yy * W + xx], d, acc[t]);
+            }
+            acc[9] += d;
+        } else {
+            const float d = img[p];
+            #pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int yy = yh + t / 3 - 1, xx = xw + t % 3 - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    acc[t] = fmaf(d, bf2f(act[(base + (long)yy * W + xx) * 32 + c]), acc[t]);
+            }
+            acc[9] += d;
+        }
+    }
+    #pragma unroll
+    for (int t = 0; t < 10; ++t) red[grp][t * 32 + c] = acc[t];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 320; i += blockDim.x) {
+        float s = 0.0f;
+        #pragma unroll
+        for (int gidx = 0; gidx < 8; ++gidx) s += red[gidx][i];
+        part[(long)blockIdx.x * 320 + i] = s;
+    }
+}
+
+// bias gradient of a 32-channel NHWC tensor: part[wg][c] = sum_p dz[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* __restrict__ dz, float* __restrict__ part, long npx)
+{
+    __shared__ float red[8][32];
+    const int c = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    float s = 0.0f;
+    for (long p = (long)blockIdx.x * 8 + grp; p < npx; p += (long)gridDim.x * 8) s += bf2f(dz[p * 32 + c]);
+    red[grp][c] = s;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float t = 0.0f;
+        for (int gidx = 0; gidx < 8; ++gidx) t += red[gidx][threadIdx.x];
+        part[(long)blockIdx.x * 32 + threadIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 32 -> 32 weight gradient on MFMA:
+//   dW[t][co][ci] = sum_p dz[p][co] in[p + off_t][ci]
+// GEMM with M = co, N = ci, K = pixels.  Each wave takes 32-pixel row chunks;
+// per chunk it transposes dz into LDS as [co][px] and the 3 x 34 input pixels
+// into 9 shifted copies [t][ci][px], so every fragment is one aligned 16-byte
+// LDS read.  All waves of a workgroup run the same number of chunk rounds
+// (out-of-range chunks contribute zeros) so block barriers are uniform.
+// ---------------------------------------------------------------------------
+constexpr int WG_WAVES = 4;
+constexpr int LDS_PER_WAVE = 32 * 32 + 9 * 32 * 32;   // bf16 elements
+
+__global__ __launch_bounds__(256) void k_wgrad_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ dz,
+                                                   float* __restrict__ part, int B, int H, int W)
+{
+    extern __shared__ uint16_t lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    uint16_t* dzT = lds + wv * LDS_PER_WAVE;        // [32 co][32 px]
+    uint16_t* inS = dzT + 32 * 32;                  // [9 t][32 ci][32 px]
+    const int chunks_per_row = (W + 31) / 32;
+    const long nchunk = (long)B * H * chunks_per_row;
+    const long stride = (long)gridDim.x * WG_WAVES;
+    f32x4 acc[9][2][2];
+    #pragma unroll
+    for (int t = 0; t < 9; ++t)
+        #pragma unroll
+        for (int a = 0; a < 2; ++a)
+            #pragma unroll
+            for (int b = 0; b < 2; ++b) acc[t][a][b] = f32x4{0, 0, 0, 0};
+    const long rounds = (nchunk + stride - 1) / stride;
+    for (long rd = 0; rd < rounds; ++rd) {
+        const long ch = rd * stride + (long)blockIdx.x * WG_WAVES + wv;
+        const bool live = ch < nchunk;
+        long bimg = 0; int yh = 0, x0 = 0;
+        if (live) {
+            x0 = (int)(ch % chunks_per_row) * 32;
+            const long by = ch / chunks_per_row;
+            yh = (int)(by % H);
+            bimg = by / H;
+        }
+        // dz chunk -> dzT[co][px]
+        for (int j = lane; j < 128; j += 64) {
+            const int px = j >> 2, c8 = (j & 3) * 8;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (live && x0 + px < W) v = *reinterpret_cast<const uint4*>(dz + ((bimg * H + yh) * W + x0 + px) * 32 + c8);
+            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+            #pragma unroll
+            for (int e = 0; e < 8; ++e)
+                dzT[(c8 + e) * 32 + px] = (uint16_t)((e & 1) ? (u[e >> 1] >> 16) : (u[e >> 1] & 0xffff));
+        }
+        // input rows yh-1..yh+1, pixels x0-1..x0+32 -> inS[t][ci][px_out], px_out = px_in - dx
+        for (int j = lane; j < 3 * 34 * 4; j += 64) {
+            const int r = j / 136, rem = j % 136, pin = rem >> 2, c8 = (rem & 3) * 8;
+            const int yy = yh + r - 1, xx = x0 + pin - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (live && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *reinterpret_cast<const uint4*>(in + ((bimg * H + yy) * W + xx) * 32 + c8);
+            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+            #pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int po = pin - dx;
+                if (po < 0 || po >= 32) continue;
+                uint16_t* dst = inS + ((r * 3 + dx) * 32 + c8) * 32 + po;
+                #pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    dst[e * 32] = (uint16_t)((e & 1) ? (u[e >> 1] >> 16) : (u[e >> 1] & 0xffff));
+            }
+        }
+        __syncthreads();
+        bf16x8 af[2];
+        #pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+            af[mt] = as_bf16x8(*reinterpret_cast<const uint4*>(dzT + (16 * mt + r16) * 32 + 8 * g4));
+        #pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            #pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const bf16x8 bfrag = as_bf16x8(*reinterpret_cast<const uint4*>(inS + (t * 32 + 16 * nt + r16) * 32 + 8 * g4));
+                acc[t][0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bfrag, acc[t][0][nt], 0, 0, 0);
+                acc[t][1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bfrag, acc[t][1][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // partial of this wave: part[wave][t][co][ci], D lane layout n = ci = 16 nt + r16, m = co = 16 mt + 4 g4 + i
+    float* dst = part + ((long)blockIdx.x * WG_WAVES + wv) * 9216;
+    #pragma unroll
+    for (int t = 0; t < 9; ++t)
+        #pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+            #pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+                #pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    dst[(t * 32 + 16 * mt + 4 * g4 + i) * 32 + 16 * nt + r16] = acc[t][mt][nt][i];
+}
+
+// out[i] (+)= sum_r part[r][i]
+__global__ void k_reduce_rows(const float* __restrict__ part, float* __restrict__ out, int nrows, int n, int accumulate)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int r = 0; r < nrows; ++r) s += part[(long)r * n + i];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+int grid_for(long work, int per_block, int cap)
+{
+    long g = (work + per_block - 1) / per_block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+} // namespace
+
+void conv_in(const float* x, const float* w, const float* b, uint16_t* out, int B, int H, int W, hipStream_t st)
+{
+    const long npx = (long)B * H * W;
+    hipLaunchKernelGGL(k_conv_in, dim3(cdiv(npx, 256)), dim3(256), 0, st, x, w, b, out, B, H, W);
+    check_launch("conv_in");
+}
+
+void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const uint16_t* mask, uint16_t* out,
+              int B, int H, int W, bool relu, hipStream_t st)
+{
+    RN_REQUIRE(mask != nullptr || bias != nullptr, "conv_mid needs a bias (forward) or a mask (data gradient)");
+    const long nseg = (long)B * H * ((W + 15) / 16);
+    const int blocks = grid_for(nseg, 4 * 4, 4096);
+    hipLaunchKernelGGL(k_conv_mid, dim3(blocks), dim3(256), 0, st, in, wt, bias, mask, out, B, H, W, relu ? 1 : 0,
+                       mask ? 1 : 0);
+    check_launch("conv_mid");
+}
+
+void conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, float* y, int B, int H, int W,
+              hipStream_t st)
+{
+    const long npx = (long)B * H * W;
+    hipLaunchKernelGGL(k_conv_out, dim3(cdiv(npx, 256)), dim3(256), 0, st, a4, w5, b5, xres, y, B, H, W);
+    check_launch("conv_out");
+}
+
+void conv_out_dgrad(const float* dy, const float* w5, const uint16_t* a4, uint16_t* dz4, int B, int H, int W,
+                    hipStream_t st)
+{
+    const long npx = (long)B * H * W;
+    hipLaunchKernelGGL(k_out_dgrad, dim3(cdiv(npx, 256)), dim3(256), 0, st, dy, w5, a4, dz4, B, H, W);
+    check_launch("conv_out_dgrad");
+}
+
+void wgrad_narrow(const float* img, const uint16_t* act, float* grad, int B, int H, int W, int mode, DevBuf& work,
+                  hipStream_t st)
+{
+    const long npx = (long)B * H * W;
+    const int blocks = grid_for(npx, 8 * 64, 1024);
+    work.ensure((size_t)blocks * 320 * sizeof(float));
+    hipLaunchKernelGGL(k_wgrad_narrow, dim3(blocks), dim3(256), 0, st, img, act, work.as<float>(), B, H, W, mode);
+    check_launch("wgrad_narrow");
+    hipLaunchKernelGGL(k_reduce_rows, dim3(cdiv(320, 256)), dim3(256), 0, st, work.as<float>(), grad, blocks, 320, 0);
+    check_launch("reduce_rows");
+}
+
+void bias_grad(const uint16_t* dz, float* grad, long npx, DevBuf& work, hipStream_t st)
+{
+    const int blocks = grid_for(npx, 8 * 64, 1024);
+    work.ensure((size_t)blocks * 32 * sizeof(float));
+    hipLaunchKernelGGL(k_bias_grad, dim3(blocks), dim3(256), 0, st, dz, work.as<float>(), npx);
+    check_launch("bias_grad");
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(32), 0, st, work.as<float>(), grad, blocks, 32, 0);
+    check_launch("reduce_rows");
+}
+
+void wgrad_mid(const uint16_t* in, const uint16_t* dz, float* grad, int B, int H, int W, DevBuf& work, hipStream_t st)
+{
+    const long nchunk = (long)B * H * ((W + 31) / 32);
+    const int blocks = grid_for(nchunk, WG_WAVES * 8, 1024);
+    work.ensure((size_t)blocks * WG_WAVES * 9216 * sizeof(float));
+    const size_t lds = (size_t)WG_WAVES * LDS_PER_WAVE * sizeof(uint16_t);
+    hipLaunchKernelGGL(k_wgrad_mid, dim3(blocks), dim3(256), lds, st, in, dz, work.as<float>(), B, H, W);
+    check_launch("wgrad_mid");
+    hipLaunchKernelGGL(k_reduce_rows, dim3(cdiv(9216, 256)), dim3(256), 0, st, work.as<float>(), grad,
+                       blocks * WG_WAVES, 9216, 0);
+    check_launch("reduce_rows");
+}
+
+} // namespace rn

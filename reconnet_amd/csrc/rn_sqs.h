@@ -40,6 +40,11 @@ public:
     void pass(const SqsStep& o, hipStream_t st, std::vector<double>* track = nullptr);
     // refresh the transposed copy after the image was changed externally
     void sync_transposed(hipStream_t st);
+    // Per-kernel timing with HIP events recorded on the pass's stream around
+    // every launch (FP+residual, BP, update+transpose).  Totals in ms and the
+    // number of timed subset steps since the last reset.
+    void set_profiling(bool on);
+    void kernel_times(double* ms_fp, double* ms_bp, double* ms_upd, long* steps) const;
     const FanTables& tables() const { return t_; }
     int nslices() const { return ns_; }
 private:
@@ -48,6 +53,12 @@ private:
     DevBuf B_, W_, D_, X_[2], Xyx_, R_, G_, red_;
     int cur_ = 0;
     bool have_w_ = false;
+    bool prof_ = false;
+    std::vector<hipEvent_t> ev_;
+    double t_fp_ = 0, t_bp_ = 0, t_upd_ = 0;
+    long t_steps_ = 0;
+public:
+    ~OsSqs();
 };
 
 } // namespace rn

@@ -241,6 +241,26 @@ void OsSqs::sync_transposed(hipStream_t st)
     launch_transpose(X_[cur_].as<float>(), Xyx_.as<float>(), t_.geo.nx, t_.geo.ny, ns_, st);
 }
 
+OsSqs::~OsSqs()
+{
+    for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+}
+
+void OsSqs::set_profiling(bool on)
+{
+    prof_ = on;
+    t_fp_ = t_bp_ = t_upd_ = 0;
+    t_steps_ = 0;
+}
+
+void OsSqs::kernel_times(double* ms_fp, double* ms_bp, double* ms_upd, long* steps) const
+{
+    if (ms_fp) *ms_fp = t_fp_;
+    if (ms_bp) *ms_bp = t_bp_;
+    if (ms_upd) *ms_upd = t_upd_;
+    if (steps) *steps = t_steps_;
+}
+
 void OsSqs::set_image(const float* x_xy, hipStream_t st)
 {
     const size_t n = (size_t)ns_ * t_.geo.nx * t_.geo.ny;
@@ -267,20 +287,41 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         RN_HIP_CHECK(hipMemcpyAsync(x0, X_[cur_].get(), ns_ * plane * sizeof(float), hipMemcpyDeviceToDevice, st));
     }
     const bool crop = o.lb < o.ub;
-    int start = 0;
+    if (prof_ && ev_.size() < 4 * (size_t)nsub_) {
+        for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+        ev_.assign(4 * (size_t)nsub_, nullptr);
+        for (auto& e : ev_) RN_HIP_CHECK(hipEventCreate(&e));
+    }
+    int start = 0, nsteps = 0;
     for (int k = 0; k < nsub_ && start < g.nview; k++) {
         const int cur = std::min(npers_, g.nview - start);
         float* X = X_[cur_].as<float>();
         float* Xn = X_[1 - cur_].as<float>();
+        if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k], st));
         launch_siddon_fan_fp_residual(t_.k, Xyx_.as<float>(), X, R_.as<float>(), ns_, start, cur,
                                       B_.as<float>(), W_.as<float>(), st);
+        if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 1], st));
         launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
+        if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 2], st));
         hipLaunchKernelGGL(k_sqs_update, grd, blk, 0, st, X, Xn, G_.as<float>(), D_.as<float>(), g.nx, g.ny,
                            o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
         check_launch("sqs_update");
         launch_transpose(Xn, Xyx_.as<float>(), g.nx, g.ny, ns_, st);
+        if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 3], st));
         cur_ = 1 - cur_;
         start += cur;
+        nsteps++;
+    }
+    if (prof_) {
+        RN_HIP_CHECK(hipEventSynchronize(ev_[4 * (nsteps - 1) + 3]));
+        for (int k = 0; k < nsteps; k++) {
+            float a = 0, b = 0, c = 0;
+            RN_HIP_CHECK(hipEventElapsedTime(&a, ev_[4 * k], ev_[4 * k + 1]));
+            RN_HIP_CHECK(hipEventElapsedTime(&b, ev_[4 * k + 1], ev_[4 * k + 2]));
+            RN_HIP_CHECK(hipEventElapsedTime(&c, ev_[4 * k + 2], ev_[4 * k + 3]));
+            t_fp_ += a; t_bp_ += b; t_upd_ += c;
+        }
+        t_steps_ += nsteps;
     }
     if (track) {
         double* partial = red_.as<double>();

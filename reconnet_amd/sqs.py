@@ -86,6 +86,17 @@ class OsSqsTV:
         check(lib().rn_sqs_get_image(self._h, _ptr(out), _stream()), "rn_sqs_get_image")
         return out
 
+    def set_profiling(self, on):
+        check(lib().rn_sqs_set_profiling(self._h, 1 if on else 0), "rn_sqs_set_profiling")
+
+    def kernel_times(self):
+        """Total HIP-event ms of (FP+residual, BP, update) launches and the step count."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_long()
+        check(lib().rn_sqs_kernel_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                        ctypes.byref(n)), "rn_sqs_kernel_times")
+        return {"fp": a.value, "bp": b.value, "update": c.value, "steps": n.value}
+
     def run_pass(self, beta=0.0, eps=1e-6, lb=None, ub=None, lam=1.0, track=False):
         """One pass over all subsets; returns per-slice (|dx|^2, |x|^2) when track."""
         lb = float(np.finfo(np.float32).min) if lb is None else float(lb)
