@@ -59,7 +59,6 @@ class OsSqsTV:
         """sino, weights: [nslices, nu, nview] float32 CUDA, natural view order."""
         s = self._reorder(sino)
         w = self._reorder(weights) if weights is not None else None
-        self._keep = (s, w)
         check(lib().rn_sqs_set_data(self._h, _ptr(s), _ptr(w) if w is not None else None, _stream()),
               "rn_sqs_set_data")
 
