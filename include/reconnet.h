@@ -226,6 +226,19 @@ int rn_cnn_bias_grad(const unsigned short* dz, float* grad, long npx, void* stre
 int rn_cnn_wgrad_mid(const unsigned short* in, const unsigned short* dz, float* grad, int B, int H, int W,
         void* stream);
 
+/* Batches of 2D fan slices of uneven size (no reference counterpart).  dims
+ * [nslices][4] = (nx, ny, nu, nview); geo [nslices][6] = (dx, dy, da, off_a,
+ * dsd, dso); angles: the nview angles of every slice, concatenated (host).
+ * Images are concatenated [nx][ny] planes, sinograms [nu][nview] planes (device).
+ * Each call is one persistent launch over the whole batch (work queue across
+ * slices); results equal independent per-slice calls. */
+void* rn_ragged_create(int nslices, const int* dims, const float* geo, const float* angles);
+void rn_ragged_destroy(void* batch);
+int rn_ragged_forward(void* batch, int kind /* RN_FP_DD | RN_FP_SIDDON */, const float* img, float* prj,
+        void* stream);
+int rn_ragged_backward(void* batch, int kind /* RN_BP_DD_ADJOINT | RN_BP_SIDDON */, const float* prj, float* img,
+        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -724,3 +724,51 @@ extern "C" int rn_cnn_wgrad_mid(const uint16_t* in, const uint16_t* dz, float* g
         wgrad_mid(in, dz, grad, B, H, W, conv_work(), static_cast<hipStream_t>(stream));
     });
 }
+
+// ----- ragged (uneven-size) persistent batches -----
+
+#include "rn_ragged.h"
+
+extern "C" void* rn_ragged_create(int nslices, const int* dims, const float* geo, const float* angles)
+{
+    RaggedBatch* out = nullptr;
+    guarded("rn_ragged_create", [&] {
+        RN_REQUIRE(nslices > 0 && dims && geo && angles, "invalid argument");
+        std::vector<FanGeo> geos;
+        std::vector<const float*> ang;
+        const float* a = angles;
+        for (int s = 0; s < nslices; s++) {
+            const int* d = dims + 4 * s;
+            const float* f = geo + 6 * s;
+            geos.push_back(make_fan(d[0], d[1], 1, f[0], f[1], 1.0f, d[2], d[3], 1, f[2], 1.0f, f[3], 0.0f, f[4], f[5]));
+            ang.push_back(a);
+            a += d[3];
+        }
+        std::unique_ptr<RaggedBatch> r(new RaggedBatch());
+        CallStream s;
+        r->setup(geos, ang, s.get());
+        out = r.release();
+    });
+    return out;
+}
+
+extern "C" void rn_ragged_destroy(void* h)
+{
+    guarded("rn_ragged_destroy", [&] { delete static_cast<RaggedBatch*>(h); });
+}
+
+extern "C" int rn_ragged_forward(void* h, int kind, const float* img, float* prj, void* stream)
+{
+    return guarded_status("rn_ragged_forward", [&] {
+        RN_REQUIRE(h && img && prj && (kind == RN_FP_DD || kind == RN_FP_SIDDON), "invalid argument");
+        static_cast<RaggedBatch*>(h)->forward(kind == RN_FP_SIDDON, img, prj, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int rn_ragged_backward(void* h, int kind, const float* prj, float* img, void* stream)
+{
+    return guarded_status("rn_ragged_backward", [&] {
+        RN_REQUIRE(h && img && prj && (kind == RN_BP_DD_ADJOINT || kind == RN_BP_SIDDON), "invalid argument");
+        static_cast<RaggedBatch*>(h)->backward(kind == RN_BP_SIDDON, prj, img, static_cast<hipStream_t>(stream));
+    });
+}

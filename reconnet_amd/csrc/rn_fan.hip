@@ -12,7 +12,7 @@
 // SiddonFanProjectionKernel / SiddonFanBackprojectionKernel (Siddon.cu:240-382),
 // bpFanKernel3D (PixelDriven.cu:11-75).
 
-#include "rn_fan.h"
+#include "rn_fan_device.h"
 
 #include <cmath>
 #include <cstring>
@@ -21,71 +21,7 @@ namespace rn {
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// geometry helpers
-// ---------------------------------------------------------------------------
-
-// Ray of detector angle a (sin sa, cos ca) in view t (cos ct, sin st):
-// X views:  X(Y) = A + B (Y - ny/2)   (absolute image coordinates)
-__device__ __forceinline__ void xline(const FanK& g, float ct, float st, float sa, float ca,
-                                      float& A, float& B, float& cu)
-{
-    cu = ca * ct + sa * st;            // cos(a - t)
-    const float su = sa * ct - ca * st;  // sin(a - t)
-    const float inv = 1.0f / cu;
-    A = 0.5f * g.nx + (g.dso / g.dx) * sa * inv;
-    B = (g.dy / g.dx) * su * inv;
-}
-
-// Y views:  Y(X) = A + B (X - nx/2)
-__device__ __forceinline__ void yline(const FanK& g, float ct, float st, float sa, float ca,
-                                      float& A, float& B, float& su)
-{
-    const float cu = ca * ct + sa * st;
-    su = sa * ct - ca * st;
-    const float inv = 1.0f / su;
-    A = 0.5f * g.ny - (g.dso / g.dy) * sa * inv;
-    B = (g.dx / g.dy) * cu * inv;
-}
-
-// Average over [lo, hi] of a piecewise-constant line of n cells (zero outside
-// [0, n)).  For hi == lo the value at lo is returned (a ray exactly along an
-// image line).  Numerator and denominator use the same rounded end points, so
-// a constant line is reproduced exactly.
-template <class T>
-__device__ __forceinline__ T seg_avg(const T* __restrict__ line, int n, float lo, float hi)
-{
-    const float w = hi - lo;
-    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
-    if (!(h > l)) {
-        if (w > 0.0f) return T(0);
-        return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : T(0);
-    }
-    const int m0 = (int)l;
-    const int m1 = min((int)h, n - 1);
-    T s = T(0);
-    for (int m = m0; m <= m1; ++m) {
-        const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
-        s += line[m] * (T)(ph - pl);
-    }
-    return s / (T)w;
-}
-
-// Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).
-__device__ __forceinline__ float seg_weight(int m, float lo, float hi)
-{
-    const float w = hi - lo;
-    const float pl = fmaxf(lo, (float)m), ph = fminf(hi, (float)(m + 1));
-    if (w > 0.0f) return ph > pl ? (ph - pl) / w : 0.0f;
-    return (lo >= (float)m && lo < (float)(m + 1)) ? 1.0f : 0.0f;
-}
-
-// conservative half width (in bins) of the set of rays through a disc of
-// radius r at distance dist from the source:  asin(r/d) <= r/(d-r)
-__device__ __forceinline__ float half_width_bins(float r, float dist, float inv_da)
-{
-    return r * inv_da / fmaxf(dist - r, 1e-3f) + 0.02f;
-}
+using namespace fan;
 
 // ---------------------------------------------------------------------------
 // forward projectors (line driven)
@@ -99,43 +35,10 @@ __global__ __launch_bounds__(128) void k_dd_fan_fp(FanK g, const float* __restri
     const int vi = blockIdx.y;
     const int s = blockIdx.z;
     if (iu >= g.nu) return;
-    const int view = view0 + vi;
-    const float ct = g.vcos[view], st = g.vsin[view];
     const long plane = (long)g.nx * g.ny;
-    float acc = 0.0f;
-    float A0, B0, A1, B1, c0, c1;
-    if (g.vcls[view] == 0) {
-        xline(g, ct, st, g.esin[iu], g.ecos[iu], A0, B0, c0);
-        xline(g, ct, st, g.esin[iu + 1], g.ecos[iu + 1], A1, B1, c1);
-        const float* R = RX + s * plane;
-        const float y0 = -0.5f * g.ny;
-        for (int iy = 0; iy < g.ny; ++iy) {
-            const float yr = y0 + (float)iy;
-            const float xa = fmaf(B0, yr, A0), xb = fmaf(B1, yr, A1);
-            acc += seg_avg(R + (long)iy * g.nx, g.nx, fminf(xa, xb), fmaxf(xa, xb));
-        }
-        const float cuc = g.bcos[iu] * ct + g.bsin[iu] * st;
-        acc *= g.dy / fabsf(cuc);
-    } else {
-        yline(g, ct, st, g.esin[iu], g.ecos[iu], A0, B0, c0);
-        yline(g, ct, st, g.esin[iu + 1], g.ecos[iu + 1], A1, B1, c1);
-        const float* R = RY + s * plane;
-        const float x0 = -0.5f * g.nx;
-        for (int ix = 0; ix < g.nx; ++ix) {
-            const float xr = x0 + (float)ix;
-            const float ya = fmaf(B0, xr, A0), yb = fmaf(B1, xr, A1);
-            acc += seg_avg(R + (long)ix * g.ny, g.ny, fminf(ya, yb), fmaxf(ya, yb));
-        }
-        const float suc = g.bsin[iu] * ct - g.bcos[iu] * st;
-        acc *= g.dx / fabsf(suc);
-    }
-    prj[((long)s * nviews + vi) * g.nu + iu] = acc;
+    prj[((long)s * nviews + vi) * g.nu + iu] = dd_fp_ray(g, RX + s * plane, RY + s * plane, view0 + vi, iu);
 }
 
-// Siddon FP (exact ray / pixel intersection lengths), one line per image row
-// (X views) or column (Y views): the ray crosses row j over Y in [j, j+1],
-// i.e. over X in [X(j), X(j+1)], with path length dy / |cos(a - t)|; the
-// length inside each pixel is that times the pixel's share of the X interval.
 // With meas != nullptr the epilogue writes the weighted residual
 // wgt * (Ax - meas) of the OS-SQS data step (ossqs.cu:101-122) instead of Ax.
 template <class T>
@@ -150,31 +53,8 @@ __global__ __launch_bounds__(128) void k_siddon_fan_fp(FanK g, const T* __restri
     const int s = blockIdx.z;
     if (iu >= g.nu) return;
     const int view = view0 + vi;
-    const float ct = g.vcos[view], st = g.vsin[view];
     const long plane = (long)g.nx * g.ny;
-    T acc = T(0);
-    float A, B, c;
-    if (g.vcls[view] == 0) {
-        xline(g, ct, st, g.bsin[iu], g.bcos[iu], A, B, c);
-        const T* P = Pyx + s * plane;
-        const float y0 = -0.5f * g.ny;
-        for (int j = 0; j < g.ny; ++j) {
-            const float yr = y0 + (float)j;
-            const float xa = fmaf(B, yr, A), xb = fmaf(B, yr + 1.0f, A);
-            acc += seg_avg<T>(P + (long)j * g.nx, g.nx, fminf(xa, xb), fmaxf(xa, xb));
-        }
-        acc *= (T)(g.dy / fabsf(c));
-    } else {
-        yline(g, ct, st, g.bsin[iu], g.bcos[iu], A, B, c);
-        const T* P = Pxy + s * plane;
-        const float x0 = -0.5f * g.nx;
-        for (int i = 0; i < g.nx; ++i) {
-            const float xr = x0 + (float)i;
-            const float ya = fmaf(B, xr, A), yb = fmaf(B, xr + 1.0f, A);
-            acc += seg_avg<T>(P + (long)i * g.ny, g.ny, fminf(ya, yb), fmaxf(ya, yb));
-        }
-        acc *= (T)(g.dx / fabsf(c));
-    }
+    T acc = siddon_fp_ray<T>(g, Pyx + s * plane, Pxy + s * plane, view, iu);
     if (meas != nullptr) {
         const long mi = ((long)s * meas_nviews + view) * g.nu + iu;
         acc = (T)wgt[mi] * (acc - (T)meas[mi]);
@@ -186,7 +66,6 @@ __global__ __launch_bounds__(128) void k_siddon_fan_fp(FanK g, const T* __restri
 // backprojectors (pixel driven gathers)
 // ---------------------------------------------------------------------------
 
-// Exact transpose of k_siddon_fan_fp.
 template <class T>
 __global__ __launch_bounds__(256) void k_siddon_fan_bp(FanK g, const T* __restrict__ prj,
                                                        T* __restrict__ img, int view0, int nviews, int accumulate)
@@ -195,45 +74,11 @@ __global__ __launch_bounds__(256) void k_siddon_fan_bp(FanK g, const T* __restri
     const int ix = blockIdx.y * 16 + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
-    const float xc = (ix - 0.5f * (g.nx - 1)) * g.dx, yc = (iy - 0.5f * (g.ny - 1)) * g.dy;
-    const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
-    const float inv_da = 1.0f / g.da;
-    const float yr = (float)iy - 0.5f * g.ny;
-    const float xr = (float)ix - 0.5f * g.nx;
-    T acc = T(0);
-    for (int vi = 0; vi < nviews; ++vi) {
-        const int view = view0 + vi;
-        const float ct = g.vcos[view], st = g.vsin[view];
-        const float rx = xc * ct + yc * st, ry = -xc * st + yc * ct;
-        const float d0 = ry + g.dso;
-        const float uc = g.cu - g.off_a - atanf(rx / d0) * inv_da;
-        const float hw = half_width_bins(rpix, sqrtf(rx * rx + d0 * d0), inv_da);
-        const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
-        const T* p = prj + ((long)s * nviews + vi) * g.nu;
-        if (g.vcls[view] == 0) {
-            for (int k = k0; k <= k1; ++k) {
-                float A, B, c;
-                xline(g, ct, st, g.bsin[k], g.bcos[k], A, B, c);
-                const float xa = fmaf(B, yr, A), xb = fmaf(B, yr + 1.0f, A);
-                const float wgt = seg_weight(ix, fminf(xa, xb), fmaxf(xa, xb));
-                acc += p[k] * (T)(g.dy / fabsf(c) * wgt);
-            }
-        } else {
-            for (int k = k0; k <= k1; ++k) {
-                float A, B, c;
-                yline(g, ct, st, g.bsin[k], g.bcos[k], A, B, c);
-                const float ya = fmaf(B, xr, A), yb = fmaf(B, xr + 1.0f, A);
-                const float wgt = seg_weight(iy, fminf(ya, yb), fmaxf(ya, yb));
-                acc += p[k] * (T)(g.dx / fabsf(c) * wgt);
-            }
-        }
-    }
+    const T acc = siddon_bp_pixel<T>(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
     T* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
     *o = accumulate ? *o + acc : acc;
 }
 
-// Exact transpose of k_dd_fan_fp (no reference counterpart: the reference DD
-// BP is not the adjoint of its FP).
 __global__ __launch_bounds__(256) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
                                                     float* __restrict__ img, int view0, int nviews, int accumulate)
 {
@@ -241,48 +86,7 @@ __global__ __launch_bounds__(256) void k_dd_fan_adj(FanK g, const float* __restr
     const int ix = blockIdx.y * 16 + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
-    const float inv_da = 1.0f / g.da;
-    float acc = 0.0f;
-    for (int vi = 0; vi < nviews; ++vi) {
-        const int view = view0 + vi;
-        const float ct = g.vcos[view], st = g.vsin[view];
-        const float* p = prj + ((long)s * nviews + vi) * g.nu;
-        const bool isx = g.vcls[view] == 0;
-        // the pixel enters line (iy or ix) and line (iy+1 or ix+1) with weight 1/2
-        const int l0 = isx ? iy : ix;
-        const int nl = isx ? g.ny : g.nx;
-        const int cell = isx ? ix : iy;
-        for (int l = l0; l <= l0 + 1 && l < nl; ++l) {
-            // centre of the pixel's segment on this line, physical coordinates
-            float px, py, r;
-            if (isx) { px = (ix + 0.5f - 0.5f * g.nx) * g.dx; py = (l - 0.5f * g.ny) * g.dy; r = 0.5f * g.dx; }
-            else     { px = (l - 0.5f * g.nx) * g.dx; py = (iy + 0.5f - 0.5f * g.ny) * g.dy; r = 0.5f * g.dy; }
-            const float rx = px * ct + py * st, ry = -px * st + py * ct;
-            const float d0 = ry + g.dso;
-            const float uc = g.cu - g.off_a - atanf(rx / d0) * inv_da;
-            // bins whose footprint (centre +- half a bin) meets the segment
-            const float hw = half_width_bins(r, sqrtf(rx * rx + d0 * d0), inv_da) + 0.5f;
-            const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
-            if (k1 < k0) continue;
-            const float lr = isx ? (float)l - 0.5f * g.ny : (float)l - 0.5f * g.nx;
-            float A, B, c;
-            float e_prev;
-            if (isx) { xline(g, ct, st, g.esin[k0], g.ecos[k0], A, B, c); }
-            else     { yline(g, ct, st, g.esin[k0], g.ecos[k0], A, B, c); }
-            e_prev = fmaf(B, lr, A);
-            for (int k = k0; k <= k1; ++k) {
-                if (isx) xline(g, ct, st, g.esin[k + 1], g.ecos[k + 1], A, B, c);
-                else     yline(g, ct, st, g.esin[k + 1], g.ecos[k + 1], A, B, c);
-                const float e_next = fmaf(B, lr, A);
-                const float wgt = seg_weight(cell, fminf(e_prev, e_next), fmaxf(e_prev, e_next));
-                float L;
-                if (isx) L = g.dy / fabsf(g.bcos[k] * ct + g.bsin[k] * st);
-                else     L = g.dx / fabsf(g.bsin[k] * ct - g.bcos[k] * st);
-                acc = fmaf(0.5f * L * p[k], wgt, acc);
-                e_prev = e_next;
-            }
-        }
-    }
+    const float acc = dd_adj_pixel(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
     float* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
     *o = accumulate ? *o + acc : acc;
 }
