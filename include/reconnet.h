@@ -239,6 +239,17 @@ int rn_ragged_forward(void* batch, int kind /* RN_FP_DD | RN_FP_SIDDON */, const
 int rn_ragged_backward(void* batch, int kind /* RN_BP_DD_ADJOINT | RN_BP_SIDDON */, const float* prj, float* img,
         void* stream);
 
+/* Launch-shape autotuner (no reference counterpart).  On the first launch of
+ * a shape key the projector launchers time their candidate block / tile shapes
+ * on the caller's stream and keep the fastest for the process; the choice
+ * never changes results.  Enabled unless RECONNET_AUTOTUNE=0 or switched off
+ * here.  rn_autotune_report writes "kernel a b c d e -> choice (ms ...)"
+ * lines into buf (NUL-terminated, truncated to len) and returns the full
+ * length. */
+int rn_autotune_enable(int on);
+int rn_autotune_report(char* buf, int len);
+int rn_autotune_reset(void);
+
 #ifdef __cplusplus
 }
 #endif

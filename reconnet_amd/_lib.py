@@ -58,6 +58,9 @@ def lib():
     _sig(h, "rn_sqs_set_profiling", c_int, [_P, c_int])
     _sig(h, "rn_sqs_kernel_times", c_int, [_P, _P, _P, _P, _P])
     _sig(h, "rn_tv_sqs", c_int, [_P, _P, _P, c_int, c_int, c_int, c_float, _P, _P])
+    _sig(h, "rn_autotune_enable", c_int, [c_int])
+    _sig(h, "rn_autotune_report", c_int, [ctypes.c_char_p, c_int])
+    _sig(h, "rn_autotune_reset", c_int, [])
     h.cTVSQS2D.restype = c_float
     _lib = h
     return _lib
@@ -72,3 +75,25 @@ def check(status, what):
     """Raise RuntimeError if a status-returning entry point failed."""
     if status != 0:
         raise RuntimeError("%s failed: %s" % (what, last_error()))
+
+
+def autotune(enable=None, reset=False):
+    """Switch the launch-shape autotuner on/off, clear its cache, and return
+    its current choices as {(kernel, a, b, c, d, e): (choice, [ms per candidate])}."""
+    h = lib()
+    if enable is not None:
+        check(h.rn_autotune_enable(1 if enable else 0), "rn_autotune_enable")
+    if reset:
+        check(h.rn_autotune_reset(), "rn_autotune_reset")
+    n = h.rn_autotune_report(None, 0)
+    if n < 0:
+        check(n, "rn_autotune_report")
+    buf = ctypes.create_string_buffer(n + 1)
+    h.rn_autotune_report(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        key, rest = line.split(" -> ")
+        choice, ms = rest.split(" (")
+        out[tuple(int(v) for v in key.split())] = (int(choice), [float(v) for v in ms.rstrip(" ms)").split()])
+    return out
+

@@ -13,8 +13,11 @@
 #include "rn_fan_ops.h"
 #include "rn_filter.h"
 #include "rn_sqs.h"
+#include "rn_tune.h"
 
+#include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -504,6 +507,31 @@ extern "C" void cHelicalBPFromParallelConebeamBackprojection(float* pImg, const 
 // =====================================================================
 
 extern "C" const char* rn_last_error(void) { return last_error(); }
+
+extern "C" int rn_autotune_enable(int on)
+{
+    return guarded_status("rn_autotune_enable", [&] { tune_set_enabled(on != 0); });
+}
+
+extern "C" int rn_autotune_report(char* buf, int len)
+{
+    int n = 0;
+    const int st = guarded_status("rn_autotune_report", [&] {
+        const std::string r = tune_report();
+        n = (int)r.size();
+        if (buf && len > 0) {
+            const int m = std::min(n, len - 1);
+            std::memcpy(buf, r.data(), m);
+            buf[m] = 0;
+        }
+    });
+    return st != 0 ? st : n;
+}
+
+extern "C" int rn_autotune_reset(void)
+{
+    return guarded_status("rn_autotune_reset", [&] { tune_reset(); });
+}
 
 extern "C" void* rn_fan_plan_create(int nx, int ny, int nz, float dx, float dy, float dz,
         int nu, int nview, int nv, float da, float dv, float off_a, float off_v,

@@ -13,6 +13,7 @@
 // bpFanKernel3D (PixelDriven.cu:11-75).
 
 #include "rn_fan_device.h"
+#include "rn_tune.h"
 
 #include <cmath>
 #include <cstring>
@@ -27,7 +28,8 @@ using namespace fan;
 // forward projectors (line driven)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(128) void k_dd_fan_fp(FanK g, const float* __restrict__ RX,
+template <int BLK>
+__global__ __launch_bounds__(BLK) void k_dd_fan_fp(FanK g, const float* __restrict__ RX,
                                                    const float* __restrict__ RY, float* __restrict__ prj,
                                                    int view0, int nviews)
 {
@@ -41,8 +43,8 @@ __global__ __launch_bounds__(128) void k_dd_fan_fp(FanK g, const float* __restri
 
 // With meas != nullptr the epilogue writes the weighted residual
 // wgt * (Ax - meas) of the OS-SQS data step (ossqs.cu:101-122) instead of Ax.
-template <class T>
-__global__ __launch_bounds__(128) void k_siddon_fan_fp(FanK g, const T* __restrict__ Pyx,
+template <class T, int BLK>
+__global__ __launch_bounds__(BLK) void k_siddon_fan_fp(FanK g, const T* __restrict__ Pyx,
                                                        const T* __restrict__ Pxy, T* __restrict__ prj,
                                                        int view0, int nviews,
                                                        const float* __restrict__ meas, const float* __restrict__ wgt,
@@ -66,12 +68,12 @@ __global__ __launch_bounds__(128) void k_siddon_fan_fp(FanK g, const T* __restri
 // backprojectors (pixel driven gathers)
 // ---------------------------------------------------------------------------
 
-template <class T>
-__global__ __launch_bounds__(256) void k_siddon_fan_bp(FanK g, const T* __restrict__ prj,
+template <class T, int BX, int BY>
+__global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp(FanK g, const T* __restrict__ prj,
                                                        T* __restrict__ img, int view0, int nviews, int accumulate)
 {
-    const int iy = blockIdx.x * 16 + threadIdx.x;
-    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
     const T acc = siddon_bp_pixel<T>(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
@@ -79,11 +81,12 @@ __global__ __launch_bounds__(256) void k_siddon_fan_bp(FanK g, const T* __restri
     *o = accumulate ? *o + acc : acc;
 }
 
-__global__ __launch_bounds__(256) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
+template <int BX, int BY>
+__global__ __launch_bounds__(BX * BY) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
                                                     float* __restrict__ img, int view0, int nviews, int accumulate)
 {
-    const int iy = blockIdx.x * 16 + threadIdx.x;
-    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
     const float acc = dd_adj_pixel(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
@@ -108,11 +111,12 @@ __global__ void k_dd_preweight(FanK g, const float* __restrict__ prj, float* __r
 
 // Reference-parity DD BP (DDBPFanKernelX/Y): pixel edges projected through
 // atan, average of the (pre-weighted) detector row over the footprint.
-__global__ __launch_bounds__(256) void k_dd_fan_bp(FanK g, const float* __restrict__ wprj,
+template <int BX, int BY>
+__global__ __launch_bounds__(BX * BY) void k_dd_fan_bp(FanK g, const float* __restrict__ wprj,
                                                    float* __restrict__ img, int fbp)
 {
-    const int iy = blockIdx.x * 16 + threadIdx.x;
-    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
     const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
@@ -138,11 +142,12 @@ __global__ __launch_bounds__(256) void k_dd_fan_bp(FanK g, const float* __restri
 }
 
 // bpFanKernel3D with isFBP = true (the only mode the reference launches).
-__global__ __launch_bounds__(256) void k_pixel_fan_bp(FanK g, const float* __restrict__ prj,
+template <int BX, int BY>
+__global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* __restrict__ prj,
                                                       float* __restrict__ img)
 {
-    const int iy = blockIdx.x * 16 + threadIdx.x;
-    const int ix = blockIdx.y * 16 + threadIdx.y;
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
     const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
@@ -306,12 +311,70 @@ bool FanTables::matches(const FanGeo& g, const float* deg_host) const
     return std::memcmp(deg.data(), deg_host, sizeof(float) * g.nview) == 0;
 }
 
+// ---- launch shapes: candidates for the autotuner (identical results for every shape) ----
+
+namespace {
+
+template <int BLK>
+void go_dd_fp(const FanK& g, const float* RX, const float* RY, float* prj, int ns, int view0, int nviews,
+              hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dd_fan_fp<BLK>), dim3(cdiv(g.nu, BLK), nviews, ns), dim3(BLK), 0, st, g, RX, RY, prj,
+                       view0, nviews);
+}
+
+template <int BLK>
+void go_siddon_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj, int ns, int view0, int nviews,
+                  const float* meas, const float* w, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK>), dim3(cdiv(g.nu, BLK), nviews, ns), dim3(BLK), 0, st, g, Pyx,
+                       Pxy, prj, view0, nviews, meas, w, g.nview);
+}
+
+template <int BX, int BY>
+void go_siddon_bp(const FanK& g, const float* prj, float* img, int ns, int view0, int nviews, int acc,
+                  hipStream_t st)
+{
+    hipLaunchKernelGGL((k_siddon_fan_bp<float, BX, BY>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0,
+                       st, g, prj, img, view0, nviews, acc);
+}
+
+template <int BX, int BY>
+void go_dd_adj(const FanK& g, const float* prj, float* img, int ns, int view0, int nviews, int acc, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dd_fan_adj<BX, BY>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0, st, g,
+                       prj, img, view0, nviews, acc);
+}
+
+template <int BX, int BY>
+void go_dd_bp(const FanK& g, const float* wprj, float* img, int ns, int fbp, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dd_fan_bp<BX, BY>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0, st, g,
+                       wprj, img, fbp);
+}
+
+using DdFpFn = void (*)(const FanK&, const float*, const float*, float*, int, int, int, hipStream_t);
+using SidFpFn = void (*)(const FanK&, const float*, const float*, float*, int, int, int, const float*, const float*,
+                         hipStream_t);
+using GatherFn = void (*)(const FanK&, const float*, float*, int, int, int, int, hipStream_t);
+using DdBpFn = void (*)(const FanK&, const float*, float*, int, int, hipStream_t);
+
+// 1-D blocks along the detector; 2-D pixel tiles (x lanes along y, the contiguous image axis)
+const DdFpFn kDdFp[] = {go_dd_fp<128>, go_dd_fp<64>, go_dd_fp<256>};
+const SidFpFn kSidFp[] = {go_siddon_fp<128>, go_siddon_fp<64>, go_siddon_fp<256>};
+const GatherFn kSidBp[] = {go_siddon_bp<16, 16>, go_siddon_bp<8, 32>, go_siddon_bp<32, 8>, go_siddon_bp<8, 8>};
+const GatherFn kDdAdj[] = {go_dd_adj<16, 16>, go_dd_adj<8, 32>, go_dd_adj<32, 8>, go_dd_adj<8, 8>};
+const DdBpFn kDdBp[] = {go_dd_bp<16, 16>, go_dd_bp<8, 32>, go_dd_bp<32, 8>, go_dd_bp<8, 8>};
+
+} // namespace
+
 void launch_dd_fan_fp(const FanK& g, const float* RX, const float* RY, float* prj_vu,
                       int nslices, int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
-    hipLaunchKernelGGL(k_dd_fan_fp, grd, blk, 0, st, g, RX, RY, prj_vu, view0, nviews);
+    const int c = tune_choose({TK_DD_FP, g.nx, g.ny, g.nu, nviews, nslices}, 3,
+                              [&](int i) { kDdFp[i](g, RX, RY, prj_vu, nslices, view0, nviews, st); }, st, true);
+    kDdFp[c](g, RX, RY, prj_vu, nslices, view0, nviews, st);
     check_launch("dd_fan_fp");
 }
 
@@ -319,9 +382,10 @@ void launch_siddon_fan_fp(const FanK& g, const float* Pyx, const float* Pxy, flo
                           int nslices, int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
-    hipLaunchKernelGGL(k_siddon_fan_fp<float>, grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
-                       (const float*)nullptr, (const float*)nullptr, 0);
+    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 3, [&](int i) {
+        kSidFp[i](g, Pyx, Pxy, prj_vu, nslices, view0, nviews, nullptr, nullptr, st);
+    }, st, true);
+    kSidFp[c](g, Pyx, Pxy, prj_vu, nslices, view0, nviews, nullptr, nullptr, st);
     check_launch("siddon_fan_fp");
 }
 
@@ -330,9 +394,10 @@ void launch_siddon_fan_fp_residual(const FanK& g, const float* Pyx, const float*
                                    hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
-    hipLaunchKernelGGL(k_siddon_fan_fp<float>, grd, blk, 0, st, g, Pyx, Pxy, res_vu, view0, nviews,
-                       meas_vu, w_vu, g.nview);
+    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 3, [&](int i) {
+        kSidFp[i](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
+    }, st, true);
+    kSidFp[c](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
     check_launch("siddon_fan_fp_residual");
 }
 
@@ -341,7 +406,7 @@ void launch_siddon_fan_fp_f64(const FanK& g, const double* Pyx, const double* Px
 {
     if (nslices <= 0 || nviews <= 0) return;
     dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
-    hipLaunchKernelGGL(k_siddon_fan_fp<double>, grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
+    hipLaunchKernelGGL((k_siddon_fan_fp<double, 128>), grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
                        (const float*)nullptr, (const float*)nullptr, 0);
     check_launch("siddon_fan_fp_f64");
 }
@@ -351,7 +416,7 @@ void launch_siddon_fan_bp_f64(const FanK& g, const double* prj_vu, double* img_x
 {
     if (nslices <= 0) return;
     dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL(k_siddon_fan_bp<double>, grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, 0);
+    hipLaunchKernelGGL((k_siddon_fan_bp<double, 16, 16>), grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, 0);
     check_launch("siddon_fan_bp_f64");
 }
 
@@ -359,8 +424,10 @@ void launch_siddon_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int
                           int view0, int nviews, bool accumulate, hipStream_t st)
 {
     if (nslices <= 0) return;
-    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL(k_siddon_fan_bp<float>, grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, accumulate ? 1 : 0);
+    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, 4, [&](int i) {
+        kSidBp[i](g, prj_vu, img_xy, nslices, view0, nviews, 0, st);
+    }, st, !accumulate);
+    kSidBp[c](g, prj_vu, img_xy, nslices, view0, nviews, accumulate ? 1 : 0, st);
     check_launch("siddon_fan_bp");
 }
 
@@ -368,8 +435,10 @@ void launch_dd_fan_adj(const FanK& g, const float* prj_vu, float* img_xy, int ns
                        int view0, int nviews, bool accumulate, hipStream_t st)
 {
     if (nslices <= 0) return;
-    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL(k_dd_fan_adj, grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, accumulate ? 1 : 0);
+    const int c = tune_choose({TK_DD_ADJ, g.nx, g.ny, g.nu, nviews, nslices}, 4, [&](int i) {
+        kDdAdj[i](g, prj_vu, img_xy, nslices, view0, nviews, 0, st);
+    }, st, !accumulate);
+    kDdAdj[c](g, prj_vu, img_xy, nslices, view0, nviews, accumulate ? 1 : 0, st);
     check_launch("dd_fan_adj");
 }
 
@@ -384,8 +453,9 @@ void launch_dd_preweight(const FanK& g, const float* prj_vu, float* wprj_vu, int
 void launch_dd_fan_bp(const FanK& g, const float* wprj_vu, float* img_xy, int nslices, bool fbp, hipStream_t st)
 {
     if (nslices <= 0) return;
-    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL(k_dd_fan_bp, grd, blk, 0, st, g, wprj_vu, img_xy, fbp ? 1 : 0);
+    const int c = tune_choose({fbp ? TK_DD_FBP : TK_DD_BP, g.nx, g.ny, g.nu, g.nview, nslices}, 4,
+                              [&](int i) { kDdBp[i](g, wprj_vu, img_xy, nslices, fbp ? 1 : 0, st); }, st, true);
+    kDdBp[c](g, wprj_vu, img_xy, nslices, fbp ? 1 : 0, st);
     check_launch("dd_fan_bp");
 }
 
@@ -393,7 +463,7 @@ void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int 
 {
     if (nslices <= 0) return;
     dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL(k_pixel_fan_bp, grd, blk, 0, st, g, prj_vu, img_xy);
+    hipLaunchKernelGGL((k_pixel_fan_bp<16, 16>), grd, blk, 0, st, g, prj_vu, img_xy);
     check_launch("pixel_fan_bp");
 }
 

@@ -1,0 +1,52 @@
+// Launch-shape autotuner (no reference counterpart).
+//
+// A tunable launch names its kernel, a shape key (the sizes that change the
+// best configuration) and a list of candidate configurations.  On the first
+// launch of a key the tuner times every candidate with HIP events on the
+// caller's stream (one warm-up and three timed runs each) and keeps the
+// fastest; later launches of the key reuse it.  Candidates differ only in the
+// block / tile shape of the launch: every output element is computed by the
+// same per-element code, so the choice never changes the results.
+//
+// Tuning is skipped (the first candidate, or the cached choice, is used) when
+// the stream is being captured into a graph, when the launch accumulates into
+// its output, or when RECONNET_AUTOTUNE=0.
+#pragma once
+
+#include "rn_common.h"
+
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace rn {
+
+struct TuneKey {
+    int kernel;
+    int a, b, c, d, e;
+    bool operator<(const TuneKey& o) const;
+};
+
+enum TuneKernel {
+    TK_SIDDON_FP = 1,
+    TK_SIDDON_BP = 2,
+    TK_DD_FP = 3,
+    TK_DD_ADJ = 4,
+    TK_DD_BP = 5,
+    TK_SQS_UPDATE = 6,
+    TK_DD_FBP = 7,
+};
+
+// Returns the candidate index to launch for `key`; runs the timing when the
+// key is new and tuning is allowed.  `launch(i)` must enqueue candidate i on
+// `st` and be repeatable (its output overwritten, not accumulated).
+int tune_choose(const TuneKey& key, int ncand, const std::function<void(int)>& launch, hipStream_t st,
+                bool may_tune);
+
+void tune_set_enabled(bool on);
+bool tune_enabled();
+// "kernel a b c d e -> choice (ms per candidate ...)" lines for every tuned key
+std::string tune_report();
+void tune_reset();
+
+} // namespace rn

@@ -109,3 +109,37 @@ class OsSqsTV:
             a = np.frombuffer(buf, dtype=np.float64).reshape(self.nslices, 2)
             return a[:, 0].copy(), a[:, 1].copy()
         return None
+
+    def capture_passes(self, npasses, beta=0.0, eps=1e-6, lb=None, ub=None, lam=1.0):
+        """Record ``npasses`` full passes into a HIP graph (see PassGraph)."""
+        return PassGraph(self, npasses, beta, eps, lb, ub, lam)
+
+
+class PassGraph:
+    """npasses OS-SQS(-TV) passes replayed as one HIP graph on the engine's
+    current image.  The engine ping-pongs between two image buffers once per
+    subset step; the graph is only valid when the recorded step count is even,
+    so that every replay starts from the buffer the host side points to."""
+
+    def __init__(self, eng, npasses, beta, eps, lb, ub, lam):
+        steps = -(-eng.geo.nview // eng.npers) * int(npasses)
+        if steps % 2:
+            raise ValueError("a pass graph needs an even number of subset steps (got %d)" % steps)
+        self.eng = eng
+        x_keep = eng.image()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            eng.run_pass(beta, eps, lb, ub, lam)   # warm-up outside the capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(int(npasses)):
+                eng.run_pass(beta, eps, lb, ub, lam)
+        torch.cuda.synchronize()
+        eng.set_image(x_keep)          # undo the warm-up pass (capturing runs nothing)
+
+    def replay(self):
+        self.graph.replay()
+

@@ -16,7 +16,7 @@ import torch
 
 from .cnn import PriorCNN
 
-__all__ = ["UnrolledOSSQS", "WEIGHTS_PATH", "load_nets", "save_nets"]
+__all__ = ["UnrolledOSSQS", "UnrolledGraph", "WEIGHTS_PATH", "load_nets", "save_nets"]
 
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unrolled_fan512.npz")
 
@@ -67,3 +67,37 @@ class UnrolledOSSQS:
                 x = torch.cat([net.forward(x[i:i + chunk]) for i in range(0, eng.nslices, chunk)])
             eng.set_image(x)
         return x
+
+
+class UnrolledGraph:
+    """The whole unrolled reconstruction (image upload, every data pass and
+    every CNN prior, image download) recorded once into a HIP graph and
+    replayed per batch.
+
+    The engine's sinograms, weights and normaliser are referenced by device
+    address, so new slices of the same shape are swapped in with
+    ``eng.set_data`` / ``eng.set_norm`` between replays.  ``x0`` is copied into
+    the graph's own input buffer; the returned tensor is the graph's output
+    buffer and is overwritten by the next replay."""
+
+    def __init__(self, model, eng, x0=None):
+        g = eng.geo
+        self.x0 = torch.zeros((eng.nslices, g.nx, g.ny), dtype=torch.float32, device="cuda")
+        if x0 is not None:
+            self.x0.copy_(x0)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            model.reconstruct(eng, self.x0)       # warm-up outside the capture (workspaces, handles)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = model.reconstruct(eng, self.x0)
+        torch.cuda.synchronize()
+
+    def replay(self, x0=None):
+        if x0 is not None:
+            self.x0.copy_(x0)
+        self.graph.replay()
+        return self.out

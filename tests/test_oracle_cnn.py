@@ -51,3 +51,52 @@ def test_gradients_match_finite_differences(name):
         q[name].reshape(-1)[i] -= 2 * h
         fm = np.sum(c * cnn.forward(x, q)[0])
         assert (fp - fm) / (2 * h) == pytest.approx(np.asarray(g).reshape(-1)[i], rel=1e-5, abs=1e-8)
+
+
+def test_bf16_round_matches_torch_conversion():
+    import torch
+
+    rng = np.random.default_rng(5)
+    v = np.concatenate([rng.normal(size=4000) * 10.0 ** rng.integers(-6, 6, 4000),
+                        # exact ties between two BF16 values (odd and even mantissa)
+                        (np.arange(1, 200) * 2 + 1).astype(np.float64) * 2.0 ** -8 + 1.0]).astype(np.float32)
+    ref = torch.from_numpy(v).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(cnn.bf16_round(v), ref)
+
+
+def _sparse_int_params(seed):
+    rng = np.random.default_rng(seed)
+    p = {"w1": rng.integers(-1, 2, (9, 32)).astype(np.float64), "b1": rng.integers(0, 2, 32).astype(np.float64)}
+    for k in (2, 3, 4):
+        p["w%d" % k] = (rng.integers(-1, 2, (9, 32, 32)) * (rng.uniform(size=(9, 32, 32)) < 0.05)).astype(np.float64)
+        p["b%d" % k] = rng.integers(0, 2, 32).astype(np.float64)
+    p["w5"] = rng.integers(-1, 2, (9, 32)).astype(np.float64)
+    p["b5"] = np.array(1.0)
+    return p
+
+
+def test_bf16_variant_equals_fp64_when_nothing_rounds():
+    # small integers everywhere: every stored value is BF16-exact, so the two
+    # variants must agree exactly, values and gradients
+    p = _sparse_int_params(6)
+    rng = np.random.default_rng(7)
+    x = rng.integers(0, 3, (1, 10, 12)).astype(np.float64)
+    c = rng.integers(-1, 2, (1, 10, 12)).astype(np.float64)
+    y, acts = cnn.forward(x, p)
+    yb, actsb = cnn.forward_bf16(x, p)
+    assert max(np.abs(a).max() for a in acts) <= 256
+    assert np.array_equal(y, yb)
+    g = cnn.backward(c, p, acts)
+    gb = cnn.backward_bf16(c, p, actsb)
+    for k in g:
+        assert np.array_equal(g[k], gb[k]), k
+
+
+def test_bf16_variant_rounds_activations():
+    p = cnn.init_params(8)
+    x = np.random.default_rng(9).uniform(size=(1, 6, 6))
+    _, acts = cnn.forward_bf16(x, p)
+    for a in acts[1:]:
+        assert np.array_equal(a, cnn.bf16_round(a))
+    _, acts64 = cnn.forward(x, p)
+    assert not np.array_equal(acts[2], acts64[2])
