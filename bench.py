@@ -6,18 +6,24 @@ learned unrolled OS-SQS against plain SQS-TV on the same GPU.
 
 Workload: 32 fan-beam slices of 512^2 at 0.7 mm, 1152 views over 2 pi, 736
 bins of 1.0 mm, SID 595 mm, SDD 1085.6 mm, 12 ordered subsets, Poisson noise.
-Synthetic data: seeded random-ellipse phantoms (held-out seeds 100000+) projected
-by the library's Siddon projector.
+Synthetic data: seeded random-ellipse phantoms projected by the library's
+Siddon projector; evaluation seeds 100000 .. 100199 (held out from training).
 
-1. Plain SQS-TV (beta = 1e-4, delta = 1e-3) is run to convergence, untimed:
-   k_conv is the first iteration whose RMSE is within 1 % of the RMSE after
-   k_max iterations and whose relative update ||dx||/||x|| is below 1e-4;
-   R* = RMSE(k_conv).  Its throughput, 32 / t(k_conv passes), is then timed.
-2. With trained weights (reconnet_amd/weights/unrolled_fan512.npz) a timed step
-   is one 10-iteration learned reconstruction of the batch (10 OS-SQS data
-   passes + 10 CNN priors on BF16 MFMA), at the noise level it was trained for;
-   value = its slices/s, valid when its RMSE <= R*.  Without weights the step is
-   the plain reconstruction and value = plain slices/s.
+Both methods start from the FBP image (Hann-windowed ramp filter, FBP-weighted
+DD backprojection); the FBP is part of every timed step.
+
+1. Plain SQS-TV (beta = 1e-4, delta = 1e-3) runs k_max passes on the 200
+   evaluation slices (untimed).  k_conv is the first iteration whose mean RMSE
+   is within 1 % of the mean RMSE after k_max passes and whose relative update
+   ||dx||/||x|| is below 1e-4; R* = mean RMSE(k_conv).
+2. A timed plain step is FBP + k_conv passes over the 32-slice batch (the
+   passes replayed from one HIP graph).
+3. With trained weights (reconnet_amd/weights/unrolled_fan512.npz, used at the
+   dose they were trained for) a timed step is FBP + the 10-iteration learned
+   reconstruction of the batch (10 OS-SQS data passes + 10 CNN priors on BF16
+   MFMA, replayed from one HIP graph); value = its slices/s, valid when its
+   mean RMSE over the 200 evaluation slices is <= R*.  Without weights the
+   headline step is the plain one.
 
 Inputs are resident in HBM before timing.  Prints one JSON line.
 """
@@ -47,7 +53,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slices", type=int, default=32)
-    ap.add_argument("--kmax", type=int, default=100)
+    ap.add_argument("--kmax", type=int, default=200)
+    ap.add_argument("--eval", type=int, default=200, help="held-out slices for k_conv, R* and the learned RMSE")
+    ap.add_argument("--no-graph", action="store_true", help="launch the timed passes eagerly")
+    ap.add_argument("--probe", action="store_true",
+                    help="profiling probe: FBP + 2 eager OS-SQS passes over the batch, then a 256 MiB device "
+                         "copy (the reference kernel for counter scaling); no timing line")
     ap.add_argument("--subsets", type=int, default=12)
     ap.add_argument("--beta", type=float, default=1e-4)
     ap.add_argument("--delta", type=float, default=1e-3)
@@ -98,7 +109,7 @@ def main():
     from reconnet_amd.phantom import add_poisson_noise, phantom_batch
     from reconnet_amd.sqs import OsSqsTV
     from reconnet_amd.torch_ops import FanPlan
-    from reconnet_amd.unrolled import WEIGHTS_PATH, UnrolledOSSQS, load_nets
+    from reconnet_amd.unrolled import WEIGHTS_PATH, UnrolledGraph, UnrolledOSSQS, load_nets
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
@@ -118,37 +129,56 @@ def main():
     log("workload: %d slices %dx%d, %d views, %d bins, I0 %g, learned=%s"
         % (ns, geo.nx, geo.ny, geo.nview, geo.nu, i0, nets is not None))
 
-    seeds = list(range(100000, 100000 + ns))
-    truth = phantom_batch(seeds, geo.nx, geo.ny, geo.dx, geo.dy, device=dev).contiguous()
+    from reconnet_amd.sqs import fbp_initial_image
+
     plan = FanPlan(geo)
-    clean = plan.forward(truth, "siddon")
-    noisy, w = add_poisson_noise(clean, i0, seed=2024)
-    del clean
     mask = torch.from_numpy(geo.fov_mask()).to(dev).float()
     npix = float(mask.sum())
+    sqs_kw = dict(beta=args.beta, eps=args.delta ** 2, lb=0.0)
 
-    def rmse_hu(x):
+    def rmse_hu(x, truth):
         d = (x - truth) * mask
         return (1000.0 / HU_WATER) * torch.sqrt((d * d).sum(dim=(1, 2)) / npix)
 
-    eng = OsSqsTV(geo, args.subsets, ns)
-    eng.set_data(noisy, w)
-    eng.compute_norm()
-    zeros = torch.zeros((ns, geo.nx, geo.ny), dtype=torch.float32, device=dev)
-    sqs_kw = dict(beta=args.beta, eps=args.delta ** 2, lb=0.0)
+    def simulate(seeds):
+        truth = phantom_batch(seeds, geo.nx, geo.ny, geo.dx, geo.dy, device=dev).contiguous()
+        noisy, w = add_poisson_noise(plan.forward(truth, "siddon"), i0, seed=2024 + seeds[0])
+        return truth, noisy, w
 
-    # ---- plain SQS-TV to convergence (untimed) ----
-    eng.set_image(zeros)
+    if args.probe:
+        truth, noisy, w = simulate(list(range(100000, 100000 + ns)))
+        eng = OsSqsTV(geo, args.subsets, ns)
+        eng.set_data(noisy, w)
+        eng.compute_norm()
+        eng.set_image(fbp_initial_image(plan, noisy))
+        for _ in range(2):
+            eng.run_pass(**sqs_kw)
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("probe done: 2 passes over %d slices, one copy of %d bytes" % (ns, src.numel() * 4))
+        return
+
+    # ---- plain SQS-TV to convergence on the evaluation slices (untimed) ----
+    ne = max(args.eval, ns)
+    etruth, enoisy, ew = simulate(list(range(100000, 100000 + ne)))
+    eeng = OsSqsTV(geo, args.subsets, ne)
+    eeng.set_data(enoisy, ew)
+    eeng.compute_norm()
+    ex0 = fbp_initial_image(plan, enoisy)
+    fbp_rmse = float(rmse_hu(ex0, etruth).mean())
+    eeng.set_image(ex0)
     rm, upd = [], []
     t0 = time.perf_counter()
     for k in range(args.kmax):
-        dd, xx = eng.run_pass(track=True, **sqs_kw)
+        dd, xx = eeng.run_pass(track=True, **sqs_kw)
         upd.append(float(np.sqrt(dd.sum() / max(xx.sum(), 1e-30))))
-        rm.append(rmse_hu(eng.image()).cpu().numpy())
+        rm.append(rmse_hu(eeng.image(), etruth).cpu().numpy())
         if (k + 1) % 20 == 0:
             log("plain pass %d: mean RMSE %.2f HU, update %.2e" % (k + 1, rm[-1].mean(), upd[-1]))
     torch.cuda.synchronize()
-    log("convergence run %.1f s" % (time.perf_counter() - t0))
+    log("convergence run on %d slices: %.1f s (FBP start %.2f HU)" % (ne, time.perf_counter() - t0, fbp_rmse))
     rm = np.array(rm)
     r_final = float(rm[-1].mean())
     k_conv = args.kmax
@@ -156,14 +186,28 @@ def main():
         if rm[k].mean() <= 1.01 * r_final and upd[k] < 1e-4:
             k_conv = k + 1
             break
-    r_star = float(rm[k_conv - 1].mean())
+    k_conv += k_conv % 2 * (args.subsets % 2)     # pass graphs need an even subset-step count
+    r_star = float(rm[min(k_conv, args.kmax) - 1].mean())
     r_best = float(rm.mean(axis=1).min())
-    log("k_conv = %d, R* = %.3f HU (k_max %.3f HU, best %.3f HU)" % (k_conv, r_star, r_final, r_best))
+    log("k_conv = %d, R* = %.3f HU (k_max %.3f HU, best %.3f HU at pass %d)"
+        % (k_conv, r_star, r_final, r_best, int(rm.mean(axis=1).argmin()) + 1))
 
-    def plain_step():
-        eng.set_image(zeros)
-        for _ in range(k_conv):
-            eng.run_pass(**sqs_kw)
+    learned_eval = None
+    if nets is not None:
+        model = UnrolledOSSQS(nets, lb=0.0)
+        xl = model.reconstruct(eeng, ex0, chunk=32)
+        l_all = rmse_hu(xl, etruth).cpu().numpy()
+        learned_eval = float(l_all.mean())
+        log("learned on %d evaluation slices: mean RMSE %.3f HU (R* %.3f)" % (ne, learned_eval, r_star))
+        del xl
+    del eeng, enoisy, ew, ex0, etruth
+
+    # ---- timed workload: the first 32 evaluation slices ----
+    truth, noisy, w = simulate(list(range(100000, 100000 + ns)))
+    eng = OsSqsTV(geo, args.subsets, ns)
+    eng.set_data(noisy, w)
+    eng.compute_norm()
+    x0 = torch.empty((ns, geo.nx, geo.ny), dtype=torch.float32, device=dev)
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -176,30 +220,40 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
-    # plain throughput (1 step; the headline step when no weights are present)
-    eng.set_profiling(True)
+    fbp_initial_image(plan, noisy, out=x0)
+    eng.set_image(x0)
+    pgraph = None if args.no_graph else eng.capture_passes(k_conv, **sqs_kw)
+
+    def plain_step():
+        fbp_initial_image(plan, noisy, out=x0)
+        eng.set_image(x0)
+        if pgraph is not None:
+            pgraph.replay()
+        else:
+            for _ in range(k_conv):
+                eng.run_pass(**sqs_kw)
+
     plain_steps = args.steps if nets is None else 1
     t_plain = timed(plain_step, plain_steps, args.warmup if nets is None else 0)
-    kt = eng.kernel_times()
-    eng.set_profiling(False)
     plain_sps = ns * plain_steps / t_plain
-    plain_rmse = float(rmse_hu(eng.image()).mean())
-    log("plain SQS-TV: %.2f slices/s (%d passes), RMSE %.3f HU" % (plain_sps, k_conv, plain_rmse))
+    plain_rmse = float(rmse_hu(eng.image(), truth).mean())
+    log("plain SQS-TV: %.2f slices/s (FBP + %d passes), batch RMSE %.3f HU" % (plain_sps, k_conv, plain_rmse))
+    del pgraph
 
     conv_ms = None
     learned = None
     if nets is not None:
-        model = UnrolledOSSQS(nets, lb=0.0)
+        ugraph = None if args.no_graph else UnrolledGraph(model, eng, x0)
 
         def learned_step():
-            model.reconstruct(eng, zeros)
+            fbp_initial_image(plan, noisy, out=x0)
+            if ugraph is not None:
+                return ugraph.replay(x0)
+            return model.reconstruct(eng, x0)
 
-        eng.set_profiling(True)
         elapsed = timed(learned_step, args.steps, args.warmup)
-        kt = eng.kernel_times()
-        eng.set_profiling(False)
-        x = eng.image()
-        l_rmse = float(rmse_hu(x).mean())
+        x = learned_step()
+        l_rmse = float(rmse_hu(x, truth).mean())
         value = ns * args.steps / elapsed
         ms_per_step = 1000.0 * elapsed / args.steps
         # CNN cost (events on the current stream around one prior application)
@@ -211,23 +265,32 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         conv_ms = e0.elapsed_time(e1) / len(nets)
-        learned = {"slices_per_s": value, "rmse_hu": l_rmse, "iterations": len(nets),
-                   "reaches_r_star": bool(l_rmse <= r_star), "speedup_vs_plain": value / plain_sps,
-                   "cnn_ms_per_application": conv_ms}
-        log("learned: %.2f slices/s, RMSE %.3f HU (R* %.3f), %.1fx plain" % (value, l_rmse, r_star,
-                                                                          value / plain_sps))
+        learned = {"slices_per_s": value, "batch_rmse_hu": l_rmse, "eval_rmse_hu": learned_eval,
+                   "iterations": len(nets), "reaches_r_star": bool(learned_eval <= r_star),
+                   "speedup_vs_plain": value / plain_sps, "cnn_ms_per_application": conv_ms}
+        log("learned: %.2f slices/s, eval RMSE %.3f HU (R* %.3f), %.1fx plain" % (value, learned_eval, r_star,
+                                                                               value / plain_sps))
         method = "learned_unrolled_%d" % len(nets)
         passes = len(nets)
+        del ugraph
     else:
         value = plain_sps
         ms_per_step = 1000.0 * t_plain / plain_steps
         method = "plain_sqs_tv"
         passes = k_conv
 
+    # per-kernel HIP-event times of eager passes on the library stream
+    eng.set_image(x0)
+    eng.set_profiling(True)
+    for _ in range(3):
+        eng.run_pass(**sqs_kw)
+    kt = eng.kernel_times()
+    eng.set_profiling(False)
+
     # ---- roofline of the dominant kernel (HIP events on the SQS stream) ----
     npers = eng.npers
     bytes_proj = 4.0 * (geo.nx * geo.ny + geo.nu * npers) * ns
-    bytes_upd = 4.0 * 5 * geo.nx * geo.ny * ns
+    bytes_upd = 4.0 * 6 * geo.nx * geo.ny * ns   # x, g, D read, x written; transposed copy read + written
     steps_k = max(kt["steps"], 1)
     kernels = {
         "siddon_fan_bp": (kt["bp"], bytes_proj),
@@ -269,9 +332,10 @@ def main():
         "dtype": "f32" if nets is None else "f32 (projectors), bf16 MFMA / f32 accumulate (prior)",
         "data": "synthetic (seeded ellipse phantoms, Poisson noise I0=%g)" % i0,
         "config": {"workload": "32 fan slices 512x512 @0.7mm, 1152 views, 736 bins, 12 subsets; " + method,
-                   "slices": ns, "method": method, "i0": i0, "k_conv": k_conv, "r_star_hu": r_star,
-                   "plain_slices_per_s": plain_sps, "plain_rmse_hu": plain_rmse, "learned": learned,
-                   "beta": args.beta, "delta": args.delta},
+                   "slices": ns, "method": method, "i0": i0, "init": "fbp_hann", "eval_slices": ne,
+                   "k_max": args.kmax, "k_conv": k_conv, "r_star_hu": r_star, "fbp_rmse_hu": fbp_rmse,
+                   "plain_slices_per_s": plain_sps, "plain_batch_rmse_hu": plain_rmse, "learned": learned,
+                   "beta": args.beta, "delta": args.delta, "hip_graph": not args.no_graph},
         "roofline": roofline,
         "kernels": per_kernel,
         "cpu_baseline": cpu,

@@ -43,6 +43,11 @@ struct FanK {
     const float* ecos;
     const float* bsin;   // [nu] sin of bin-centre angles
     const float* bcos;
+    // [nview][nu] line of the ray through bin centre k, in image coordinates:
+    // X views X(Y) = A + B (Y - ny/2), Y views Y(X) = A + B (X - nx/2);
+    // (A, B, L, L / |B|) with L the path length per image line (dy / |cos(a - t)|
+    // or dx / |sin(a - t)|) and L / |B| = 0 when B = 0.  Computed in FP64.
+    const float4* sray;
 };
 
 // Device tables for one (geometry, angle set).  Angles are kept on the host
@@ -56,7 +61,7 @@ public:
     std::vector<float> deg;
     std::vector<int> cls;
 private:
-    DevBuf buf_;
+    DevBuf buf_, ray_;
     std::vector<float> host_;
 };
 

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp(FanK g, const T* __restri
 // backprojectors (pixel driven gathers)
 // ---------------------------------------------------------------------------
 
-template <class T, int BX, int BY>
+template <class T, int BX, int BY, int NB>
 __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp(FanK g, const T* __restrict__ prj,
                                                        T* __restrict__ img, int view0, int nviews, int accumulate)
 {
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp(FanK g, const T* __re
     const int ix = blockIdx.y * BY + threadIdx.y;
     const int s = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
-    const T acc = siddon_bp_pixel<T>(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
+    const T acc = siddon_bp_pixel<T, NB>(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
     T* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
     *o = accumulate ? *o + acc : acc;
 }
@@ -302,6 +302,32 @@ void FanTables::build(const FanGeo& g, const float* deg_host, hipStream_t st)
     k.vcos = d; k.vsin = d + nview; k.vcls = reinterpret_cast<const int*>(d + 2 * nview);
     k.esin = d + 3 * nview; k.ecos = k.esin + (nu + 1);
     k.bsin = k.ecos + (nu + 1); k.bcos = k.bsin + nu;
+
+    // Siddon ray lines per (view, bin), FP64 then rounded
+    std::vector<float4> ray((size_t)nview * nu);
+    for (int v = 0; v < nview; v++) {
+        const double t = (double)deg_host[v];
+        for (int kb = 0; kb < nu; kb++) {
+            const double a = (-(kb - cu) - off) * da;
+            const double c = std::cos(a - t), sn = std::sin(a - t), sa = std::sin(a);
+            double A, B, L;
+            if (cls[v] == 0) {
+                A = 0.5 * g.nx + ((double)g.dso / g.dx) * sa / c;
+                B = ((double)g.dy / g.dx) * sn / c;
+                L = (double)g.dy / std::fabs(c);
+            } else {
+                A = 0.5 * g.ny - ((double)g.dso / g.dy) * sa / sn;
+                B = ((double)g.dx / g.dy) * c / sn;
+                L = (double)g.dx / std::fabs(sn);
+            }
+            const float Bf = (float)B;
+            ray[(size_t)v * nu + kb] = make_float4((float)A, Bf, (float)L, Bf != 0.0f ? (float)(L / std::fabs((double)Bf)) : 0.0f);
+        }
+    }
+    ray_.alloc(ray.size() * sizeof(float4));
+    RN_HIP_CHECK(hipMemcpyAsync(ray_.get(), ray.data(), ray.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+    k.sray = reinterpret_cast<const float4*>(ray_.get());
 }
 
 bool FanTables::matches(const FanGeo& g, const float* deg_host) const
@@ -331,11 +357,11 @@ void go_siddon_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj,
                        Pxy, prj, view0, nviews, meas, w, g.nview);
 }
 
-template <int BX, int BY>
+template <int BX, int BY, int NB>
 void go_siddon_bp(const FanK& g, const float* prj, float* img, int ns, int view0, int nviews, int acc,
                   hipStream_t st)
 {
-    hipLaunchKernelGGL((k_siddon_fan_bp<float, BX, BY>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0,
+    hipLaunchKernelGGL((k_siddon_fan_bp<float, BX, BY, NB>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0,
                        st, g, prj, img, view0, nviews, acc);
 }
 
@@ -362,7 +388,9 @@ using DdBpFn = void (*)(const FanK&, const float*, float*, int, int, hipStream_t
 // 1-D blocks along the detector; 2-D pixel tiles (x lanes along y, the contiguous image axis)
 const DdFpFn kDdFp[] = {go_dd_fp<128>, go_dd_fp<64>, go_dd_fp<256>};
 const SidFpFn kSidFp[] = {go_siddon_fp<128>, go_siddon_fp<64>, go_siddon_fp<256>};
-const GatherFn kSidBp[] = {go_siddon_bp<16, 16>, go_siddon_bp<8, 32>, go_siddon_bp<32, 8>, go_siddon_bp<8, 8>};
+const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 4>, go_siddon_bp<8, 32, 4>, go_siddon_bp<32, 8, 4>,
+                           go_siddon_bp<8, 8, 4>,   go_siddon_bp<16, 16, 3>, go_siddon_bp<8, 32, 3>,
+                           go_siddon_bp<32, 8, 3>,  go_siddon_bp<8, 8, 3>};
 const GatherFn kDdAdj[] = {go_dd_adj<16, 16>, go_dd_adj<8, 32>, go_dd_adj<32, 8>, go_dd_adj<8, 8>};
 const DdBpFn kDdBp[] = {go_dd_bp<16, 16>, go_dd_bp<8, 32>, go_dd_bp<32, 8>, go_dd_bp<8, 8>};
 
@@ -416,7 +444,7 @@ void launch_siddon_fan_bp_f64(const FanK& g, const double* prj_vu, double* img_x
 {
     if (nslices <= 0) return;
     dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL((k_siddon_fan_bp<double, 16, 16>), grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, 0);
+    hipLaunchKernelGGL((k_siddon_fan_bp<double, 16, 16, 4>), grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, 0);
     check_launch("siddon_fan_bp_f64");
 }
 
@@ -424,7 +452,7 @@ void launch_siddon_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int
                           int view0, int nviews, bool accumulate, hipStream_t st)
 {
     if (nslices <= 0) return;
-    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, 4, [&](int i) {
+    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, 8, [&](int i) {
         kSidBp[i](g, prj_vu, img_xy, nslices, view0, nviews, 0, st);
     }, st, !accumulate);
     kSidBp[c](g, prj_vu, img_xy, nslices, view0, nviews, accumulate ? 1 : 0, st);

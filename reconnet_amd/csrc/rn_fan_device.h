@@ -113,42 +113,128 @@ __device__ __forceinline__ float dd_fp_ray(const FanK& g, const float* __restric
     return acc;
 }
 
+// Sum over the cells of a line of n cells (zero outside [0, n)) of value times
+// overlap with [lo, hi] (no division: the caller scales by L / |B|).
+template <class T>
+__device__ __forceinline__ T seg_sum(const T* __restrict__ line, int n, float lo, float hi)
+{
+    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
+    if (!(h > l)) return T(0);
+    const int m0 = (int)l;
+    const int m1 = min((int)h, n - 1);
+    T s = T(0);
+    for (int m = m0; m <= m1; ++m) {
+        const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
+        s += line[m] * (T)(ph - pl);
+    }
+    return s;
+}
+
+// seg_sum for hi - lo <= 2 (at most three cells), without branches: the three
+// loads are independent and every weight max(0, min(h, m+1) - max(l, m)) is
+// zero for cells outside [l, h] (indices are clamped into the line).
+template <class T>
+__device__ __forceinline__ T seg_sum3(const T* __restrict__ line, int n, float lo, float hi)
+{
+    const float fn = (float)n;
+    const float l = fminf(fmaxf(lo, 0.0f), fn), h = fminf(hi, fn);
+    const int m0 = (int)l;
+    T s = T(0);
+    #pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int m = m0 + j;
+        const float w = fmaxf(fminf(h, (float)(m + 1)) - fmaxf(l, (float)m), 0.0f);
+        s += line[min(m, n - 1)] * (T)w;
+    }
+    return s;
+}
+
 // Siddon FP (exact ray / pixel intersection lengths), one line per image row
-// (X views) or column (Y views): the ray crosses row j over Y in [j, j+1],
-// i.e. over X in [X(j), X(j+1)], with path length dy / |cos(a - t)|; the
-// length inside each pixel is that times the pixel's share of the X interval.
+// (X views) or column (Y views): the ray crosses line j over [j, j+1] of the
+// line axis, i.e. over [A + B (j - n/2), A + B (j + 1 - n/2)] of the cell axis,
+// with path length L per line; the length inside a cell is L times the cell's
+// share of that interval (overlap / |B|).  A ray with B = 0 crosses every line
+// at the same cell coordinate A, with length L in each.
 template <class T>
 __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ Pyx, const T* __restrict__ Pxy,
                                            int view, int iu)
 {
-    const float ct = g.vcos[view], st = g.vsin[view];
+    const float4 r = g.sray[(long)view * g.nu + iu];
+    const bool xv = g.vcls[view] == 0;
+    const T* __restrict__ P = xv ? Pyx : Pxy;
+    const int nl = xv ? g.ny : g.nx;     // lines
+    const int n = xv ? g.nx : g.ny;      // cells per line
+    const float base = -0.5f * (float)nl;
     T acc = T(0);
-    float A, B, c;
-    if (g.vcls[view] == 0) {
-        xline(g, ct, st, g.bsin[iu], g.bcos[iu], A, B, c);
-        const float y0 = -0.5f * g.ny;
-        for (int j = 0; j < g.ny; ++j) {
-            const float yr = y0 + (float)j;
-            const float xa = fmaf(B, yr, A), xb = fmaf(B, yr + 1.0f, A);
-            acc += seg_avg<T>(Pyx + (long)j * g.nx, g.nx, fminf(xa, xb), fmaxf(xa, xb));
+    if (r.y != 0.0f) {
+        if (fabsf(r.y) <= 2.0f) {
+            for (int j = 0; j < nl; ++j) {
+                const float lr = base + (float)j;
+                const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+                acc += seg_sum3<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+            }
+        } else {
+            for (int j = 0; j < nl; ++j) {
+                const float lr = base + (float)j;
+                const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+                acc += seg_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+            }
         }
-        acc *= (T)(g.dy / fabsf(c));
-    } else {
-        yline(g, ct, st, g.bsin[iu], g.bcos[iu], A, B, c);
-        const float x0 = -0.5f * g.nx;
-        for (int i = 0; i < g.nx; ++i) {
-            const float xr = x0 + (float)i;
-            const float ya = fmaf(B, xr, A), yb = fmaf(B, xr + 1.0f, A);
-            acc += seg_avg<T>(Pxy + (long)i * g.ny, g.ny, fminf(ya, yb), fmaxf(ya, yb));
-        }
-        acc *= (T)(g.dx / fabsf(c));
+        return acc * (T)r.w;
     }
-    return acc;
+    if (r.x >= 0.0f && r.x < (float)n) {
+        const int m = (int)r.x;
+        for (int j = 0; j < nl; ++j) acc += P[(long)j * n + m];
+    }
+    return acc * (T)r.z;
+}
+
+// atan on all of R from an odd polynomial on [-1, 1] (coefficients fitted
+// here by iteratively reweighted least squares on Chebyshev nodes; maximum
+// error 2.5e-7 rad) and atan(z) = sign(z) pi/2 - atan(1/z) outside.  Used only
+// to locate the bins a pixel can meet; ATAN_FAST_ERR bounds its error with
+// FP32 evaluation included.
+#define RN_ATAN_FAST_ERR 1.0e-6f
+__device__ __forceinline__ float atan_fast(float z)
+{
+    const float az = fabsf(z);
+    const bool inv = az > 1.0f;
+    const float t = inv ? __builtin_amdgcn_rcpf(az) : az;
+    const float s = t * t;
+    float p = 0.006811782025349205f;
+    p = fmaf(p, s, -0.033604191884230736f);
+    p = fmaf(p, s, 0.07962364933877517f);
+    p = fmaf(p, s, -0.13233341755196557f);
+    p = fmaf(p, s, 0.1980781590519013f);
+    p = fmaf(p, s, -0.3331736818605284f);
+    p = fmaf(p, s, 0.9999961116602261f);
+    float r = p * t;
+    if (inv) r = 1.5707963267948966f - r;
+    return copysignf(r, z);
+}
+
+// First and last bin whose centre ray can meet pixel (ix, iy) of view `view`:
+// the pixel lies in the disc of radius rpix around its centre, whose rays span
+// the detector angles atan(rx / d0) +- asin(rpix / dist), asin(r/d) <= r / (d - r).
+__device__ __forceinline__ void bin_range(const FanK& g, float xc, float yc, float rpix, float inv_da, int view,
+                                          int& k0, int& k1)
+{
+    const float ct = g.vcos[view], st = g.vsin[view];
+    const float rx = xc * ct + yc * st, ry = -xc * st + yc * ct;
+    const float d0 = ry + g.dso;
+    const float uc = g.cu - g.off_a - atan_fast(rx * __builtin_amdgcn_rcpf(d0)) * inv_da;
+    const float dist = __builtin_amdgcn_sqrtf(fmaf(rx, rx, d0 * d0));
+    const float hw = rpix * inv_da * __builtin_amdgcn_rcpf(fmaxf(dist - rpix, 1e-3f)) + 0.02f +
+                     RN_ATAN_FAST_ERR * fabsf(inv_da);
+    k0 = max((int)ceilf(uc - hw), 0);
+    k1 = min((int)floorf(uc + hw), g.nu - 1);
 }
 
 // Exact transpose of siddon_fp_ray: pixel (ix, iy) gathers, over views
-// view0 .. view0+nviews-1 of prj ([nviews][nu]), the rays that cross it.
-template <class T>
+// view0 .. view0+nviews-1 of prj ([nviews][nu]), the rays that cross it, with
+// the weights siddon_fp_ray gives it.  The first NB bins of a pixel's range
+// are taken without branches, the rest in a loop (NB changes only the speed).
+template <class T, int NB = 4>
 __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict__ prj, int view0, int nviews,
                                              int ix, int iy)
 {
@@ -160,29 +246,32 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
     T acc = T(0);
     for (int vi = 0; vi < nviews; ++vi) {
         const int view = view0 + vi;
-        const float ct = g.vcos[view], st = g.vsin[view];
-        const float rx = xc * ct + yc * st, ry = -xc * st + yc * ct;
-        const float d0 = ry + g.dso;
-        const float uc = g.cu - g.off_a - atanf(rx / d0) * inv_da;
-        const float hw = half_width_bins(rpix, sqrtf(rx * rx + d0 * d0), inv_da);
-        const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
-        const T* p = prj + (long)vi * g.nu;
-        if (g.vcls[view] == 0) {
-            for (int k = k0; k <= k1; ++k) {
-                float A, B, c;
-                xline(g, ct, st, g.bsin[k], g.bcos[k], A, B, c);
-                const float xa = fmaf(B, yr, A), xb = fmaf(B, yr + 1.0f, A);
-                const float wgt = seg_weight(ix, fminf(xa, xb), fmaxf(xa, xb));
-                acc += p[k] * (T)(g.dy / fabsf(c) * wgt);
-            }
-        } else {
-            for (int k = k0; k <= k1; ++k) {
-                float A, B, c;
-                yline(g, ct, st, g.bsin[k], g.bcos[k], A, B, c);
-                const float ya = fmaf(B, xr, A), yb = fmaf(B, xr + 1.0f, A);
-                const float wgt = seg_weight(iy, fminf(ya, yb), fmaxf(ya, yb));
-                acc += p[k] * (T)(g.dx / fabsf(c) * wgt);
-            }
+        int k0, k1;
+        bin_range(g, xc, yc, rpix, inv_da, view, k0, k1);
+        const bool xv = g.vcls[view] == 0;
+        const float lr = xv ? yr : xr;          // this pixel's line
+        const float cell = xv ? (float)ix : (float)iy;
+        const float4* __restrict__ R = g.sray + (long)view * g.nu;
+        const T* __restrict__ p = prj + (long)vi * g.nu;
+        // the first NB bins without branches (clamped, zero-weighted when
+        // past k1), the rest in a loop
+        #pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int kk = min(k0 + j, g.nu - 1);
+            const float4 r = R[kk];
+            const T pv = p[kk];
+            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+            const float ov = fmaxf(fminf(fmaxf(ea, eb), cell + 1.0f) - fmaxf(fminf(ea, eb), cell), 0.0f) * r.w;
+            const float pt = (r.x >= cell && r.x < cell + 1.0f) ? r.z : 0.0f;
+            const float w = (k0 + j <= k1) ? (r.y != 0.0f ? ov : pt) : 0.0f;
+            acc += pv * (T)w;
+        }
+        for (int k = k0 + NB; k <= k1; ++k) {
+            const float4 r = R[k];
+            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+            const float ov = fmaxf(fminf(fmaxf(ea, eb), cell + 1.0f) - fmaxf(fminf(ea, eb), cell), 0.0f) * r.w;
+            const float pt = (r.x >= cell && r.x < cell + 1.0f) ? r.z : 0.0f;
+            acc += p[k] * (T)(r.y != 0.0f ? ov : pt);
         }
     }
     return acc;

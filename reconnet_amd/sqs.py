@@ -18,7 +18,20 @@ from ._lib import check, lib
 from .geometry import FanGeometry, interleaved_order
 from .torch_ops import FanPlan, _ptr, _stream
 
-__all__ = ["OsSqsTV"]
+__all__ = ["OsSqsTV", "PassGraph", "fbp_initial_image"]
+
+
+def fbp_initial_image(plan, sino, filter_type=2, chunk=16, out=None):
+    """Non-negative FBP images [B, nx, ny] of sinograms [B, nu, nview] (natural
+    view order): windowed ramp filter (2 = Hann) and the FBP-weighted DD
+    backprojection, the usual starting image of the iterative methods."""
+    B = sino.shape[0]
+    g = plan.geo
+    if out is None:
+        out = torch.empty((B, g.nx, g.ny), dtype=torch.float32, device=sino.device)
+    for i in range(0, B, chunk):
+        out[i:i + chunk] = plan.fbp(sino[i:i + chunk].contiguous(), filter_type, "dd_fbp")
+    return out.clamp_(min=0.0)
 
 
 class OsSqsTV:

@@ -7,7 +7,9 @@ geometry (0.7 mm pixels, 1152 views, 736 bins), projected by the library's
 Siddon projector and corrupted with Poisson noise at I0 (quarter dose 2.5e4 by
 default).  All N slices stay resident in HBM in one OS-SQS engine.
 
-Training is greedy: stage k takes the current images of all slices, applies one
+Every slice starts from its FBP image (Hann-windowed ramp, FBP-weighted DD
+backprojection), as in the reconstruction itself.  Training is greedy: stage k
+takes the current images of all slices, applies one
 OS-SQS data pass (12 subsets), and fits the stage-k CNN prior (initialised from
 stage k-1) to map the result to the phantom (masked MSE in HU^2, Adam).  The
 trained stage is applied to every slice before the next stage starts.  Every
@@ -25,7 +27,7 @@ import torch
 from .cnn import PriorCNN, init_params
 from .geometry import HU_WATER, WORKLOADS
 from .phantom import add_poisson_noise, phantom_batch
-from .sqs import OsSqsTV
+from .sqs import OsSqsTV, fbp_initial_image
 from .torch_ops import FanPlan
 from .unrolled import save_nets
 
@@ -37,9 +39,10 @@ def log(msg):
 
 
 def simulate(geo, seeds, i0, noise_seed, chunk=64):
-    """Phantoms, noisy sinograms and weights for the given seeds (CUDA tensors)."""
+    """Phantoms, noisy sinograms, weights and FBP starting images for the given
+    seeds (CUDA tensors)."""
     plan = FanPlan(geo)
-    truths, sinos, wts = [], [], []
+    truths, sinos, wts, inits = [], [], [], []
     for i in range(0, len(seeds), chunk):
         s = list(seeds[i:i + chunk])
         t = phantom_batch(s, geo.nx, geo.ny, geo.dx, geo.dy, device="cuda").contiguous()
@@ -48,7 +51,8 @@ def simulate(geo, seeds, i0, noise_seed, chunk=64):
         truths.append(t)
         sinos.append(n)
         wts.append(w)
-    return torch.cat(truths), torch.cat(sinos), torch.cat(wts)
+        inits.append(fbp_initial_image(plan, n))
+    return torch.cat(truths), torch.cat(sinos), torch.cat(wts), torch.cat(inits)
 
 
 def masked_rmse_hu(x, truth, mask):
@@ -75,8 +79,8 @@ def main(argv=None):
     geo = WORKLOADS["fan512"]
     mask = torch.from_numpy(geo.fov_mask()).cuda().float()
     t0 = time.time()
-    truth, sino, w = simulate(geo, list(range(args.train)), args.i0, noise_seed=7)
-    vtruth, vsino, vw = simulate(geo, list(range(100000, 100000 + args.val)), args.i0, noise_seed=7)
+    truth, sino, w, x0 = simulate(geo, list(range(args.train)), args.i0, noise_seed=7)
+    vtruth, vsino, vw, vx0 = simulate(geo, list(range(50000, 50000 + args.val)), args.i0, noise_seed=7)
     log("simulated %d + %d slices in %.1f s" % (args.train, args.val, time.time() - t0))
 
     eng = OsSqsTV(geo, args.subsets, args.train)
@@ -89,10 +93,10 @@ def main(argv=None):
     del vsino, vw
     log("normalisers ready (%.1f s)" % (time.time() - t0))
 
-    zeros = torch.zeros((args.train, geo.nx, geo.ny), device="cuda")
-    eng.set_image(zeros)
-    veng.set_image(torch.zeros((args.val, geo.nx, geo.ny), device="cuda"))
-    del zeros
+    eng.set_image(x0)
+    veng.set_image(vx0)
+    log("FBP start: val RMSE %.2f HU" % float(masked_rmse_hu(vx0, vtruth, mask).mean()))
+    del x0, vx0
     nets, history = [], []
     params = init_params(seed=0)
     gen = torch.Generator(device="cpu").manual_seed(1)
