@@ -57,8 +57,8 @@ def parse():
     ap.add_argument("--eval", type=int, default=200, help="held-out slices for k_conv, R* and the learned RMSE")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed passes eagerly")
     ap.add_argument("--probe", action="store_true",
-                    help="profiling probe: FBP + 2 eager OS-SQS passes over the batch, then a 256 MiB device "
-                         "copy (the reference kernel for counter scaling); no timing line")
+                    help="profiling probe: FBP + 2 eager OS-SQS passes over the batch, a 256 MiB elementwise "
+                         "copy (the reference kernel for counter scaling) and one CNN prior; no timing line")
     ap.add_argument("--subsets", type=int, default=12)
     ap.add_argument("--beta", type=float, default=1e-4)
     ap.add_argument("--delta", type=float, default=1e-3)
@@ -153,11 +153,20 @@ def main():
         eng.set_image(fbp_initial_image(plan, noisy))
         for _ in range(2):
             eng.run_pass(**sqs_kw)
+        # reference kernel for counter scaling: an elementwise kernel that reads
+        # 256 MiB and writes 256 MiB (a plain copy_ would go to the DMA blit path)
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
+        if nets is not None:
+            xx = eng.image()
+            for net in nets:
+                xx = net.forward(xx)
+        else:
+            from reconnet_amd.cnn import PriorCNN, init_params
+            PriorCNN(init_params(seed=0, device=dev), scale=50.0).forward(eng.image())
         torch.cuda.synchronize()
-        log("probe done: 2 passes over %d slices, one copy of %d bytes" % (ns, src.numel() * 4))
+        log("probe done: 2 passes over %d slices, CNN prior, one elementwise copy of %d bytes" % (ns, src.numel() * 4))
         return
 
     # ---- plain SQS-TV to convergence on the evaluation slices (untimed) ----
