@@ -239,6 +239,23 @@ int rn_ragged_forward(void* batch, int kind /* RN_FP_DD | RN_FP_SIDDON */, const
 int rn_ragged_backward(void* batch, int kind /* RN_BP_DD_ADJOINT | RN_BP_SIDDON */, const float* prj, float* img,
         void* stream);
 
+/* Circular cone-beam plans (no reference counterpart; the reference's DD 3D
+ * pair, cDistanceDrivenTomo*, is tomosynthesis only).  Per-view detector
+ * centre, axes and source as in cSiddonConeProjectionAbitrary (host arrays
+ * [nview][3]); the DD pair needs detV = (0, 0, v) and detU.z = 0.  The DD
+ * projector integrates, per view, over the voxel planes normal to the main
+ * axis (x iff |C.x - S.x| >= |C.y - S.y|) the rectangle footprints of the
+ * detector cells; its backward is the exact adjoint.  The Siddon kind is the
+ * SiddonCone pair (SiddonCone.cu:117,196).  Device pointers, reference
+ * layouts, caller stream. */
+enum { RN_CONE_DD = 0, RN_CONE_SIDDON = 1 };
+void* rn_cone_plan_create(int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
+        int nu, int nview, int nv, float du, float dv, float off_u, float off_v,
+        const float* detCenter, const float* detU, const float* detV, const float* src);
+void rn_cone_plan_destroy(void* plan);
+int rn_cone_forward(void* plan, int kind, const float* img, float* prj, int nBatches, int nChannels, void* stream);
+int rn_cone_backward(void* plan, int kind, const float* prj, float* img, int nBatches, int nChannels, void* stream);
+
 /* Launch-shape autotuner (no reference counterpart).  On the first launch of
  * a shape key the projector launchers time their candidate block / tile shapes
  * on the caller's stream and keep the fastest for the process; the choice

@@ -217,6 +217,14 @@ def siddon_cone_bp(prj, det_center, det_u, det_v, src, geo, img_shape, precision
     return _arb_bp("siddon_cone_bp", prj, [det_center, det_u, det_v, src], geo, img_shape, precision)
 
 
+def dd_cone_fp(img, det_center, det_u, det_v, src, geo, nu, nv):
+    return _arb_fp("dd_cone_fp", img, [det_center, det_u, det_v, src], geo, nu, nv, "f64")
+
+
+def dd_cone_adj(prj, det_center, det_u, det_v, src, geo, img_shape):
+    return _arb_bp("dd_cone_adj", prj, [det_center, det_u, det_v, src], geo, img_shape, "f64")
+
+
 def dd_tomo_fp(img, det_center, src, geo, nu, nv, precision="f64"):
     return _arb_fp("dd_tomo_fp", img, [det_center, src], geo, nu, nv, precision)
 

@@ -972,6 +972,113 @@ void helical_bp(R* imgOut, const float* prj, int nB, int nC, int nx, int ny, int
 
 // ===========================================================================
 // C exports (f64 = parity oracle, f32 = CPU timing baseline)
+// ---------------------------------------------------------------------------
+// Circular cone-beam DD pair with a per-view main axis (no reference
+// counterpart: the reference's DD 3D pair, DistanceDriven3D.cu:139-227, is
+// for tomosynthesis with the detector normal to z).  Definition shared with
+// the GPU kernels:
+//  * a view is an X view iff |C.x - S.x| >= |C.y - S.y| (float32), C the
+//    detector centre and S the source; X views integrate over the planes
+//    x = voxel-centre x of every voxel layer, Y views over y planes;
+//  * detector cell (iu, iv): u = (iu - off_u - (nu-1)/2) du, v likewise, point
+//    P(u, v) = C + u U + v V; on plane x = xp the cell's footprint is the
+//    rectangle [y(u-), y(u+)] x [z(v-), z(v+)] where y(u+-) are the plane
+//    crossings of the rays S -> P(u+-, v_c) and z(v+-) those of S -> P(u_c, v+-);
+//  * ray value = sum over planes of the rectangle average of the plane (zero
+//    outside the volume) times dx |P_c - S| / |P_c.x - S.x| (dy ... for Y views).
+// The adjoint is the exact transpose (a scatter here; a gather on the GPU).
+// Parity of this definition against the line integral itself is checked with
+// the Siddon cone projector on smooth volumes.
+// ---------------------------------------------------------------------------
+template <class R, class F>
+void dd_cone_ray(const Arb& g, const float* C, const float* U, const float* V, const float* S, int view, int iu,
+                 int iv, F f)
+{
+    const double cx0 = C[3 * view], cy0 = C[3 * view + 1], cz0 = C[3 * view + 2];
+    const double sx = S[3 * view], sy = S[3 * view + 1], sz = S[3 * view + 2];
+    const bool xv = std::fabs((float)C[3 * view] - (float)S[3 * view]) >= std::fabs((float)C[3 * view + 1] - (float)S[3 * view + 1]);
+    const double uc = (iu - g.off_u - (g.nu - 1) / 2.0) * g.du, vc = (iv - g.off_v - (g.nv - 1) / 2.0) * g.dv;
+    auto P = [&](double u, double v, double out[3]) {
+        for (int k = 0; k < 3; k++) out[k] = C[3 * view + k] + u * U[3 * view + k] + v * V[3 * view + k];
+    };
+    double Pc[3], Ea[3], Eb[3], Fa[3], Fb[3];
+    P(uc, vc, Pc);
+    P(uc - g.du / 2, vc, Ea); P(uc + g.du / 2, vc, Eb);
+    P(uc, vc - g.dv / 2, Fa); P(uc, vc + g.dv / 2, Fb);
+    (void)cx0; (void)cy0; (void)cz0;
+    const double Sv[3] = {sx, sy, sz};
+    const int ax = xv ? 0 : 1, bx = xv ? 1 : 0;       // main axis, in-plane axis
+    const int nmain = xv ? g.nx : g.ny, nin = xv ? g.ny : g.nx;
+    const double dmain = xv ? g.dx : g.dy, din = xv ? g.dy : g.dx, cmain = xv ? g.cx : g.cy, cin = xv ? g.cy : g.cx;
+    const double len = std::sqrt((Pc[0] - sx) * (Pc[0] - sx) + (Pc[1] - sy) * (Pc[1] - sy) + (Pc[2] - sz) * (Pc[2] - sz));
+    const double dpc = Pc[ax] - Sv[ax];
+    if (std::fabs(dpc) < 1e-9) return;
+    const double L = dmain * len / std::fabs(dpc);
+    for (int i = 0; i < nmain; i++) {
+        const double xp = (i + 0.5 - nmain / 2.0) * dmain + cmain;
+        auto cross = [&](const double E[3], int axis, double c0, double d0, int n0) {
+            const double t = (xp - Sv[ax]) / (E[ax] - Sv[ax]);
+            return (Sv[axis] + t * (E[axis] - Sv[axis]) - c0) / d0 + n0 / 2.0;
+        };
+        double y1 = cross(Ea, bx, cin, din, nin), y2 = cross(Eb, bx, cin, din, nin);
+        double z1 = cross(Fa, 2, g.cz, g.dz, g.nz), z2 = cross(Fb, 2, g.cz, g.dz, g.nz);
+        if (y1 > y2) std::swap(y1, y2);
+        if (z1 > z2) std::swap(z1, z2);
+        const double area = (y2 - y1) * (z2 - z1);
+        if (!(area > 0)) continue;
+        const double ly = std::max(y1, 0.0), hy = std::min(y2, (double)nin);
+        const double lz = std::max(z1, 0.0), hz = std::min(z2, (double)g.nz);
+        if (!(hy > ly) || !(hz > lz)) continue;
+        for (int j = (int)ly; j <= std::min((int)hy, nin - 1); j++) {
+            const double oy = std::min(hy, j + 1.0) - std::max(ly, (double)j);
+            if (oy <= 0) continue;
+            for (int k = (int)lz; k <= std::min((int)hz, g.nz - 1); k++) {
+                const double oz = std::min(hz, k + 1.0) - std::max(lz, (double)k);
+                if (oz <= 0) continue;
+                const long ix = xv ? i : j, iy = xv ? j : i;
+                f(((long)ix * g.ny + iy) * g.nz + k, (R)(oy * oz / area * L));
+            }
+        }
+    }
+}
+
+template <class R>
+void dd_cone_fp(R* prj, const float* img, const float* C, const float* U, const float* V, const float* S, const Arb& g)
+{
+    const long nvox = (long)g.nx * g.ny * g.nz;
+    for (int b = 0; b < g.nB; b++)
+        for (int c = 0; c < g.nC; c++) {
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+            for (int iu = 0; iu < g.nu; iu++)
+                for (int view = 0; view < g.nview; view++)
+                    for (int iv = 0; iv < g.nv; iv++) {
+                        R acc = 0;
+                        dd_cone_ray<R>(g, C, U, V, S, view, iu, iv, [&](long vox, R w) {
+                            acc += w * (R)img[((long)b * nvox + vox) * g.nC + c];
+                        });
+                        prj[((((long)b * g.nu + iu) * g.nview + view) * g.nv + iv) * g.nC + c] = acc;
+                    }
+        }
+}
+
+template <class R>
+void dd_cone_adj(R* img, const float* prj, const float* C, const float* U, const float* V, const float* S, const Arb& g)
+{
+    const long nvox = (long)g.nx * g.ny * g.nz;
+    std::fill(img, img + nvox * g.nB * g.nC, R(0));
+    for (int b = 0; b < g.nB; b++)
+        for (int c = 0; c < g.nC; c++)
+            for (int view = 0; view < g.nview; view++)
+                for (int iu = 0; iu < g.nu; iu++)
+                    for (int iv = 0; iv < g.nv; iv++) {
+                        const R p = (R)prj[((((long)b * g.nu + iu) * g.nview + view) * g.nv + iv) * g.nC + c];
+                        if (p == 0) continue;
+                        dd_cone_ray<R>(g, C, U, V, S, view, iu, iv, [&](long vox, R w) {
+                            img[((long)b * nvox + vox) * g.nC + c] += w * p;
+                        });
+                    }
+}
+
 // ===========================================================================
 #define FAN_ARGS int nB, int nC, int nx, int ny, int nz, float dx, float dy, float dz, \
     int nu, int nview, int nv, float da, float dv, float off_a, float off_v, float dsd, float dso, int type
@@ -1082,6 +1189,13 @@ void orc_dd_tomo_fp_f32(float* prj, const float* img, const float* C, const floa
 { dd_tomo_fp<float>(prj, img, C, src, ARB_MAKE); }
 void orc_dd_tomo_bp_f32(float* img, const float* prj, const float* C, const float* src, ARB_ARGS)
 { dd_tomo_bp<float>(img, prj, C, src, ARB_MAKE); }
+
+void orc_dd_cone_fp_f64(double* prj, const float* img, const float* C, const float* U, const float* V,
+                        const float* src, ARB_ARGS)
+{ dd_cone_fp<double>(prj, img, C, U, V, src, ARB_MAKE); }
+void orc_dd_cone_adj_f64(double* img, const float* prj, const float* C, const float* U, const float* V,
+                         const float* src, ARB_ARGS)
+{ dd_cone_adj<double>(img, prj, C, U, V, src, ARB_MAKE); }
 
 void orc_helical_bp_f64(double* img, const float* prj, int nB, int nC, int nx, int ny, int nz, float dx, float dy, float dz,
                         int nu, int nview, int nv, float du, float dv, float off_u, float off_v, float dsd, float dso,

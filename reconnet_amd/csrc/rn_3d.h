@@ -34,6 +34,24 @@ void os_sqs_cone(const ArbGeo& g, int nB, int nC, float* x, const float* prj, co
                  const float* detCenter, const float* detU, const float* detV, const float* src,
                  float lb, float ub, int nviewPerSubset, int nSubsets, float lambda, DevBuf& work, hipStream_t st);
 
+// Circular cone-beam plan (no reference counterpart): per-view detector
+// centre, axes and source on the device plus the host's float32 choice of each
+// view's main axis (x iff |C.x - S.x| >= |C.y - S.y|).  The DD pair needs
+// detector columns parallel to z (detV = (0, 0, v)) and rows normal to it
+// (detU.z = 0), the geometry of circular CT.
+class ConePlan {
+public:
+    void setup(const ArbGeo& g, const float* C, const float* U, const float* V, const float* S, hipStream_t st);
+    // reference layouts: img [B][nx][ny][nz][C], prj [B][nu][nview][nv][C] (device)
+    void dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t st) const;
+    void dd_adj(int nB, int nC, const float* prj, float* img, hipStream_t st) const;
+    void siddon_fp(int nB, int nC, const float* img, float* prj, DevBuf& work, hipStream_t st) const;
+    void siddon_bp(int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st) const;
+    ArbGeo geo{};
+private:
+    DevBuf vec_, cls_;
+};
+
 struct HelicalGeo {
     int nx, ny, nz;
     float dx, dy, dz;

@@ -271,6 +271,187 @@ __global__ __launch_bounds__(256) void k_dd_tomo_bp(ArbGeo a, Grid3K g, const fl
     img[((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + c] = val;
 }
 
+// ---- circular cone-beam DD with a per-view main axis ----
+//
+// X views (|C.x - S.x| >= |C.y - S.y|, decided on the host) integrate over the
+// planes x = voxel-centre x; Y views over y planes.  On plane p the footprint
+// of detector cell (iu, iv) is [e1(p), e2(p)] x [z1(p), z2(p)] in index
+// coordinates of the in-plane axis and of z: the plane crossings of the rays
+// through the cell's u edges (at the cell's v centre) and through its v edges
+// (at its u centre).  Each crossing is affine in the plane index, so a ray is
+// described by four (alpha, beta) pairs and its per-plane length L.  The
+// forward value is sum_p (rectangle average of plane p) * L; the adjoint
+// gathers, per voxel, exactly those weights (overlap area / rectangle area * L).
+
+struct ConeRay {
+    float a[4], b[4];   // e1, e2, z1, z2 = a + b * plane
+    float L;
+    bool ok;
+};
+
+__device__ __forceinline__ ConeRay cone_ray(const ArbGeo& a, const Grid3K& g, const F3& c, const F3& U, const F3& V,
+                                            const F3& s, bool xv, int iu, int iv)
+{
+    ConeRay r;
+    const float uc = (iu - a.off_u - (a.nu - 1) / 2.0f) * a.du, vc = (iv - a.off_v - (a.nv - 1) / 2.0f) * a.dv;
+    auto P = [&](float u, float v) {
+        return F3{fmaf(v, V.x, fmaf(u, U.x, c.x)), fmaf(v, V.y, fmaf(u, U.y, c.y)), fmaf(v, V.z, fmaf(u, U.z, c.z))};
+    };
+    const F3 pc = P(uc, vc);
+    const F3 pts[4] = {P(uc - 0.5f * a.du, vc), P(uc + 0.5f * a.du, vc), P(uc, vc - 0.5f * a.dv), P(uc, vc + 0.5f * a.dv)};
+    // main axis m, in-plane axis n (plane coordinate of plane 0 and its step)
+    const float sm = xv ? s.x : s.y, sn = xv ? s.y : s.x;
+    const float dm = xv ? g.dx : g.dy, dn = xv ? g.dy : g.dx;
+    const float cn = xv ? g.cy : g.cx;
+    const int nn = xv ? g.ny : g.nx, nm = xv ? g.nx : g.ny;
+    const float m0 = (0.5f - 0.5f * nm) * dm + (xv ? g.cx : g.cy);
+    const float dpc = (xv ? pc.x : pc.y) - sm;
+    r.ok = fabsf(dpc) > 1e-6f;
+    const float lx = pc.x - s.x, ly = pc.y - s.y, lz = pc.z - s.z;
+    r.L = dm * sqrtf(fmaf(lx, lx, fmaf(ly, ly, lz * lz))) / fabsf(dpc);
+    #pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float em = (xv ? pts[e].x : pts[e].y) - sm;
+        const float eo = e < 2 ? (xv ? pts[e].y : pts[e].x) - sn : pts[e].z - s.z;
+        const float k = eo / em;                                  // slope along the main axis
+        const float so = e < 2 ? sn : s.z, co = e < 2 ? cn : g.cz, d = e < 2 ? dn : g.dz;
+        const float no = e < 2 ? (float)nn : (float)g.nz;
+        r.b[e] = k * dm / d;
+        r.a[e] = (fmaf(m0 - sm, k, so) - co) / d + 0.5f * no;
+        r.ok = r.ok && fabsf(em) > 1e-6f;
+    }
+    return r;
+}
+
+// vol, prj: reference layouts [b][nx][ny][nz][c], [b][nu][nview][nv][c]
+__global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const float* __restrict__ vol,
+                                                   float* __restrict__ prj, const float* Cp, const float* Up,
+                                                   const float* Vp, const float* Sp, const int* __restrict__ cls,
+                                                   int nC)
+{
+    const int iv = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iu = blockIdx.y;
+    const int view = blockIdx.z % a.nview, bc = blockIdx.z / a.nview;
+    if (iv >= a.nv) return;
+    const bool xv = cls[view] == 0;
+    const ConeRay r = cone_ray(a, g, ld3(Cp, view), ld3(Up, view), ld3(Vp, view), ld3(Sp, view), xv, iu, iv);
+    const int nm = xv ? g.nx : g.ny, nn = xv ? g.ny : g.nx;
+    const long sP = xv ? (long)g.ny * g.nz : g.nz, sJ = xv ? g.nz : (long)g.ny * g.nz;
+    const int c = bc % nC, b = bc / nC;
+    const float* base = vol + (long)b * g.nx * g.ny * g.nz * nC + c;   // reference layout, channel innermost
+    float acc = 0.f;
+    if (r.ok) {
+        for (int p = 0; p < nm; ++p) {
+            const float fp = (float)p;
+            float e1 = fmaf(r.b[0], fp, r.a[0]), e2 = fmaf(r.b[1], fp, r.a[1]);
+            float z1 = fmaf(r.b[2], fp, r.a[2]), z2 = fmaf(r.b[3], fp, r.a[3]);
+            if (e1 > e2) { const float t = e1; e1 = e2; e2 = t; }
+            if (z1 > z2) { const float t = z1; z1 = z2; z2 = t; }
+            const float area = (e2 - e1) * (z2 - z1);
+            const float le = fmaxf(e1, 0.f), he = fminf(e2, (float)nn);
+            const float lz = fmaxf(z1, 0.f), hz = fminf(z2, (float)g.nz);
+            if (!(area > 0.f) || !(he > le) || !(hz > lz)) continue;
+            const float* P = base + p * sP * nC;
+            const int j0 = (int)le, j1 = min((int)he, nn - 1), k0 = (int)lz, k1 = min((int)hz, g.nz - 1);
+            float sum = 0.f;
+            for (int j = j0; j <= j1; ++j) {
+                const float oe = fminf(he, (float)(j + 1)) - fmaxf(le, (float)j);
+                if (oe <= 0.f) continue;
+                float row = 0.f;
+                for (int k = k0; k <= k1; ++k) {
+                    const float oz = fminf(hz, (float)(k + 1)) - fmaxf(lz, (float)k);
+                    row = fmaf(P[(j * sJ + k) * nC], fmaxf(oz, 0.f), row);
+                }
+                sum = fmaf(row, oe, sum);
+            }
+            acc += sum / area;
+        }
+        acc *= r.L;
+    }
+    prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c] = acc;
+}
+
+// Exact transpose of k_dd_cone_fp: voxel (ix, iy, iz) gathers, per view, the
+// cells whose footprint on its plane overlaps it.  Candidate columns come
+// from inverting the plane crossing of the u-edge rays, u = (A - q Cc) /
+// (q D - B) with q the in-plane slope of the voxel edge; candidate rows from
+// the (linear) v mapping of the column's centre ray.  The weights are
+// recomputed with cone_ray, as in the forward pass.  Reference layouts.
+__global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const float* __restrict__ prj,
+                                                     float* __restrict__ img, const float* Cp, const float* Up,
+                                                     const float* Vp, const float* Sp, const int* __restrict__ cls,
+                                                     int nC)
+{
+    const int iz = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int ix = blockIdx.z % g.nx, bc = blockIdx.z / g.nx;
+    if (iz >= g.nz || iy >= g.ny) return;
+    const int cc = bc % nC, b = bc / nC;
+    const float* q = prj + (long)b * a.nu * a.nview * a.nv * nC + cc;
+    const float cu = (a.nu - 1) / 2.0f + a.off_u, cv = (a.nv - 1) / 2.0f + a.off_v;
+    const float margin = 0.05f;
+    float acc = 0.f;
+    for (int view = 0; view < a.nview; ++view) {
+        const bool xv = cls[view] == 0;
+        const F3 c = ld3(Cp, view), U = ld3(Up, view), V = ld3(Vp, view), s = ld3(Sp, view);
+        const int p = xv ? ix : iy, j = xv ? iy : ix;
+        const int nm = xv ? g.nx : g.ny;
+        const float dm = xv ? g.dx : g.dy, dn = xv ? g.dy : g.dx;
+        const float sm = xv ? s.x : s.y, sn = xv ? s.y : s.x;
+        const float xp = ((float)p + 0.5f - 0.5f * nm) * dm + (xv ? g.cx : g.cy) - sm;   // plane, from the source
+        // u of the rays through the voxel's two in-plane edges
+        const float cn0 = (xv ? g.cy : g.cx) - 0.5f * (xv ? g.ny : g.nx) * dn;
+        const float A = (xv ? c.y : c.x) - sn, Bu = xv ? U.y : U.x, Cc = (xv ? c.x : c.y) - sm, D = xv ? U.x : U.y;
+        float ulo = 3.0e38f, uhi = -3.0e38f;
+        #pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float qe = (cn0 + (float)(j + e) * dn - sn) / xp;
+            const float u = (A - qe * Cc) / (qe * D - Bu);
+            ulo = fminf(ulo, u);
+            uhi = fmaxf(uhi, u);
+        }
+        const float flo = fmaxf(ulo / a.du + cu - 0.5f - margin, -1.0f);
+        const float fhi = fminf(uhi / a.du + cu + 0.5f + margin, (float)a.nu);
+        if (!(fhi >= flo)) continue;                // also rejects NaN
+        const int u0 = max((int)ceilf(flo), 0), u1 = min((int)floorf(fhi), a.nu - 1);
+        for (int iu = u0; iu <= u1; ++iu) {
+            // rows: z of the column's centre ray is linear in v
+            const float uc = (iu - cu) * a.du;
+            const float fx = fmaf(uc, xv ? U.x : U.y, xv ? c.x : c.y) - sm;
+            const float zc0 = fmaf(uc, U.z, c.z) - s.z;
+            const float zs = xp / fx;                    // plane distance / column distance
+            float vlo = 3.0e38f, vhi = -3.0e38f;
+            #pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float zw = ((float)(iz + e) - 0.5f * g.nz) * g.dz + g.cz - s.z;
+                const float v = (zw / zs - zc0) / V.z;
+                vlo = fminf(vlo, v);
+                vhi = fmaxf(vhi, v);
+            }
+            const float glo = fmaxf(vlo / a.dv + cv - 0.5f - margin, -1.0f);
+            const float ghi = fminf(vhi / a.dv + cv + 0.5f + margin, (float)a.nv);
+            if (!(ghi >= glo)) continue;
+            const int v0 = max((int)ceilf(glo), 0), v1 = min((int)floorf(ghi), a.nv - 1);
+            for (int iv = v0; iv <= v1; ++iv) {
+                const ConeRay r = cone_ray(a, g, c, U, V, s, xv, iu, iv);
+                if (!r.ok) continue;
+                const float fp = (float)p;
+                float e1 = fmaf(r.b[0], fp, r.a[0]), e2 = fmaf(r.b[1], fp, r.a[1]);
+                float z1 = fmaf(r.b[2], fp, r.a[2]), z2 = fmaf(r.b[3], fp, r.a[3]);
+                if (e1 > e2) { const float t = e1; e1 = e2; e2 = t; }
+                if (z1 > z2) { const float t = z1; z1 = z2; z2 = t; }
+                const float area = (e2 - e1) * (z2 - z1);
+                if (!(area > 0.f)) continue;
+                const float oe = fminf(e2, (float)(j + 1)) - fmaxf(e1, (float)j);
+                const float oz = fminf(z2, (float)(iz + 1)) - fmaxf(z1, (float)iz);
+                if (oe <= 0.f || oz <= 0.f) continue;
+                acc = fmaf(q[(((long)iu * a.nview + view) * a.nv + iv) * nC], oe * oz / area * r.L, acc);
+            }
+        }
+    }
+    img[((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc] = acc;
+}
+
 // ---- cone OS-SQS helpers ----
 
 __global__ void k_cone_residual(const float* __restrict__ fp, const float* __restrict__ prj, float* __restrict__ r,
@@ -422,6 +603,71 @@ void dd_tomo_bp(const ArbGeo& g, int nB, int nC, const float* prj, float* img,
     dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nB * nC * g.nz);
     hipLaunchKernelGGL(k_dd_tomo_bp, grd, blk, 0, st, g, grid_of(g), work.as<float>(), img, detCenter, src, nC);
     check_launch("dd_tomo_bp");
+}
+
+void ConePlan::setup(const ArbGeo& g, const float* C, const float* U, const float* V, const float* S, hipStream_t st)
+{
+    RN_REQUIRE(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.nu > 0 && g.nview > 0 && g.nv > 0, "cone dimensions must be positive");
+    RN_REQUIRE(g.dx > 0 && g.dy > 0 && g.dz > 0 && g.du > 0 && g.dv > 0, "cone spacings must be positive");
+    RN_REQUIRE(C && U && V && S, "null geometry array");
+    const int n = g.nview;
+    std::vector<float> h((size_t)12 * n);
+    std::vector<int> cls(n);
+    for (int v = 0; v < n; v++) {
+        const float* u = U + 3 * v;
+        const float* w = V + 3 * v;
+        const float un = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const float wn = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        RN_REQUIRE(un > 0 && wn > 0, "detector axes must be non-zero");
+        RN_REQUIRE(std::fabs(w[0]) <= 1e-6f * wn && std::fabs(w[1]) <= 1e-6f * wn,
+                   "the cone DD projector needs detector columns parallel to z (detV = (0, 0, v))");
+        RN_REQUIRE(std::fabs(u[2]) <= 1e-6f * un, "the cone DD projector needs detector rows normal to z (detU.z = 0)");
+        cls[v] = std::fabs(C[3 * v] - S[3 * v]) >= std::fabs(C[3 * v + 1] - S[3 * v + 1]) ? 0 : 1;
+    }
+    std::copy(C, C + 3 * n, h.begin());
+    std::copy(U, U + 3 * n, h.begin() + 3 * n);
+    std::copy(V, V + 3 * n, h.begin() + 6 * n);
+    std::copy(S, S + 3 * n, h.begin() + 9 * n);
+    vec_.alloc(h.size() * sizeof(float));
+    cls_.alloc(cls.size() * sizeof(int));
+    RN_HIP_CHECK(hipMemcpyAsync(vec_.get(), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipMemcpyAsync(cls_.get(), cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+    geo = g;
+}
+
+void ConePlan::dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t st) const
+{
+    const ArbGeo& g = geo;
+    RN_REQUIRE(nB > 0 && nC > 0 && (long)g.nview * nB * nC <= 65535, "cone batch too large for one launch");
+    const float* v = vec_.as<float>();
+    dim3 blk(64), grd(cdiv(g.nv, 64), g.nu, g.nview * nB * nC);
+    hipLaunchKernelGGL(k_dd_cone_fp, grd, blk, 0, st, g, grid_of(g), img, prj, v, v + 3 * g.nview, v + 6 * g.nview,
+                       v + 9 * g.nview, cls_.as<int>(), nC);
+    check_launch("dd_cone_fp");
+}
+
+void ConePlan::dd_adj(int nB, int nC, const float* prj, float* img, hipStream_t st) const
+{
+    const ArbGeo& g = geo;
+    RN_REQUIRE(nB > 0 && nC > 0 && (long)g.nx * nB * nC <= 65535, "cone batch too large for one launch");
+    const float* v = vec_.as<float>();
+    dim3 blk(256), grd(cdiv(g.nz, 64), cdiv(g.ny, 4), g.nx * nB * nC);
+    hipLaunchKernelGGL(k_dd_cone_adj, grd, blk, 0, st, g, grid_of(g), prj, img, v, v + 3 * g.nview, v + 6 * g.nview,
+                       v + 9 * g.nview, cls_.as<int>(), nC);
+    check_launch("dd_cone_adj");
+}
+
+void ConePlan::siddon_fp(int nB, int nC, const float* img, float* prj, DevBuf& work, hipStream_t st) const
+{
+    const float* v = vec_.as<float>();
+    siddon_arb_fp(geo, false, nB, nC, img, prj, v, v + 3 * geo.nview, v + 6 * geo.nview, v + 9 * geo.nview, work, st);
+}
+
+void ConePlan::siddon_bp(int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st) const
+{
+    const float* v = vec_.as<float>();
+    siddon_arb_bp(geo, false, nB, nC, prj, img, v, v + 3 * geo.nview, v + 6 * geo.nview, v + 9 * geo.nview, work, st);
 }
 
 void os_sqs_cone(const ArbGeo& g, int nB, int nC, float* x, const float* prj, const float* norm, const float* mask,

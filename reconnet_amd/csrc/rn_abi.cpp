@@ -597,6 +597,63 @@ extern "C" int rn_fan_filter(void* plan, const float* prj, float* fprj, int nBat
     });
 }
 
+// ----- circular cone-beam plans -----
+
+namespace {
+struct ConeHandle {
+    ConePlan plan;
+    DevBuf work;
+};
+} // namespace
+
+extern "C" void* rn_cone_plan_create(int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy,
+        float cz, int nu, int nview, int nv, float du, float dv, float off_u, float off_v,
+        const float* detCenter, const float* detU, const float* detV, const float* src)
+{
+    ConeHandle* p = nullptr;
+    guarded("rn_cone_plan_create", [&] {
+        ArbGeo g{nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nview, nv, du, dv, off_u, off_v};
+        std::unique_ptr<ConeHandle> q(new ConeHandle());
+        CallStream s;
+        q->plan.setup(g, detCenter, detU, detV, src, s.get());
+        p = q.release();
+    });
+    return p;
+}
+
+extern "C" void rn_cone_plan_destroy(void* plan)
+{
+    guarded("rn_cone_plan_destroy", [&] { delete static_cast<ConeHandle*>(plan); });
+}
+
+extern "C" int rn_cone_forward(void* plan, int kind, const float* img, float* prj, int nBatches, int nChannels,
+                               void* stream)
+{
+    return guarded_status("rn_cone_forward", [&] {
+        RN_REQUIRE(plan != nullptr && img != nullptr && prj != nullptr, "null pointer");
+        RN_REQUIRE(nBatches > 0 && nChannels > 0, "nBatches and nChannels must be positive");
+        ConeHandle* p = static_cast<ConeHandle*>(plan);
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        if (kind == RN_CONE_DD) p->plan.dd_fp(nBatches, nChannels, img, prj, st);
+        else if (kind == RN_CONE_SIDDON) p->plan.siddon_fp(nBatches, nChannels, img, prj, p->work, st);
+        else throw Error("unknown cone projector kind");
+    });
+}
+
+extern "C" int rn_cone_backward(void* plan, int kind, const float* prj, float* img, int nBatches, int nChannels,
+                                void* stream)
+{
+    return guarded_status("rn_cone_backward", [&] {
+        RN_REQUIRE(plan != nullptr && img != nullptr && prj != nullptr, "null pointer");
+        RN_REQUIRE(nBatches > 0 && nChannels > 0, "nBatches and nChannels must be positive");
+        ConeHandle* p = static_cast<ConeHandle*>(plan);
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        if (kind == RN_CONE_DD) p->plan.dd_adj(nBatches, nChannels, prj, img, st);
+        else if (kind == RN_CONE_SIDDON) p->plan.siddon_bp(nBatches, nChannels, prj, img, p->work, st);
+        else throw Error("unknown cone backprojector kind");
+    });
+}
+
 extern "C" void* rn_sqs_create(void* plan, int nslices, int nviewPerSubset, int nSubsets)
 {
     Sqs* out = nullptr;

@@ -45,8 +45,8 @@ struct FanK {
     const float* bcos;
     // [nview][nu] line of the ray through bin centre k, in image coordinates:
     // X views X(Y) = A + B (Y - ny/2), Y views Y(X) = A + B (X - nx/2);
-    // (A, B, L, L / |B|) with L the path length per image line (dy / |cos(a - t)|
-    // or dx / |sin(a - t)|) and L / |B| = 0 when B = 0.  Computed in FP64.
+    // (A, B, L, 0) with L the path length per image line (dy / |cos(a - t)| or
+    // dx / |sin(a - t)|).  Computed in FP64.
     const float4* sray;
 };
 

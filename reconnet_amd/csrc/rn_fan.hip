@@ -320,8 +320,7 @@ void FanTables::build(const FanGeo& g, const float* deg_host, hipStream_t st)
                 B = ((double)g.dx / g.dy) * c / sn;
                 L = (double)g.dx / std::fabs(sn);
             }
-            const float Bf = (float)B;
-            ray[(size_t)v * nu + kb] = make_float4((float)A, Bf, (float)L, Bf != 0.0f ? (float)(L / std::fabs((double)Bf)) : 0.0f);
+            ray[(size_t)v * nu + kb] = make_float4((float)A, (float)B, (float)L, 0.0f);
         }
     }
     ray_.alloc(ray.size() * sizeof(float4));

@@ -130,7 +130,7 @@ __device__ __forceinline__ T seg_sum(const T* __restrict__ line, int n, float lo
     return s;
 }
 
-// seg_sum for hi - lo <= 2 (at most three cells), without branches: the three
+// seg_sum for hi - lo < 2 (at most three cells), without branches: the three
 // loads are independent and every weight max(0, min(h, m+1) - max(l, m)) is
 // zero for cells outside [l, h] (indices are clamped into the line).
 template <class T>
@@ -151,10 +151,23 @@ __device__ __forceinline__ T seg_sum3(const T* __restrict__ line, int n, float l
 
 // Siddon FP (exact ray / pixel intersection lengths), one line per image row
 // (X views) or column (Y views): the ray crosses line j over [j, j+1] of the
-// line axis, i.e. over [A + B (j - n/2), A + B (j + 1 - n/2)] of the cell axis,
-// with path length L per line; the length inside a cell is L times the cell's
-// share of that interval (overlap / |B|).  A ray with B = 0 crosses every line
-// at the same cell coordinate A, with length L in each.
+// line axis, i.e. over [lo, hi] = [A + B (j - n/2), A + B (j + 1 - n/2)] of the
+// cell axis, with path length L per line; the length inside a cell is L times
+// the cell's share of [lo, hi], overlap / (hi - lo) with hi - lo taken from the
+// same rounded end points (so a line's shares add up to one).  When the end
+// points round to the same value (a ray along the line axis) the cell holding
+// lo gets the whole length.
+template <class T>
+__device__ __forceinline__ T siddon_line_sum(const T* __restrict__ line, int n, float lo, float hi, bool narrow)
+{
+    const float w = hi - lo;
+    if (w > 0.0f) {
+        const T s = narrow ? seg_sum3<T>(line, n, lo, hi) : seg_sum<T>(line, n, lo, hi);
+        return s * (T)__builtin_amdgcn_rcpf(w);
+    }
+    return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : T(0);
+}
+
 template <class T>
 __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ Pyx, const T* __restrict__ Pxy,
                                            int view, int iu)
@@ -165,26 +178,12 @@ __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ 
     const int nl = xv ? g.ny : g.nx;     // lines
     const int n = xv ? g.nx : g.ny;      // cells per line
     const float base = -0.5f * (float)nl;
+    const bool narrow = fabsf(r.y) <= 1.999f;   // [lo, hi] (rounded) spans at most three cells
     T acc = T(0);
-    if (r.y != 0.0f) {
-        if (fabsf(r.y) <= 2.0f) {
-            for (int j = 0; j < nl; ++j) {
-                const float lr = base + (float)j;
-                const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
-                acc += seg_sum3<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
-            }
-        } else {
-            for (int j = 0; j < nl; ++j) {
-                const float lr = base + (float)j;
-                const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
-                acc += seg_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
-            }
-        }
-        return acc * (T)r.w;
-    }
-    if (r.x >= 0.0f && r.x < (float)n) {
-        const int m = (int)r.x;
-        for (int j = 0; j < nl; ++j) acc += P[(long)j * n + m];
+    for (int j = 0; j < nl; ++j) {
+        const float lr = base + (float)j;
+        const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+        acc += siddon_line_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb), narrow);
     }
     return acc * (T)r.z;
 }
@@ -230,6 +229,16 @@ __device__ __forceinline__ void bin_range(const FanK& g, float xc, float yc, flo
     k1 = min((int)floorf(uc + hw), g.nu - 1);
 }
 
+// Length of ray r in cell `cell` of line lr (the weight siddon_fp_ray gives it).
+__device__ __forceinline__ float siddon_cell_weight(const float4 r, float lr, float cell)
+{
+    const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+    const float lo = fminf(ea, eb), hi = fmaxf(ea, eb), w = hi - lo;
+    const float ov = fmaxf(fminf(hi, cell + 1.0f) - fmaxf(lo, cell), 0.0f);
+    const float share = w > 0.0f ? ov * __builtin_amdgcn_rcpf(w) : ((lo >= cell && lo < cell + 1.0f) ? 1.0f : 0.0f);
+    return share * r.z;
+}
+
 // Exact transpose of siddon_fp_ray: pixel (ix, iy) gathers, over views
 // view0 .. view0+nviews-1 of prj ([nviews][nu]), the rays that cross it, with
 // the weights siddon_fp_ray gives it.  The first NB bins of a pixel's range
@@ -260,18 +269,11 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
             const int kk = min(k0 + j, g.nu - 1);
             const float4 r = R[kk];
             const T pv = p[kk];
-            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
-            const float ov = fmaxf(fminf(fmaxf(ea, eb), cell + 1.0f) - fmaxf(fminf(ea, eb), cell), 0.0f) * r.w;
-            const float pt = (r.x >= cell && r.x < cell + 1.0f) ? r.z : 0.0f;
-            const float w = (k0 + j <= k1) ? (r.y != 0.0f ? ov : pt) : 0.0f;
+            const float w = (k0 + j <= k1) ? siddon_cell_weight(r, lr, cell) : 0.0f;
             acc += pv * (T)w;
         }
         for (int k = k0 + NB; k <= k1; ++k) {
-            const float4 r = R[k];
-            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
-            const float ov = fmaxf(fminf(fmaxf(ea, eb), cell + 1.0f) - fmaxf(fminf(ea, eb), cell), 0.0f) * r.w;
-            const float pt = (r.x >= cell && r.x < cell + 1.0f) ? r.z : 0.0f;
-            acc += p[k] * (T)(r.y != 0.0f ? ov : pt);
+            acc += p[k] * (T)siddon_cell_weight(R[k], lr, cell);
         }
     }
     return acc;

@@ -19,7 +19,7 @@ import torch
 from ._lib import check, lib
 from .geometry import FanGeometry
 
-__all__ = ["FanPlan", "FanProjection", "FP_KINDS", "BP_KINDS"]
+__all__ = ["FanPlan", "FanProjection", "ConePlan", "ConeProjection", "FP_KINDS", "BP_KINDS", "CONE_KINDS"]
 
 FP_KINDS = {"dd": 0, "siddon": 1}
 BP_KINDS = {"dd": 0, "dd_fbp": 1, "dd_adjoint": 2, "siddon": 3, "pixel": 4}
@@ -129,3 +129,82 @@ class FanProjection(torch.autograd.Function):
     def backward(ctx, gy):
         gx = ctx.plan.backward(gy.contiguous(), ADJOINT_OF[ctx.kind])
         return gx, None, None
+
+
+CONE_KINDS = {"dd": 0, "siddon": 1}
+
+
+def _cone_sigs():
+    h = lib()
+    if not getattr(h, "_cone_sigs", False):
+        ci, cf, P = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+        h.rn_cone_plan_create.restype = P
+        h.rn_cone_plan_create.argtypes = [ci, ci, ci, cf, cf, cf, cf, cf, cf, ci, ci, ci, cf, cf, cf, cf, P, P, P, P]
+        h.rn_cone_plan_destroy.restype = None
+        h.rn_cone_plan_destroy.argtypes = [P]
+        for n in ("rn_cone_forward", "rn_cone_backward"):
+            getattr(h, n).restype = ci
+            getattr(h, n).argtypes = [P, ci, P, P, ci, ci, P]
+        h._cone_sigs = True
+    return h
+
+
+class ConePlan:
+    """Circular cone-beam geometry on the current CUDA device.
+
+    Volumes [B, nx, ny, nz, C] and projections [B, nu, nview, nv, C] (the
+    reference layouts); det_center / det_u / det_v / src are [nview, 3] arrays as
+    for cSiddonConeProjectionAbitrary.  kind "dd": distance-driven pair whose
+    backward is the exact adjoint; kind "siddon": the Siddon pair."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, nu, nv, du, dv, det_center, det_u, det_v, src,
+                 cx=0.0, cy=0.0, cz=0.0, off_u=0.0, off_v=0.0):
+        h = _cone_sigs()
+        self.shape_img = (nx, ny, nz)
+        self.nu, self.nv = nu, nv
+        vecs = [np.ascontiguousarray(a, np.float32).reshape(-1, 3) for a in (det_center, det_u, det_v, src)]
+        self.nview = vecs[0].shape[0]
+        self._vecs = vecs
+        self._h = h.rn_cone_plan_create(nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, self.nview, nv, du, dv, off_u, off_v,
+                                        *[v.ctypes.data_as(ctypes.c_void_p) for v in vecs])
+        if not self._h:
+            check(-1, "rn_cone_plan_create")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rn_cone_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, x, kind="dd"):
+        B, C = x.shape[0], x.shape[4]
+        _check_tensor(x, (B,) + self.shape_img + (C,), "volume")
+        out = torch.empty((B, self.nu, self.nview, self.nv, C), dtype=torch.float32, device=x.device)
+        check(lib().rn_cone_forward(self._h, CONE_KINDS[kind], _ptr(x), _ptr(out), B, C, _stream()), "rn_cone_forward")
+        return out
+
+    def backward(self, y, kind="dd"):
+        B, C = y.shape[0], y.shape[4]
+        _check_tensor(y, (B, self.nu, self.nview, self.nv, C), "projections")
+        out = torch.empty((B,) + self.shape_img + (C,), dtype=torch.float32, device=y.device)
+        check(lib().rn_cone_backward(self._h, CONE_KINDS[kind], _ptr(y), _ptr(out), B, C, _stream()),
+              "rn_cone_backward")
+        return out
+
+
+class ConeProjection(torch.autograd.Function):
+    """y = A x with A the cone DD (or Siddon) projector; dL/dx = A^T dL/dy."""
+
+    @staticmethod
+    def forward(ctx, x, plan, kind="dd"):
+        ctx.plan, ctx.kind = plan, kind
+        return plan.forward(x.contiguous(), kind)
+
+    @staticmethod
+    def backward(ctx, gy):
+        return ctx.plan.backward(gy.contiguous(), ctx.kind), None, None
