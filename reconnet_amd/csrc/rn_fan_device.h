@@ -133,22 +133,6 @@ __device__ __forceinline__ T seg_sum(const T* __restrict__ line, int n, float lo
 // seg_sum for hi - lo < 2 (at most three cells), without branches: the three
 // loads are independent and every weight max(0, min(h, m+1) - max(l, m)) is
 // zero for cells outside [l, h] (indices are clamped into the line).
-template <class T>
-__device__ __forceinline__ T seg_sum3(const T* __restrict__ line, int n, float lo, float hi)
-{
-    const float fn = (float)n;
-    const float l = fminf(fmaxf(lo, 0.0f), fn), h = fminf(hi, fn);
-    const int m0 = (int)l;
-    T s = T(0);
-    #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        const int m = m0 + j;
-        const float w = fmaxf(fminf(h, (float)(m + 1)) - fmaxf(l, (float)m), 0.0f);
-        s += line[min(m, n - 1)] * (T)w;
-    }
-    return s;
-}
-
 // Siddon FP (exact ray / pixel intersection lengths), one line per image row
 // (X views) or column (Y views): the ray crosses line j over [j, j+1] of the
 // line axis, i.e. over [lo, hi] = [A + B (j - n/2), A + B (j + 1 - n/2)] of the
@@ -158,14 +142,35 @@ __device__ __forceinline__ T seg_sum3(const T* __restrict__ line, int n, float l
 // points round to the same value (a ray along the line axis) the cell holding
 // lo gets the whole length.
 template <class T>
-__device__ __forceinline__ T siddon_line_sum(const T* __restrict__ line, int n, float lo, float hi, bool narrow)
+__device__ __forceinline__ T siddon_line_sum(const T* __restrict__ line, int n, float lo, float hi)
 {
     const float w = hi - lo;
-    if (w > 0.0f) {
-        const T s = narrow ? seg_sum3<T>(line, n, lo, hi) : seg_sum<T>(line, n, lo, hi);
-        return s * (T)__builtin_amdgcn_rcpf(w);
-    }
+    if (w > 0.0f) return seg_sum<T>(line, n, lo, hi) * (T)__builtin_amdgcn_rcpf(w);
     return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : T(0);
+}
+
+// siddon_line_sum for hi - lo < 2 (at most three cells) without branches:
+// three independent loads (indices clamped into the line) and the weights
+// max(0, min(h, m + 1) - max(l, m)) of cells m0 .. m0 + 2, which vanish
+// outside [l, h]; a zero-width interval selects the cell holding lo.
+template <class T>
+__device__ __forceinline__ T siddon_line_sum3(const T* __restrict__ line, int n, float lo, float hi)
+{
+    const float fn = (float)n;
+    const float w = hi - lo;
+    const float l = fminf(fmaxf(lo, 0.0f), fn), h = fminf(hi, fn);
+    const int m0 = (int)l;                 // l in [m0, m0 + 1) unless l == n (then every weight is 0)
+    const float f0 = (float)m0;
+    float w0 = fmaxf(fminf(h, f0 + 1.0f) - l, 0.0f);
+    const float w1 = fmaxf(fminf(h, f0 + 2.0f) - (f0 + 1.0f), 0.0f);
+    const float w2 = fmaxf(fminf(h, f0 + 3.0f) - (f0 + 2.0f), 0.0f);
+    const bool pos = w > 0.0f;
+    w0 = pos ? w0 : ((lo >= 0.0f && lo < fn) ? 1.0f : 0.0f);
+    const unsigned last = (unsigned)(n - 1);
+    const T v0 = line[min((unsigned)m0, last)];
+    const T v1 = line[min((unsigned)m0 + 1u, last)];
+    const T v2 = line[min((unsigned)m0 + 2u, last)];
+    return (v0 * (T)w0 + v1 * (T)w1 + v2 * (T)w2) * (T)(pos ? __builtin_amdgcn_rcpf(w) : 1.0f);
 }
 
 template <class T>
@@ -178,12 +183,20 @@ __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ 
     const int nl = xv ? g.ny : g.nx;     // lines
     const int n = xv ? g.nx : g.ny;      // cells per line
     const float base = -0.5f * (float)nl;
-    const bool narrow = fabsf(r.y) <= 1.999f;   // [lo, hi] (rounded) spans at most three cells
     T acc = T(0);
-    for (int j = 0; j < nl; ++j) {
-        const float lr = base + (float)j;
-        const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
-        acc += siddon_line_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb), narrow);
+    if (fabsf(r.y) <= 1.999f) {          // [lo, hi] (rounded) spans at most three cells
+        #pragma unroll 4
+        for (int j = 0; j < nl; ++j) {
+            const float lr = base + (float)j;
+            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+            acc += siddon_line_sum3<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+        }
+    } else {
+        for (int j = 0; j < nl; ++j) {
+            const float lr = base + (float)j;
+            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+            acc += siddon_line_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+        }
     }
     return acc * (T)r.z;
 }
