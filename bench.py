@@ -43,6 +43,20 @@ PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (lines "traffic <kernel> <bytes> ..."), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_fan512.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "traffic" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -311,8 +325,8 @@ def main():
     avg_ms = tot_ms / steps_k
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": name, "avg_launch_ms": avg_ms,
-                "bytes_per_launch": nbytes, "launches": steps_k}
+                "frac": achieved / PEAK_HBM_GBS, "traffic": profiled_traffic("k_" + name), "kernel": name,
+                "avg_launch_ms": avg_ms, "bytes_per_launch": nbytes, "launches": steps_k}
     per_kernel = {k: {"avg_ms": v[0] / steps_k, "bytes_per_launch": v[1],
                       "gbs": v[1] / (v[0] / steps_k * 1e-3) / 1e9} for k, v in kernels.items()}
     if conv_ms is not None:

@@ -166,11 +166,12 @@ __device__ __forceinline__ T siddon_line_sum3(const T* __restrict__ line, int n,
     const float w2 = fmaxf(fminf(h, f0 + 3.0f) - (f0 + 2.0f), 0.0f);
     const bool pos = w > 0.0f;
     w0 = pos ? w0 : ((lo >= 0.0f && lo < fn) ? 1.0f : 0.0f);
+    const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
     const unsigned last = (unsigned)(n - 1);
     const T v0 = line[min((unsigned)m0, last)];
     const T v1 = line[min((unsigned)m0 + 1u, last)];
     const T v2 = line[min((unsigned)m0 + 2u, last)];
-    return (v0 * (T)w0 + v1 * (T)w1 + v2 * (T)w2) * (T)(pos ? __builtin_amdgcn_rcpf(w) : 1.0f);
+    return (v0 * (T)w0 + v1 * (T)w1 + v2 * (T)w2) * (T)inv;
 }
 
 template <class T>
@@ -183,16 +184,29 @@ __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ 
     const int nl = xv ? g.ny : g.nx;     // lines
     const int n = xv ? g.nx : g.ny;      // cells per line
     const float base = -0.5f * (float)nl;
+    // lines whose interval can meet [0, n]: A + B (j - nl/2) in [-|B| - 1, n + 1]
+    // (one line of margin on each side; the other lines add exact zeros)
+    int j0 = 0, j1 = nl - 1;
+    if (r.y != 0.0f) {
+        const float ib = 1.0f / r.y;
+        const float ta = (-fabsf(r.y) - 1.0f - r.x) * ib - base, tb = ((float)n + 1.0f - r.x) * ib - base;
+        const float tlo = fminf(fmaxf(fminf(ta, tb) - 1.0f, 0.0f), (float)nl);
+        const float thi = fminf(fmaxf(ta, tb) + 1.0f, (float)(nl - 1));
+        j0 = (int)tlo;
+        j1 = thi >= tlo ? (int)thi : -1;
+    } else if (!(r.x >= 0.0f && r.x < (float)n)) {
+        j1 = -1;
+    }
     T acc = T(0);
     if (fabsf(r.y) <= 1.999f) {          // [lo, hi] (rounded) spans at most three cells
         #pragma unroll 4
-        for (int j = 0; j < nl; ++j) {
+        for (int j = j0; j <= j1; ++j) {
             const float lr = base + (float)j;
             const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
             acc += siddon_line_sum3<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
         }
     } else {
-        for (int j = 0; j < nl; ++j) {
+        for (int j = j0; j <= j1; ++j) {
             const float lr = base + (float)j;
             const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
             acc += siddon_line_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
@@ -242,14 +256,17 @@ __device__ __forceinline__ void bin_range(const FanK& g, float xc, float yc, flo
     k1 = min((int)floorf(uc + hw), g.nu - 1);
 }
 
-// Length of ray r in cell `cell` of line lr (the weight siddon_fp_ray gives it).
+// Length of ray r in cell `cell` of line lr (the weight siddon_fp_ray gives
+// it), computed without branches.
 __device__ __forceinline__ float siddon_cell_weight(const float4 r, float lr, float cell)
 {
     const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
     const float lo = fminf(ea, eb), hi = fmaxf(ea, eb), w = hi - lo;
     const float ov = fmaxf(fminf(hi, cell + 1.0f) - fmaxf(lo, cell), 0.0f);
-    const float share = w > 0.0f ? ov * __builtin_amdgcn_rcpf(w) : ((lo >= cell && lo < cell + 1.0f) ? 1.0f : 0.0f);
-    return share * r.z;
+    const bool pos = w > 0.0f;
+    const float pt = (lo >= cell && lo < cell + 1.0f) ? 1.0f : 0.0f;
+    const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
+    return (pos ? ov * inv : pt) * r.z;
 }
 
 // Exact transpose of siddon_fp_ray: pixel (ix, iy) gathers, over views
@@ -282,8 +299,8 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
             const int kk = min(k0 + j, g.nu - 1);
             const float4 r = R[kk];
             const T pv = p[kk];
-            const float w = (k0 + j <= k1) ? siddon_cell_weight(r, lr, cell) : 0.0f;
-            acc += pv * (T)w;
+            const float valid = (k0 + j <= k1) ? 1.0f : 0.0f;
+            acc += pv * (T)(siddon_cell_weight(r, lr, cell) * valid);
         }
         for (int k = k0 + NB; k <= k1; ++k) {
             acc += p[k] * (T)siddon_cell_weight(R[k], lr, cell);

@@ -11,7 +11,8 @@ Every slice starts from its FBP image (Hann-windowed ramp, FBP-weighted DD
 backprojection), as in the reconstruction itself.  Training is greedy: stage k
 takes the current images of all slices, applies one
 OS-SQS data pass (12 subsets), and fits the stage-k CNN prior (initialised from
-stage k-1) to map the result to the phantom (masked MSE in HU^2, Adam).  The
+stage k-1) to map the result to the phantom (Adam on the batch mean of the
+per-slice RMSE in HU over the field of view, the evaluation metric).  The
 trained stage is applied to every slice before the next stage starts.  Every
 forward and backward convolution runs on the library's BF16 MFMA kernels.
 """
@@ -67,8 +68,8 @@ def main(argv=None):
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--subsets", type=int, default=12)
     ap.add_argument("--i0", type=float, default=2.5e4)
-    ap.add_argument("--epochs", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--scale", type=float, default=50.0)
     ap.add_argument("--out", default="gpurun_out/unrolled_fan512.npz")
@@ -110,24 +111,28 @@ def main(argv=None):
         opt = torch.optim.Adam(net.parameters(), lr=args.lr)
         nsteps = args.epochs * (args.train // args.batch)
         sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, nsteps)
-        losses = []
+        losses, epoch_means = [], []
+        npix = float(mask.sum())
         for ep in range(args.epochs):
             perm = torch.randperm(args.train, generator=gen).cuda()
+            first = len(losses)
             for i in range(0, args.train - args.batch + 1, args.batch):
                 idx = perm[i:i + args.batch]
                 xb = xin.index_select(0, idx)
                 tb = truth.index_select(0, idx)
                 y = net.forward(xb, keep=True)
                 r = (y - tb) * mask
-                cnt = float(mask.sum()) * args.batch
-                losses.append(float((r * r).sum()) / cnt * HU * HU)
-                grads = net.backward(2.0 * HU * HU * r / cnt)
+                # loss: mean over the batch of the per-slice RMSE in HU (the evaluation metric)
+                rms = torch.sqrt((r * r).sum(dim=(1, 2)) / npix).clamp_min(1e-12)
+                losses.append(float(HU * rms.mean()))
+                grads = net.backward((HU / (args.batch * npix)) * r / rms[:, None, None])
                 opt.zero_grad(set_to_none=False)
                 for name, p in net.p.items():
                     p.grad = grads[name].reshape(p.shape).contiguous()
                 opt.step()
                 sched.step()
                 net.refresh()
+            epoch_means.append(float(np.mean(losses[first:])))
         with torch.no_grad():
             xout = torch.cat([net.forward(xin[i:i + 64]) for i in range(0, args.train, 64)])
             vout = net.forward(vin)
@@ -137,10 +142,11 @@ def main(argv=None):
         v_after = float(masked_rmse_hu(vout, vtruth, mask).mean())
         t_after = float(masked_rmse_hu(xout, truth, mask).mean())
         history.append({"stage": k, "loss_first": losses[0], "loss_last": float(np.mean(losses[-50:])),
+                        "epoch_mean_rmse_hu": epoch_means,
                         "val_rmse_in": v_before, "val_rmse_out": v_after, "train_rmse_out": t_after,
                         "seconds": time.time() - ts})
-        log("stage %d: val RMSE %.2f -> %.2f HU (train %.2f), loss %.1f -> %.1f, %.1f s"
-            % (k, v_before, v_after, t_after, losses[0], history[-1]["loss_last"], time.time() - ts))
+        log("stage %d: val RMSE %.2f -> %.2f HU (train %.2f), epoch mean batch RMSE %s, %.1f s"
+            % (k, v_before, v_after, t_after, " ".join("%.1f" % v for v in epoch_means), time.time() - ts))
         nets.append(net)
         params = {n: p.detach().clone() for n, p in net.p.items()}
         del xin, xout
