@@ -143,6 +143,7 @@ def main():
     log("workload: %d slices %dx%d, %d views, %d bins, I0 %g, learned=%s"
         % (ns, geo.nx, geo.ny, geo.nview, geo.nu, i0, nets is not None))
 
+    from reconnet_amd.metrics import convergence_pass
     from reconnet_amd.sqs import fbp_initial_image
 
     plan = FanPlan(geo)
@@ -204,11 +205,7 @@ def main():
     log("convergence run on %d slices: %.1f s (FBP start %.2f HU)" % (ne, time.perf_counter() - t0, fbp_rmse))
     rm = np.array(rm)
     r_final = float(rm[-1].mean())
-    k_conv = args.kmax
-    for k in range(args.kmax):
-        if rm[k].mean() <= 1.01 * r_final and upd[k] < 1e-4:
-            k_conv = k + 1
-            break
+    k_conv = convergence_pass(rm.mean(axis=1), upd)
     k_conv += k_conv % 2 * (args.subsets % 2)     # pass graphs need an even subset-step count
     r_star = float(rm[min(k_conv, args.kmax) - 1].mean())
     r_best = float(rm.mean(axis=1).min())

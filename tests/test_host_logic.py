@@ -104,3 +104,28 @@ def test_workloads_match_benchmark_configs():
     assert (c1.nx, c1.nu, c1.nview, c1.dx) == (512, 736, 1152, 0.7)
     assert c1.da == pytest.approx(1.0 / 1085.6)
     assert c1.fov_radius() == pytest.approx(179.2, abs=0.1)
+
+
+def test_convergence_rule():
+    from reconnet_amd.metrics import convergence_pass
+
+    # RMSE reaches 1% of its final value at pass 3, the update drops below 1e-4 at pass 5
+    r = [10.0, 5.0, 4.04, 4.02, 4.01, 4.0]
+    u = [1e-2, 1e-3, 5e-4, 2e-4, 9e-5, 5e-5]
+    assert convergence_pass(r, u) == 5
+    # never converged: k_max
+    assert convergence_pass(r, [1.0] * 6) == 6
+    # a dip below the final RMSE counts only with a small update
+    assert convergence_pass([9.0, 3.0, 4.0], [1e-5, 1e-5, 1e-5]) == 2
+    with pytest.raises(ValueError):
+        convergence_pass([1.0], [1.0, 2.0])
+
+
+def test_rmse_hu_scale():
+    from reconnet_amd.metrics import rmse_hu
+
+    t = np.zeros((2, 4, 4))
+    x = t.copy()
+    x[1] += 0.019      # one mu_water everywhere = 1000 HU
+    m = np.ones((4, 4))
+    assert np.allclose(rmse_hu(x, t, m), [0.0, 1000.0])
