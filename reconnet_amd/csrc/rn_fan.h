@@ -48,6 +48,7 @@ struct FanK {
     // (A, B, L, 0) with L the path length per image line (dy / |cos(a - t)| or
     // dx / |sin(a - t)|).  Computed in FP64.
     const float4* sray;
+    const int* vwide;    // [nview] 1: every Siddon line interval of the view has a positive FP32 width
 };
 
 // Device tables for one (geometry, angle set).  Angles are kept on the host

@@ -43,16 +43,32 @@ __global__ __launch_bounds__(BLK) void k_dd_fan_fp(FanK g, const float* __restri
 
 // With meas != nullptr the epilogue writes the weighted residual
 // wgt * (Ax - meas) of the OS-SQS data step (ossqs.cu:101-122) instead of Ax.
-template <class T, int BLK>
+template <class T, int BLK, int XCD>
 __global__ __launch_bounds__(BLK) void k_siddon_fan_fp(FanK g, const T* __restrict__ Pyx,
                                                        const T* __restrict__ Pxy, T* __restrict__ prj,
                                                        int view0, int nviews,
                                                        const float* __restrict__ meas, const float* __restrict__ wgt,
                                                        int meas_nviews)
 {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int vi = blockIdx.y;
-    const int s = blockIdx.z;
+    // XCD > 0: 1-D grid of tiles * nviews * nslices workgroups (a multiple of
+    // XCD).  The hardware deals workgroups round-robin to the XCDs, so
+    // workgroup h runs on XCD h % XCD; h is renumbered so that each XCD walks a
+    // contiguous, slice-major range of (slice, view, bin tile) and its L2 holds
+    // the images of only the slice or two it is working on.
+    int tile, vi, s;
+    if (XCD > 0) {
+        const int per = gridDim.x / XCD;
+        const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
+        const int tiles = (g.nu + BLK - 1) / BLK;
+        tile = l % tiles;
+        vi = (l / tiles) % nviews;
+        s = l / (tiles * nviews);
+    } else {
+        tile = blockIdx.x;
+        vi = blockIdx.y;
+        s = blockIdx.z;
+    }
+    const int iu = tile * BLK + threadIdx.x;
     if (iu >= g.nu) return;
     const int view = view0 + vi;
     const long plane = (long)g.nx * g.ny;
@@ -323,10 +339,26 @@ void FanTables::build(const FanGeo& g, const float* deg_host, hipStream_t st)
             ray[(size_t)v * nu + kb] = make_float4((float)A, (float)B, (float)L, 0.0f);
         }
     }
-    ray_.alloc(ray.size() * sizeof(float4));
-    RN_HIP_CHECK(hipMemcpyAsync(ray_.get(), ray.data(), ray.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    // per view: 1 if every ray's line interval has a positive FP32 width on
+    // every image line, i.e. |B| exceeds the rounding of the end points
+    // A + B (j - n/2) (a few ulp of their largest magnitude) by a wide margin
+    std::vector<int> wide(nview, 1);
+    for (int v = 0; v < nview; v++) {
+        const int nl = cls[v] == 0 ? g.ny : g.nx;
+        for (int kb = 0; kb < nu && wide[v]; kb++) {
+            const float4 r = ray[(size_t)v * nu + kb];
+            const double emax = std::fabs((double)r.x) + std::fabs((double)r.y) * (0.5 * nl + 1.0) + 1.0;
+            if (!(std::fabs((double)r.y) > 1e-6 * emax)) wide[v] = 0;
+        }
+    }
+    const size_t ray_bytes = ray.size() * sizeof(float4);
+    ray_.alloc(ray_bytes + wide.size() * sizeof(int));
+    RN_HIP_CHECK(hipMemcpyAsync(ray_.get(), ray.data(), ray_bytes, hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipMemcpyAsync(static_cast<char*>(ray_.get()) + ray_bytes, wide.data(), wide.size() * sizeof(int),
+                                hipMemcpyHostToDevice, st));
     RN_HIP_CHECK(hipStreamSynchronize(st));
     k.sray = reinterpret_cast<const float4*>(ray_.get());
+    k.vwide = reinterpret_cast<const int*>(static_cast<const char*>(ray_.get()) + ray_bytes);
 }
 
 bool FanTables::matches(const FanGeo& g, const float* deg_host) const
@@ -348,12 +380,21 @@ void go_dd_fp(const FanK& g, const float* RX, const float* RY, float* prj, int n
                        view0, nviews);
 }
 
-template <int BLK>
+// MI355X: 8 XCDs, each with its own L2
+constexpr int kXcds = 8;
+
+template <int BLK, int XCD>
 void go_siddon_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj, int ns, int view0, int nviews,
                   const float* meas, const float* w, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK>), dim3(cdiv(g.nu, BLK), nviews, ns), dim3(BLK), 0, st, g, Pyx,
-                       Pxy, prj, view0, nviews, meas, w, g.nview);
+    const long nblk = (long)cdiv(g.nu, BLK) * nviews * ns;
+    if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
+        hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK, XCD>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Pyx, Pxy,
+                           prj, view0, nviews, meas, w, g.nview);
+    } else {
+        hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK, 0>), dim3(cdiv(g.nu, BLK), nviews, ns), dim3(BLK), 0, st, g,
+                           Pyx, Pxy, prj, view0, nviews, meas, w, g.nview);
+    }
 }
 
 template <int BX, int BY, int NB>
@@ -386,7 +427,8 @@ using DdBpFn = void (*)(const FanK&, const float*, float*, int, int, hipStream_t
 
 // 1-D blocks along the detector; 2-D pixel tiles (x lanes along y, the contiguous image axis)
 const DdFpFn kDdFp[] = {go_dd_fp<128>, go_dd_fp<64>, go_dd_fp<256>};
-const SidFpFn kSidFp[] = {go_siddon_fp<128>, go_siddon_fp<64>, go_siddon_fp<256>};
+const SidFpFn kSidFp[] = {go_siddon_fp<128, 0>, go_siddon_fp<64, 0>, go_siddon_fp<256, 0>,
+                          go_siddon_fp<128, kXcds>, go_siddon_fp<64, kXcds>, go_siddon_fp<256, kXcds>};
 const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 4>, go_siddon_bp<8, 32, 4>, go_siddon_bp<32, 8, 4>,
                            go_siddon_bp<8, 8, 4>,   go_siddon_bp<16, 16, 3>, go_siddon_bp<8, 32, 3>,
                            go_siddon_bp<32, 8, 3>,  go_siddon_bp<8, 8, 3>};
@@ -409,7 +451,7 @@ void launch_siddon_fan_fp(const FanK& g, const float* Pyx, const float* Pxy, flo
                           int nslices, int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 3, [&](int i) {
+    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 6, [&](int i) {
         kSidFp[i](g, Pyx, Pxy, prj_vu, nslices, view0, nviews, nullptr, nullptr, st);
     }, st, true);
     kSidFp[c](g, Pyx, Pxy, prj_vu, nslices, view0, nviews, nullptr, nullptr, st);
@@ -421,7 +463,7 @@ void launch_siddon_fan_fp_residual(const FanK& g, const float* Pyx, const float*
                                    hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 3, [&](int i) {
+    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 6, [&](int i) {
         kSidFp[i](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
     }, st, true);
     kSidFp[c](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
@@ -433,7 +475,7 @@ void launch_siddon_fan_fp_f64(const FanK& g, const double* Pyx, const double* Px
 {
     if (nslices <= 0 || nviews <= 0) return;
     dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
-    hipLaunchKernelGGL((k_siddon_fan_fp<double, 128>), grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
+    hipLaunchKernelGGL((k_siddon_fan_fp<double, 128, 0>), grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
                        (const float*)nullptr, (const float*)nullptr, 0);
     check_launch("siddon_fan_fp_f64");
 }

@@ -257,16 +257,38 @@ __device__ __forceinline__ void bin_range(const FanK& g, float xc, float yc, flo
 }
 
 // Length of ray r in cell `cell` of line lr (the weight siddon_fp_ray gives
-// it), computed without branches.
+// it), computed without branches.  WIDE: the interval's FP32 width is known to
+// be positive (no zero-width case to select).
+template <bool WIDE>
 __device__ __forceinline__ float siddon_cell_weight(const float4 r, float lr, float cell)
 {
     const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
     const float lo = fminf(ea, eb), hi = fmaxf(ea, eb), w = hi - lo;
     const float ov = fmaxf(fminf(hi, cell + 1.0f) - fmaxf(lo, cell), 0.0f);
+    if (WIDE) return ov * __builtin_amdgcn_rcpf(w) * r.z;
     const bool pos = w > 0.0f;
     const float pt = (lo >= cell && lo < cell + 1.0f) ? 1.0f : 0.0f;
     const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
     return (pos ? ov * inv : pt) * r.z;
+}
+
+// the bins k0 .. k1 of one view: the first NB without branches (clamped,
+// zero-weighted when past k1), the rest in a loop
+template <class T, int NB, bool WIDE>
+__device__ __forceinline__ T siddon_gather_bins(const float4* __restrict__ R, const T* __restrict__ p, int nu, int k0,
+                                                int k1, float lr, float cell)
+{
+    T acc = T(0);
+    #pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int kk = min(k0 + j, nu - 1);
+        const float4 r = R[kk];
+        const T pv = p[kk];
+        const float valid = (k0 + j <= k1) ? 1.0f : 0.0f;
+        acc += pv * (T)(siddon_cell_weight<WIDE>(r, lr, cell) * valid);
+    }
+    for (int k = k0 + NB; k <= k1; ++k) acc += p[k] * (T)siddon_cell_weight<WIDE>(R[k], lr, cell);
+    return acc;
 }
 
 // Exact transpose of siddon_fp_ray: pixel (ix, iy) gathers, over views
@@ -292,19 +314,8 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
         const float cell = xv ? (float)ix : (float)iy;
         const float4* __restrict__ R = g.sray + (long)view * g.nu;
         const T* __restrict__ p = prj + (long)vi * g.nu;
-        // the first NB bins without branches (clamped, zero-weighted when
-        // past k1), the rest in a loop
-        #pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const int kk = min(k0 + j, g.nu - 1);
-            const float4 r = R[kk];
-            const T pv = p[kk];
-            const float valid = (k0 + j <= k1) ? 1.0f : 0.0f;
-            acc += pv * (T)(siddon_cell_weight(r, lr, cell) * valid);
-        }
-        for (int k = k0 + NB; k <= k1; ++k) {
-            acc += p[k] * (T)siddon_cell_weight(R[k], lr, cell);
-        }
+        if (g.vwide[view]) acc += siddon_gather_bins<T, NB, true>(R, p, g.nu, k0, k1, lr, cell);
+        else acc += siddon_gather_bins<T, NB, false>(R, p, g.nu, k0, k1, lr, cell);
     }
     return acc;
 }
