@@ -69,10 +69,11 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp(FanK g, const T* __restri
         s = blockIdx.z;
     }
     const int iu = tile * BLK + threadIdx.x;
-    if (iu >= g.nu) return;
+    const bool active = iu < g.nu;
     const int view = view0 + vi;
     const long plane = (long)g.nx * g.ny;
-    T acc = siddon_fp_ray<T>(g, Pyx + s * plane, Pxy + s * plane, view, iu);
+    T acc = siddon_fp_ray<T>(g, Pyx + s * plane, Pxy + s * plane, view, active ? iu : g.nu - 1, active);
+    if (!active) return;
     if (meas != nullptr) {
         const long mi = ((long)s * meas_nviews + view) * g.nu + iu;
         acc = (T)wgt[mi] * (acc - (T)meas[mi]);

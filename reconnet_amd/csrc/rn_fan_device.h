@@ -174,9 +174,25 @@ __device__ __forceinline__ T siddon_line_sum3(const T* __restrict__ line, int n,
     return (v0 * (T)w0 + v1 * (T)w1 + v2 * (T)w2) * (T)inv;
 }
 
+// wave-wide minimum / maximum (every lane of the wavefront must take part)
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// One ray (bin iu of `view`); every lane of the wavefront calls it (inactive
+// lanes with active = false and any valid iu).  The lines are walked over the
+// union of the lanes' line ranges, so the loop and the line addresses are
+// wave-uniform; lanes outside their own range add exact zeros.
 template <class T>
 __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ Pyx, const T* __restrict__ Pxy,
-                                           int view, int iu)
+                                           int view, int iu, bool active = true)
 {
     const float4 r = g.sray[(long)view * g.nu + iu];
     const bool xv = g.vcls[view] == 0;
@@ -197,16 +213,24 @@ __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ 
     } else if (!(r.x >= 0.0f && r.x < (float)n)) {
         j1 = -1;
     }
+    if (!active) { j0 = nl; j1 = -1; }
+    const int wj0 = __builtin_amdgcn_readfirstlane(wave_min(j0));
+    const int wj1 = __builtin_amdgcn_readfirstlane(wave_max(j1));
+    // [lo, hi] (rounded) spans at most three cells on every lane
+    const bool narrow = __all(!active || fabsf(r.y) <= 1.999f);
     T acc = T(0);
-    if (fabsf(r.y) <= 1.999f) {          // [lo, hi] (rounded) spans at most three cells
+    if (narrow) {
+        // line j ends where line j + 1 starts: base + j + 1 is exact, so the
+        // shared end point is the same FMA result for both lines
+        float ea = fmaf(r.y, base + (float)wj0, r.x);
         #pragma unroll 4
-        for (int j = j0; j <= j1; ++j) {
-            const float lr = base + (float)j;
-            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+        for (int j = wj0; j <= wj1; ++j) {
+            const float eb = fmaf(r.y, base + (float)(j + 1), r.x);
             acc += siddon_line_sum3<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+            ea = eb;
         }
     } else {
-        for (int j = j0; j <= j1; ++j) {
+        for (int j = wj0; j <= wj1; ++j) {
             const float lr = base + (float)j;
             const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
             acc += siddon_line_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
@@ -314,6 +338,7 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
         const float cell = xv ? (float)ix : (float)iy;
         const float4* __restrict__ R = g.sray + (long)view * g.nu;
         const T* __restrict__ p = prj + (long)vi * g.nu;
+        if (k1 < k0) continue;                  // the pixel is outside this view's fan
         if (g.vwide[view]) acc += siddon_gather_bins<T, NB, true>(R, p, g.nu, k0, k1, lr, cell);
         else acc += siddon_gather_bins<T, NB, false>(R, p, g.nu, k0, k1, lr, cell);
     }

@@ -55,9 +55,13 @@ __global__ __launch_bounds__(256) void k_ragged_fp(const RagSlice* __restrict__ 
         const RagSlice& d = sl[find_slice(sl, nsl, item)];
         const int local = item - d.item0;
         const int view = local / d.tiles, iu = (local % d.tiles) * 64 + lane;
-        if (iu < d.g.nu) {
-            const float v = siddon ? siddon_fp_ray<float>(d.g, d.pa, d.pb, view, iu) : dd_fp_ray(d.g, d.pa, d.pb, view, iu);
-            d.out[(long)iu * d.g.nview + view] = v;
+        const bool active = iu < d.g.nu;
+        if (siddon) {
+            // every lane takes part (wave-uniform line loop)
+            const float v = siddon_fp_ray<float>(d.g, d.pa, d.pb, view, active ? iu : d.g.nu - 1, active);
+            if (active) d.out[(long)iu * d.g.nview + view] = v;
+        } else if (active) {
+            d.out[(long)iu * d.g.nview + view] = dd_fp_ray(d.g, d.pa, d.pb, view, iu);
         }
     }
 }

@@ -129,3 +129,16 @@ def test_rmse_hu_scale():
     x[1] += 0.019      # one mu_water everywhere = 1000 HU
     m = np.ones((4, 4))
     assert np.allclose(rmse_hu(x, t, m), [0.0, 1000.0])
+
+
+def test_custom_ops_registered():
+    import torch
+
+    from reconnet_amd import ops
+
+    for name in ("fan_forward", "fan_backward", "cone_forward", "cone_backward"):
+        assert hasattr(torch.ops.reconnet, name)
+    with pytest.raises(TypeError):
+        ops.register_plan(object())
+    with pytest.raises(ValueError):
+        ops.fan_backproject(torch.zeros(1, 2, 3), 1, "nonsense")
