@@ -430,9 +430,8 @@ using DdBpFn = void (*)(const FanK&, const float*, float*, int, int, hipStream_t
 const DdFpFn kDdFp[] = {go_dd_fp<128>, go_dd_fp<64>, go_dd_fp<256>};
 const SidFpFn kSidFp[] = {go_siddon_fp<128, 0>, go_siddon_fp<64, 0>, go_siddon_fp<256, 0>,
                           go_siddon_fp<128, kXcds>, go_siddon_fp<64, kXcds>, go_siddon_fp<256, kXcds>};
-const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 4>, go_siddon_bp<8, 32, 4>, go_siddon_bp<32, 8, 4>,
-                           go_siddon_bp<8, 8, 4>,   go_siddon_bp<16, 16, 3>, go_siddon_bp<8, 32, 3>,
-                           go_siddon_bp<32, 8, 3>,  go_siddon_bp<8, 8, 3>};
+const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 3>, go_siddon_bp<8, 8, 3>, go_siddon_bp<16, 16, 2>,
+                           go_siddon_bp<8, 8, 2>,   go_siddon_bp<16, 16, 4>, go_siddon_bp<8, 8, 4>};
 const GatherFn kDdAdj[] = {go_dd_adj<16, 16>, go_dd_adj<8, 32>, go_dd_adj<32, 8>, go_dd_adj<8, 8>};
 const DdBpFn kDdBp[] = {go_dd_bp<16, 16>, go_dd_bp<8, 32>, go_dd_bp<32, 8>, go_dd_bp<8, 8>};
 
@@ -494,7 +493,7 @@ void launch_siddon_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int
                           int view0, int nviews, bool accumulate, hipStream_t st)
 {
     if (nslices <= 0) return;
-    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, 8, [&](int i) {
+    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, 6, [&](int i) {
         kSidBp[i](g, prj_vu, img_xy, nslices, view0, nviews, 0, st);
     }, st, !accumulate);
     kSidBp[c](g, prj_vu, img_xy, nslices, view0, nviews, accumulate ? 1 : 0, st);
