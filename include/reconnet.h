@@ -239,6 +239,15 @@ int rn_ragged_forward(void* batch, int kind /* RN_FP_DD | RN_FP_SIDDON */, const
 int rn_ragged_backward(void* batch, int kind /* RN_BP_DD_ADJOINT | RN_BP_SIDDON */, const float* prj, float* img,
         void* stream);
 
+/* Isotropic TV separable surrogate of PriorFunctionSolver/TVSQS.cu:8-344
+ * (device pointers, layout [nb][nx][ny][nz][nc], nz ignored for dims == 2):
+ * var = sqrt(|backward differences|^2 + eps), s1 = gradient of sum(var),
+ * s2 = dims / var + sum over axes of 1 / var(next voxel).  The same kernels
+ * serve the host-pointer cTVSQS2D / cTVSQS3D of libPriorFunctionSolver.so
+ * (include/priorfunctionsolver.h). */
+int rn_tv_iso_sqs(const float* img, float* s1, float* s2, float* var, int nb, int nx, int ny, int nz, int nc,
+        int dims, float eps, void* stream);
+
 /* Circular cone-beam plans (no reference counterpart; the reference's DD 3D
  * pair, cDistanceDrivenTomo*, is tomosynthesis only).  Per-view detector
  * centre, axes and source as in cSiddonConeProjectionAbitrary (host arrays

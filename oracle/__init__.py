@@ -217,6 +217,21 @@ def siddon_cone_bp(prj, det_center, det_u, det_v, src, geo, img_shape, precision
     return _arb_bp("siddon_cone_bp", prj, [det_center, det_u, det_v, src], geo, img_shape, precision)
 
 
+def tv_iso_sqs(img, dims, eps=1e-8):
+    """(s1, s2, var) of the isotropic TV surrogate; img [nb, nx, ny, nc] (dims 2)
+    or [nb, nx, ny, nz, nc] (dims 3)."""
+    img = _f32(img)
+    if dims == 2:
+        nb, nx, ny, nc = img.shape
+        nz = 1
+    else:
+        nb, nx, ny, nz, nc = img.shape
+    out = [np.zeros(img.shape, np.float64) for _ in range(3)]
+    lib().orc_tv_iso_sqs_f64(*[_p(o) for o in out], _p(img), ctypes.c_int(nb), ctypes.c_int(nx), ctypes.c_int(ny),
+                             ctypes.c_int(nz), ctypes.c_int(nc), ctypes.c_int(dims), ctypes.c_double(eps))
+    return tuple(out)
+
+
 def dd_cone_fp(img, det_center, det_u, det_v, src, geo, nu, nv):
     return _arb_fp("dd_cone_fp", img, [det_center, det_u, det_v, src], geo, nu, nv, "f64")
 

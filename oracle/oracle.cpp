@@ -1079,6 +1079,49 @@ void dd_cone_adj(R* img, const float* prj, const float* C, const float* U, const
                     }
 }
 
+// ---------------------------------------------------------------------------
+// Isotropic TV separable surrogate: PriorFunctionSolver/TVSQS.cu:8-100
+// (2D: Variation2DKernel, TVSQS2DKernel) and :165-288 (3D).  Layout
+// [nb][nx][ny][nz][nc]; backward differences are 0 at the low edge, forward
+// ones 0 at the high edge where the neighbour's variation is replaced by the
+// voxel's own (TVSQS.cu:64-80,223-257); s2 = dims / var + sum 1 / var_next.
+// ---------------------------------------------------------------------------
+template <class R>
+void tv_iso_sqs(R* s1, R* s2, R* var, const float* img, int nb, int nx, int ny, int nz, int nc, int dims, R eps)
+{
+    if (dims == 2) nz = 1;
+    const long sx = (long)ny * nz * nc, sy = (long)nz * nc, sz = nc;
+    const long n = (long)nb * nx * ny * nz * nc;
+    auto at = [&](long i) { return (R)img[i]; };
+    for (long i = 0; i < n; i++) {
+        const long r = i % ((long)nx * ny * nz * nc);
+        const int iz = (int)((r / nc) % nz), iy = (int)((r / sy) % ny), ix = (int)(r / sx);
+        const R x = at(i);
+        const R dx = ix > 0 ? x - at(i - sx) : R(0);
+        const R dy = iy > 0 ? x - at(i - sy) : R(0);
+        const R dz = (dims == 3 && iz > 0) ? x - at(i - sz) : R(0);
+        var[i] = std::sqrt(dx * dx + dy * dy + dz * dz + eps);
+    }
+    for (long i = 0; i < n; i++) {
+        const long r = i % ((long)nx * ny * nz * nc);
+        const int iz = (int)((r / nc) % nz), iy = (int)((r / sy) % ny), ix = (int)(r / sx);
+        const R x = at(i), v = var[i];
+        R d0 = 0, g1 = 0, c1 = 0;
+        auto axis = [&](int idx, int nn, long st) {
+            if (idx > 0) d0 += x - at(i - st);
+            const bool up = idx < nn - 1;
+            const R va = up ? var[i + st] : v;
+            if (up) g1 += (x - at(i + st)) / va;
+            c1 += R(1) / va;
+        };
+        axis(ix, nx, sx);
+        axis(iy, ny, sy);
+        if (dims == 3) axis(iz, nz, sz);
+        s1[i] = d0 / v + g1;
+        s2[i] = R(dims) / v + c1;
+    }
+}
+
 // ===========================================================================
 #define FAN_ARGS int nB, int nC, int nx, int ny, int nz, float dx, float dy, float dz, \
     int nu, int nview, int nv, float da, float dv, float off_a, float off_v, float dsd, float dso, int type
@@ -1196,6 +1239,10 @@ void orc_dd_cone_fp_f64(double* prj, const float* img, const float* C, const flo
 void orc_dd_cone_adj_f64(double* img, const float* prj, const float* C, const float* U, const float* V,
                          const float* src, ARB_ARGS)
 { dd_cone_adj<double>(img, prj, C, U, V, src, ARB_MAKE); }
+
+void orc_tv_iso_sqs_f64(double* s1, double* s2, double* var, const float* img, int nb, int nx, int ny, int nz, int nc,
+                        int dims, double eps)
+{ tv_iso_sqs<double>(s1, s2, var, img, nb, nx, ny, nz, nc, dims, eps); }
 
 void orc_helical_bp_f64(double* img, const float* prj, int nB, int nC, int nx, int ny, int nz, float dx, float dy, float dz,
                         int nu, int nview, int nv, float du, float dv, float off_u, float off_v, float dsd, float dso,

@@ -12,6 +12,7 @@
 #include "rn_3d.h"
 #include "rn_fan_ops.h"
 #include "rn_filter.h"
+#include "rn_prior.h"
 #include "rn_sqs.h"
 #include "rn_tune.h"
 
@@ -46,30 +47,6 @@ ThreadCtx& ctx()
         c.stream.reset(new CallStream());
     }
     return c;
-}
-
-template <class F>
-void guarded(const char* name, F&& f)
-{
-    try {
-        clear_last_error();
-        f();
-    } catch (const std::exception& e) {
-        std::string m = std::string(name) + " failed: " + e.what();
-        std::fprintf(stderr, "%s\n", m.c_str());
-        set_last_error(m);
-    } catch (...) {
-        std::string m = std::string(name) + " failed: unknown error";
-        std::fprintf(stderr, "%s\n", m.c_str());
-        set_last_error(m);
-    }
-}
-
-template <class F>
-int guarded_status(const char* name, F&& f)
-{
-    guarded(name, std::forward<F>(f));
-    return last_error()[0] == '\0' ? 0 : -1;
 }
 
 void h2d(DevBuf& buf, const float* host, size_t n, hipStream_t st)
@@ -721,6 +698,14 @@ extern "C" int rn_sqs_kernel_times(void* sqs, double* ms_fp, double* ms_bp, doub
 {
     void* stream = nullptr;
     SQS_CALL("rn_sqs_kernel_times", { (void)st; e.kernel_times(ms_fp, ms_bp, ms_update, steps); });
+}
+
+extern "C" int rn_tv_iso_sqs(const float* img, float* s1, float* s2, float* var, int nb, int nx, int ny, int nz,
+                             int nc, int dims, float eps, void* stream)
+{
+    return guarded_status("rn_tv_iso_sqs", [&] {
+        tv_iso_sqs(img, s1, s2, var, nb, nx, ny, nz, nc, dims, eps, static_cast<hipStream_t>(stream));
+    });
 }
 
 extern "C" int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,

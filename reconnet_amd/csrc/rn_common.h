@@ -1,6 +1,9 @@
 // Shared definitions for the MI355X (gfx950) ReconNet projector library.
 #pragma once
 
+#include <cstdio>
+#include <utility>
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -83,6 +86,32 @@ public:
 private:
     hipStream_t s_ = nullptr;
 };
+
+// Run f for the C-ABI: exceptions never cross extern "C"; a failure prints
+// "<name> failed: <reason>" on stderr and becomes the thread's last error.
+template <class F>
+void guarded(const char* name, F&& f)
+{
+    try {
+        clear_last_error();
+        f();
+    } catch (const std::exception& e) {
+        std::string m = std::string(name) + " failed: " + e.what();
+        std::fprintf(stderr, "%s\n", m.c_str());
+        set_last_error(m);
+    } catch (...) {
+        std::string m = std::string(name) + " failed: unknown error";
+        std::fprintf(stderr, "%s\n", m.c_str());
+        set_last_error(m);
+    }
+}
+
+template <class F>
+int guarded_status(const char* name, F&& f)
+{
+    guarded(name, std::forward<F>(f));
+    return last_error()[0] == '\0' ? 0 : -1;
+}
 
 inline void check_launch(const char* what) {
     hipError_t e = hipGetLastError();

@@ -21,16 +21,16 @@ REFERENCE_EXPORTS = [
 ]
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?(?:void|int|float|double|char)\s*\**\s*\*?\s*([A-Za-z_]\w*)\s*\(",
                        text, flags=re.M)
     return sorted(set(names))
 
 
-def exported_symbols():
-    out = subprocess.check_output(["nm", "-D", "--defined-only", LIB_PATH]).decode()
+def exported_symbols(path=LIB_PATH):
+    out = subprocess.check_output(["nm", "-D", "--defined-only", path]).decode()
     return {line.split()[-1] for line in out.splitlines() if " T " in line}
 
 
@@ -58,3 +58,21 @@ def test_library_loads_with_ctypes_by_file_name():
         assert hasattr(lib, name)
     lib.rn_last_error.restype = ctypes.c_char_p
     assert lib.rn_last_error() == b""
+
+
+PFS_HEADER = os.path.join(ROOT, "include", "priorfunctionsolver.h")
+PFS_PATH = os.path.join(os.path.dirname(LIB_PATH), "libPriorFunctionSolver.so")
+
+
+def test_prior_function_solver_library_exports():
+    # the TV-SQS entry points of the reference's libPriorFunctionSolver.so
+    # (PriorFunctionSolver/TVSQS.cu:108,290) and its device selector
+    assert os.path.exists(PFS_PATH), "run `make -C reconnet_amd`"
+    decl = declared_functions(PFS_HEADER)
+    for name in ("cSetDevice", "cTVSQS2D", "cTVSQS3D"):
+        assert name in decl
+    syms = exported_symbols(PFS_PATH)
+    assert not [n for n in decl if n not in syms]
+    lib = ctypes.cdll.LoadLibrary(PFS_PATH)
+    lib.pfs_last_error.restype = ctypes.c_char_p
+    assert lib.pfs_last_error() == b""

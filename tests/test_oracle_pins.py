@@ -511,3 +511,44 @@ def test_dd_cone_uniform_volume_central_ray_is_chord_length():
         main = max(abs(d[0]), abs(d[1]))
         # the central ray crosses all n planes of the main axis inside the volume
         assert dd[0, nu // 2, view, nv // 2, 0] == pytest.approx(n * np.linalg.norm(d) / main, rel=1e-12)
+
+
+# ------------------------------------------------------- isotropic TV-SQS --
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_tv_iso_s1_is_gradient_of_total_variation(dims):
+    # s1 = d/dx sum(var) exactly (boundary terms included): central differences in FP64
+    rng = np.random.default_rng(11 + dims)
+    shape = (2, 5, 4, 2) if dims == 2 else (1, 4, 3, 5, 2)
+    x = rng.uniform(0, 1, shape).astype(np.float32)
+    eps = 0.1          # curvature scale sqrt(eps) >> h keeps the difference quotient accurate
+    s1, s2, var = orc.tv_iso_sqs(x, dims, eps)
+    h = 1e-3
+
+    def tv(z):
+        return orc.tv_iso_sqs(z, dims, eps)[2].sum()
+
+    # the oracle reads float32 images: perturb by an exactly representable step
+    flat = x.reshape(-1)
+    for i in rng.choice(flat.size, size=12, replace=False):
+        zp = flat.copy(); zp[i] += np.float32(h)
+        zm = flat.copy(); zm[i] -= np.float32(h)
+        dp = float(zp[i]) - float(flat[i]); dm = float(flat[i]) - float(zm[i])
+        fd = (tv(zp.reshape(shape)) - tv(zm.reshape(shape))) / (dp + dm)
+        assert fd == pytest.approx(s1.reshape(-1)[i], rel=2e-4, abs=1e-6)
+
+
+def test_tv_iso_2x2_closed_form():
+    a, b, c, d = 0.0, 1.0, 3.0, 7.0          # img[ix][iy]: [[a, b], [c, d]]
+    x = np.array([[a, b], [c, d]], np.float32)[None, :, :, None]
+    eps = 0.5
+    s1, s2, var = orc.tv_iso_sqs(x, 2, eps)
+    v = lambda dx, dy: np.sqrt(dx * dx + dy * dy + eps)
+    V = np.array([[v(0, 0), v(0, b - a)], [v(c - a, 0), v(d - b, d - c)]])
+    assert np.allclose(var[0, :, :, 0], V)
+    # pixel (0,0): no backward terms; forward neighbours (1,0) and (0,1)
+    assert s1[0, 0, 0, 0] == pytest.approx((a - c) / V[1, 0] + (a - b) / V[0, 1])
+    assert s2[0, 0, 0, 0] == pytest.approx(2 / V[0, 0] + 1 / V[1, 0] + 1 / V[0, 1])
+    # pixel (1,1): backward terms only, both forward neighbours replaced by itself
+    assert s1[0, 1, 1, 0] == pytest.approx(((d - b) + (d - c)) / V[1, 1])
+    assert s2[0, 1, 1, 0] == pytest.approx(4 / V[1, 1])
