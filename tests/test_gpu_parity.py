@@ -344,3 +344,27 @@ def test_helical_bp_parity():
     g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.0, dv=1.0, off_u=0.25, off_v=-0.5, dsd=1000.0, dso=600.0)
     ref = orc.helical_bp(prj, g, (32, 32, 12), npp, 0.1, 8.0, 12.0)
     assert rel_l2(out, ref) <= TOL
+
+
+def test_helical_bp_full_size_slices_match_oracle():
+    # 256^2 x 128 volume at 1 mm, 64 rows x 512 columns (1.0 x 1.2 mm), SID 595 mm,
+    # SDD 1085.6 mm, 720 views per turn, 3 turns, pitch 1 (table feed per turn =
+    # 64 * 1.0 * 595 / 1085.6 mm).  The GPU reconstructs the whole volume; the
+    # FP64 oracle recomputes two of its slices (z = volZ0 + iz dz is exact here,
+    # so the sub-volume with volZ0 + iz0 dz has identical voxel positions).
+    from reconnet_amd.recon import ReconNet
+
+    npp, turns, nu, nv = 360, 3, 512, 64
+    zrot = np.float32(64 * 1.0 * 595.0 / 1085.6)
+    r = ReconNet()
+    r.nx, r.ny, r.nz, r.dx, r.dy, r.dz = 256, 256, 128, 1.0, 1.0, 1.0
+    r.du, r.dv, r.off_u, r.off_v, r.dsd, r.dso = 1.2, 1.0, 0.0, 0.0, 1085.6, 595.0
+    rng = np.random.default_rng(1234)
+    prj = rng.uniform(0, 1, (1, nu, 2 * npp * turns, nv, 1)).astype(np.float32)
+    vol_z0 = -11.0
+    out = r.cHelicalBP4d(prj, npp, 0.0, vol_z0, float(zrot))
+    assert np.isfinite(out).all()
+    g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.2, dv=1.0, off_u=0.0, off_v=0.0, dsd=1085.6, dso=595.0)
+    for iz0 in (30, 77):
+        ref = orc.helical_bp(prj, g, (256, 256, 2), npp, 0.0, vol_z0 + iz0, float(zrot))
+        assert rel_l2(out[:, :, :, iz0:iz0 + 2], ref) <= TOL
