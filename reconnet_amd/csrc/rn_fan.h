@@ -49,6 +49,7 @@ struct FanK {
     // dx / |sin(a - t)|).  Computed in FP64.
     const float4* sray;
     const int* vwide;    // [nview] 1: every Siddon line interval of the view has a positive FP32 width
+    int unit_angle;      // 1: |rx / (ry + dso)| < 1 for every pixel centre and view (image well inside the circle)
 };
 
 // Device tables for one (geometry, angle set).  Angles are kept on the host

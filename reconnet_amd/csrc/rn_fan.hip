@@ -359,6 +359,9 @@ void FanTables::build(const FanGeo& g, const float* deg_host, hipStream_t st)
                                 hipMemcpyHostToDevice, st));
     RN_HIP_CHECK(hipStreamSynchronize(st));
     k.sray = reinterpret_cast<const float4*>(ray_.get());
+    // pixel centres lie within R of the centre: |rx| / (dso - |ry|) <= R / (dso - R) < 1 if R < 0.45 dso
+    const double R = 0.5 * std::sqrt((double)g.nx * g.dx * g.nx * g.dx + (double)g.ny * g.dy * g.ny * g.dy);
+    k.unit_angle = R < 0.45 * (double)g.dso ? 1 : 0;
     k.vwide = reinterpret_cast<const int*>(static_cast<const char*>(ray_.get()) + ray_bytes);
 }
 

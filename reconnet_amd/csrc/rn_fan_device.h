@@ -239,16 +239,17 @@ __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ 
     return acc * (T)r.z;
 }
 
-// atan on all of R from an odd polynomial on [-1, 1] (coefficients fitted
-// here by iteratively reweighted least squares on Chebyshev nodes; maximum
-// error 2.5e-7 rad) and atan(z) = sign(z) pi/2 - atan(1/z) outside.  Used only
-// to locate the bins a pixel can meet; ATAN_FAST_ERR bounds its error with
-// FP32 evaluation included.
+// atan from an odd polynomial on [-1, 1] (coefficients fitted here by
+// iteratively reweighted least squares on Chebyshev nodes; maximum error
+// 2.5e-7 rad) and, unless UNIT (|z| <= 1 known), atan(z) = sign(z) pi/2 -
+// atan(1/z) outside.  Used only to locate the bins a pixel can meet;
+// RN_ATAN_FAST_ERR bounds its error with FP32 evaluation included.
 #define RN_ATAN_FAST_ERR 1.0e-6f
+template <bool UNIT>
 __device__ __forceinline__ float atan_fast(float z)
 {
     const float az = fabsf(z);
-    const bool inv = az > 1.0f;
+    const bool inv = !UNIT && az > 1.0f;
     const float t = inv ? __builtin_amdgcn_rcpf(az) : az;
     const float s = t * t;
     float p = 0.006811782025349205f;
@@ -265,17 +266,19 @@ __device__ __forceinline__ float atan_fast(float z)
 
 // First and last bin whose centre ray can meet pixel (ix, iy) of view `view`:
 // the pixel lies in the disc of radius rpix around its centre, whose rays span
-// the detector angles atan(rx / d0) +- asin(rpix / dist), asin(r/d) <= r / (d - r).
+// the detector angles atan(rx / d0) +- asin(s), s = rpix / dist, and
+// asin(s) <= s (1 + 2 s) for 0 <= s <= 1.  UNIT: |rx / d0| <= 1 for every pixel
+// and view of the geometry (FanK::unit_angle).
+template <bool UNIT>
 __device__ __forceinline__ void bin_range(const FanK& g, float xc, float yc, float rpix, float inv_da, int view,
                                           int& k0, int& k1)
 {
     const float ct = g.vcos[view], st = g.vsin[view];
     const float rx = xc * ct + yc * st, ry = -xc * st + yc * ct;
     const float d0 = ry + g.dso;
-    const float uc = g.cu - g.off_a - atan_fast(rx * __builtin_amdgcn_rcpf(d0)) * inv_da;
-    const float dist = __builtin_amdgcn_sqrtf(fmaf(rx, rx, d0 * d0));
-    const float hw = rpix * inv_da * __builtin_amdgcn_rcpf(fmaxf(dist - rpix, 1e-3f)) + 0.02f +
-                     RN_ATAN_FAST_ERR * fabsf(inv_da);
+    const float uc = g.cu - g.off_a - atan_fast<UNIT>(rx * __builtin_amdgcn_rcpf(d0)) * inv_da;
+    const float sr = fminf(rpix * __builtin_amdgcn_rsqf(fmaf(rx, rx, d0 * d0)), 1.0f);
+    const float hw = sr * fmaf(2.0f, sr, 1.0f) * fabsf(inv_da) + 0.02f + RN_ATAN_FAST_ERR * fabsf(inv_da);
     k0 = max((int)ceilf(uc - hw), 0);
     k1 = min((int)floorf(uc + hw), g.nu - 1);
 }
@@ -319,9 +322,9 @@ __device__ __forceinline__ T siddon_gather_bins(const float4* __restrict__ R, co
 // view0 .. view0+nviews-1 of prj ([nviews][nu]), the rays that cross it, with
 // the weights siddon_fp_ray gives it.  The first NB bins of a pixel's range
 // are taken without branches, the rest in a loop (NB changes only the speed).
-template <class T, int NB = 4>
-__device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict__ prj, int view0, int nviews,
-                                             int ix, int iy)
+template <class T, int NB, bool UNIT>
+__device__ __forceinline__ T siddon_bp_views(const FanK& g, const T* __restrict__ prj, int view0, int nviews, int ix,
+                                             int iy)
 {
     const float xc = (ix - 0.5f * (g.nx - 1)) * g.dx, yc = (iy - 0.5f * (g.ny - 1)) * g.dy;
     const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
@@ -332,7 +335,7 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
     for (int vi = 0; vi < nviews; ++vi) {
         const int view = view0 + vi;
         int k0, k1;
-        bin_range(g, xc, yc, rpix, inv_da, view, k0, k1);
+        bin_range<UNIT>(g, xc, yc, rpix, inv_da, view, k0, k1);
         const bool xv = g.vcls[view] == 0;
         const float lr = xv ? yr : xr;          // this pixel's line
         const float cell = xv ? (float)ix : (float)iy;
@@ -343,6 +346,14 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
         else acc += siddon_gather_bins<T, NB, false>(R, p, g.nu, k0, k1, lr, cell);
     }
     return acc;
+}
+
+template <class T, int NB = 4>
+__device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict__ prj, int view0, int nviews,
+                                             int ix, int iy)
+{
+    return g.unit_angle ? siddon_bp_views<T, NB, true>(g, prj, view0, nviews, ix, iy)
+                        : siddon_bp_views<T, NB, false>(g, prj, view0, nviews, ix, iy);
 }
 
 // Exact transpose of dd_fp_ray (no reference counterpart: the reference DD BP

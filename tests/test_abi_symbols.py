@@ -57,6 +57,8 @@ def test_library_loads_with_ctypes_by_file_name():
     for name in REFERENCE_EXPORTS:
         assert hasattr(lib, name)
     lib.rn_last_error.restype = ctypes.c_char_p
+    # a call that needs no device succeeds and clears the thread's last error
+    assert lib.rn_autotune_reset() == 0
     assert lib.rn_last_error() == b""
 
 
