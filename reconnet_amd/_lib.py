@@ -61,6 +61,7 @@ def lib():
     _sig(h, "rn_autotune_enable", c_int, [c_int])
     _sig(h, "rn_autotune_report", c_int, [ctypes.c_char_p, c_int])
     _sig(h, "rn_autotune_reset", c_int, [])
+    _sig(h, "rn_autotune_pin", c_int, [c_int, c_int])
     h.cTVSQS2D.restype = c_float
     _lib = h
     return _lib
@@ -75,6 +76,18 @@ def check(status, what):
     """Raise RuntimeError if a status-returning entry point failed."""
     if status != 0:
         raise RuntimeError("%s failed: %s" % (what, last_error()))
+
+
+# kernel ids of the tuned launches (TuneKernel in csrc/rn_tune.h)
+TUNED_KERNELS = {"siddon_fp": 1, "siddon_bp": 2, "dd_fp": 3, "dd_adjoint": 4, "dd_bp": 5, "sqs_update": 6,
+                 "dd_fbp": 7}
+
+
+def autotune_pin(kernel, candidate):
+    """Force launch candidate `candidate` of a tuned kernel (a TUNED_KERNELS
+    name or id) for every shape; None or a negative value removes the pin."""
+    kid = TUNED_KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    check(lib().rn_autotune_pin(kid, -1 if candidate is None else int(candidate)), "rn_autotune_pin")
 
 
 def autotune(enable=None, reset=False):

@@ -510,6 +510,11 @@ extern "C" int rn_autotune_reset(void)
     return guarded_status("rn_autotune_reset", [&] { tune_reset(); });
 }
 
+extern "C" int rn_autotune_pin(int kernel, int candidate)
+{
+    return guarded_status("rn_autotune_pin", [&] { tune_pin(kernel, candidate); });
+}
+
 extern "C" void* rn_fan_plan_create(int nx, int ny, int nz, float dx, float dy, float dz,
         int nu, int nview, int nv, float da, float dv, float off_a, float off_v,
         float dsd, float dso, const float* deg)

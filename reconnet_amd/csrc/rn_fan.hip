@@ -43,59 +43,81 @@ __global__ __launch_bounds__(BLK) void k_dd_fan_fp(FanK g, const float* __restri
 
 // With meas != nullptr the epilogue writes the weighted residual
 // wgt * (Ax - meas) of the OS-SQS data step (ossqs.cu:101-122) instead of Ax.
-template <class T, int BLK, int XCD>
+// Each lane projects its ray through S consecutive slices (the ray's lines,
+// cells and weights are shared; the grid's slice index counts groups of S).
+template <class T, int BLK, int XCD, int S>
 __global__ __launch_bounds__(BLK) void k_siddon_fan_fp(FanK g, const T* __restrict__ Pyx,
                                                        const T* __restrict__ Pxy, T* __restrict__ prj,
-                                                       int view0, int nviews,
+                                                       int view0, int nviews, int ns,
                                                        const float* __restrict__ meas, const float* __restrict__ wgt,
                                                        int meas_nviews)
 {
-    // XCD > 0: 1-D grid of tiles * nviews * nslices workgroups (a multiple of
-    // XCD).  The hardware deals workgroups round-robin to the XCDs, so
+    // XCD > 0: 1-D grid of tiles * nviews * slice groups workgroups (a multiple
+    // of XCD).  The hardware deals workgroups round-robin to the XCDs, so
     // workgroup h runs on XCD h % XCD; h is renumbered so that each XCD walks a
-    // contiguous, slice-major range of (slice, view, bin tile) and its L2 holds
-    // the images of only the slice or two it is working on.
-    int tile, vi, s;
+    // contiguous, slice-major range of (slice group, view, bin tile) and its L2
+    // holds the images of only the slices it is working on.
+    int tile, vi, sg;
     if (XCD > 0) {
         const int per = gridDim.x / XCD;
         const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
         const int tiles = (g.nu + BLK - 1) / BLK;
         tile = l % tiles;
         vi = (l / tiles) % nviews;
-        s = l / (tiles * nviews);
+        sg = l / (tiles * nviews);
     } else {
         tile = blockIdx.x;
         vi = blockIdx.y;
-        s = blockIdx.z;
+        sg = blockIdx.z;
     }
     const int iu = tile * BLK + threadIdx.x;
     const bool active = iu < g.nu;
     const int view = view0 + vi;
     const long plane = (long)g.nx * g.ny;
-    T acc = siddon_fp_ray<T>(g, Pyx + s * plane, Pxy + s * plane, view, active ? iu : g.nu - 1, active);
+    const int s0 = sg * S;
+    const int nsl = min(S, ns - s0);
+    T acc[S];
+    siddon_fp_ray_slices<T, S>(g, Pyx + s0 * plane, Pxy + s0 * plane, plane, nsl, view, active ? iu : g.nu - 1,
+                               active, acc);
     if (!active) return;
-    if (meas != nullptr) {
-        const long mi = ((long)s * meas_nviews + view) * g.nu + iu;
-        acc = (T)wgt[mi] * (acc - (T)meas[mi]);
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        if (q >= nsl) break;
+        const int s = s0 + q;
+        T v = acc[q];
+        if (meas != nullptr) {
+            const long mi = ((long)s * meas_nviews + view) * g.nu + iu;
+            v = (T)wgt[mi] * (v - (T)meas[mi]);
+        }
+        prj[((long)s * nviews + vi) * g.nu + iu] = v;
     }
-    prj[((long)s * nviews + vi) * g.nu + iu] = acc;
 }
 
 // ---------------------------------------------------------------------------
 // backprojectors (pixel driven gathers)
 // ---------------------------------------------------------------------------
 
-template <class T, int BX, int BY, int NB>
+// One lane per pixel of S consecutive slices (the grid's z index counts
+// groups of S): the bin ranges and weights are shared by the slices.
+template <class T, int BX, int BY, int NB, int S>
 __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp(FanK g, const T* __restrict__ prj,
-                                                       T* __restrict__ img, int view0, int nviews, int accumulate)
+                                                       T* __restrict__ img, int view0, int nviews, int ns,
+                                                       int accumulate)
 {
     const int iy = blockIdx.x * BX + threadIdx.x;
     const int ix = blockIdx.y * BY + threadIdx.y;
-    const int s = blockIdx.z;
+    const int s0 = blockIdx.z * S;
+    const int nsl = min(S, ns - s0);
     if (ix >= g.nx || iy >= g.ny) return;
-    const T acc = siddon_bp_pixel<T, NB>(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
-    T* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
-    *o = accumulate ? *o + acc : acc;
+    const long sstride = (long)nviews * g.nu;
+    T acc[S];
+    siddon_bp_pixel_slices<T, NB, S>(g, prj + s0 * sstride, sstride, nsl, view0, nviews, ix, iy, acc);
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        if (q >= nsl) break;
+        T* o = img + (long)(s0 + q) * g.nx * g.ny + (long)ix * g.ny + iy;
+        *o = accumulate ? *o + acc[q] : acc[q];
+    }
 }
 
 template <int BX, int BY>
@@ -127,19 +149,29 @@ __global__ void k_dd_preweight(FanK g, const float* __restrict__ prj, float* __r
 }
 
 // Reference-parity DD BP (DDBPFanKernelX/Y): pixel edges projected through
-// atan, average of the (pre-weighted) detector row over the footprint.
-template <int BX, int BY>
+// atan, average of the (pre-weighted) detector row over the footprint.  One
+// lane per pixel of S consecutive slices (grid z counts groups of S): the
+// footprint and the FBP weight are computed once for the S slices.
+template <int BX, int BY, int S>
 __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp(FanK g, const float* __restrict__ wprj,
-                                                   float* __restrict__ img, int fbp)
+                                                   float* __restrict__ img, int ns, int fbp)
 {
     const int iy = blockIdx.x * BX + threadIdx.x;
     const int ix = blockIdx.y * BY + threadIdx.y;
-    const int s = blockIdx.z;
+    const int s0 = blockIdx.z * S;
+    const int nsl = min(S, ns - s0);
     if (ix >= g.nx || iy >= g.ny) return;
     const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
     const float inv_da = 1.0f / g.da;
     const float boff = 0.5f * g.nu - g.off_a;
-    float acc = 0.0f;
+    const long sstride = (long)g.nview * g.nu;
+    const float* rows[S];   // wave-uniform slice bases
+    float acc[S], val[S];
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        rows[q] = wprj + (long)(s0 + min(q, nsl - 1)) * sstride;
+        acc[q] = 0.0f;
+    }
     for (int view = 0; view < g.nview; ++view) {
         const float ct = g.vcos[view], st = g.vsin[view];
         const float rx = x * ct + y * st, ry = -x * st + y * ct;
@@ -148,14 +180,20 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp(FanK g, const float* __re
         else                   { ex = 0.5f * g.dy * st; ey = 0.5f * g.dy * ct; }
         const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
         const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
-        float val = seg_avg(wprj + ((long)s * g.nview + view) * g.nu, g.nu, fminf(b1, b2), fmaxf(b1, b2));
+        seg_avg_rows<float, S>(rows, (unsigned)(view * g.nu) * 4u, g.nu, fminf(b1, b2), fmaxf(b1, b2), val);
+        float fw = 1.0f;
         if (fbp) {
             const float d = g.dso + ry;
-            val *= g.dso * g.dso / (rx * rx + d * d);
+            fw = g.dso * g.dso / (rx * rx + d * d);
         }
-        acc += val;
+        #pragma unroll
+        for (int q = 0; q < S; ++q) acc[q] += fbp ? val[q] * fw : val[q];
     }
-    img[(long)s * g.nx * g.ny + (long)ix * g.ny + iy] = acc;
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        if (q >= nsl) break;
+        img[(long)(s0 + q) * g.nx * g.ny + (long)ix * g.ny + iy] = acc[q];
+    }
 }
 
 // bpFanKernel3D with isFBP = true (the only mode the reference launches).
@@ -387,26 +425,27 @@ void go_dd_fp(const FanK& g, const float* RX, const float* RY, float* prj, int n
 // MI355X: 8 XCDs, each with its own L2
 constexpr int kXcds = 8;
 
-template <int BLK, int XCD>
+template <int BLK, int XCD, int S>
 void go_siddon_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj, int ns, int view0, int nviews,
                   const float* meas, const float* w, hipStream_t st)
 {
-    const long nblk = (long)cdiv(g.nu, BLK) * nviews * ns;
+    const int ngroups = cdiv(ns, S);
+    const long nblk = (long)cdiv(g.nu, BLK) * nviews * ngroups;
     if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
-        hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK, XCD>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Pyx, Pxy,
-                           prj, view0, nviews, meas, w, g.nview);
+        hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK, XCD, S>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Pyx,
+                           Pxy, prj, view0, nviews, ns, meas, w, g.nview);
     } else {
-        hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK, 0>), dim3(cdiv(g.nu, BLK), nviews, ns), dim3(BLK), 0, st, g,
-                           Pyx, Pxy, prj, view0, nviews, meas, w, g.nview);
+        hipLaunchKernelGGL((k_siddon_fan_fp<float, BLK, 0, S>), dim3(cdiv(g.nu, BLK), nviews, ngroups), dim3(BLK), 0,
+                           st, g, Pyx, Pxy, prj, view0, nviews, ns, meas, w, g.nview);
     }
 }
 
-template <int BX, int BY, int NB>
+template <int BX, int BY, int NB, int S>
 void go_siddon_bp(const FanK& g, const float* prj, float* img, int ns, int view0, int nviews, int acc,
                   hipStream_t st)
 {
-    hipLaunchKernelGGL((k_siddon_fan_bp<float, BX, BY, NB>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0,
-                       st, g, prj, img, view0, nviews, acc);
+    hipLaunchKernelGGL((k_siddon_fan_bp<float, BX, BY, NB, S>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(ns, S)),
+                       dim3(BX, BY), 0, st, g, prj, img, view0, nviews, ns, acc);
 }
 
 template <int BX, int BY>
@@ -416,11 +455,11 @@ void go_dd_adj(const FanK& g, const float* prj, float* img, int ns, int view0, i
                        prj, img, view0, nviews, acc);
 }
 
-template <int BX, int BY>
+template <int BX, int BY, int S>
 void go_dd_bp(const FanK& g, const float* wprj, float* img, int ns, int fbp, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dd_fan_bp<BX, BY>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0, st, g,
-                       wprj, img, fbp);
+    hipLaunchKernelGGL((k_dd_fan_bp<BX, BY, S>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(ns, S)), dim3(BX, BY), 0,
+                       st, g, wprj, img, ns, fbp);
 }
 
 using DdFpFn = void (*)(const FanK&, const float*, const float*, float*, int, int, int, hipStream_t);
@@ -431,12 +470,34 @@ using DdBpFn = void (*)(const FanK&, const float*, float*, int, int, hipStream_t
 
 // 1-D blocks along the detector; 2-D pixel tiles (x lanes along y, the contiguous image axis)
 const DdFpFn kDdFp[] = {go_dd_fp<128>, go_dd_fp<64>, go_dd_fp<256>};
-const SidFpFn kSidFp[] = {go_siddon_fp<128, 0>, go_siddon_fp<64, 0>, go_siddon_fp<256, 0>,
-                          go_siddon_fp<128, kXcds>, go_siddon_fp<64, kXcds>, go_siddon_fp<256, kXcds>};
-const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 3>, go_siddon_bp<8, 8, 3>, go_siddon_bp<16, 16, 2>,
-                           go_siddon_bp<8, 8, 2>,   go_siddon_bp<16, 16, 4>, go_siddon_bp<8, 8, 4>};
+// S: slices per lane (1 = one slice per lane; larger S shares the geometry work)
+const SidFpFn kSidFp[] = {go_siddon_fp<128, 0, 1>,     go_siddon_fp<64, 0, 1>,     go_siddon_fp<256, 0, 1>,
+                          go_siddon_fp<128, kXcds, 1>, go_siddon_fp<64, kXcds, 1>, go_siddon_fp<256, kXcds, 1>,
+                          go_siddon_fp<128, 0, 2>,     go_siddon_fp<128, kXcds, 2>, go_siddon_fp<64, 0, 4>,
+                          go_siddon_fp<128, 0, 4>,     go_siddon_fp<64, kXcds, 4>, go_siddon_fp<128, kXcds, 4>,
+                          go_siddon_fp<256, kXcds, 4>, go_siddon_fp<128, kXcds, 8>};
+const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 3, 1>, go_siddon_bp<8, 8, 3, 1>,  go_siddon_bp<16, 16, 2, 1>,
+                           go_siddon_bp<8, 8, 2, 1>,   go_siddon_bp<16, 16, 4, 1>, go_siddon_bp<8, 8, 4, 1>,
+                           go_siddon_bp<8, 8, 2, 2>,   go_siddon_bp<8, 8, 2, 4>,   go_siddon_bp<16, 16, 2, 4>,
+                           go_siddon_bp<8, 8, 3, 4>,   go_siddon_bp<8, 8, 2, 8>,   go_siddon_bp<16, 16, 2, 8>,
+                           go_siddon_bp<8, 8, 3, 8>,   go_siddon_bp<8, 8, 2, 16>,  go_siddon_bp<16, 16, 2, 16>};
+constexpr int kNSidFp = sizeof(kSidFp) / sizeof(kSidFp[0]);
+constexpr int kNSidBp = sizeof(kSidBp) / sizeof(kSidBp[0]);
 const GatherFn kDdAdj[] = {go_dd_adj<16, 16>, go_dd_adj<8, 32>, go_dd_adj<32, 8>, go_dd_adj<8, 8>};
-const DdBpFn kDdBp[] = {go_dd_bp<16, 16>, go_dd_bp<8, 32>, go_dd_bp<32, 8>, go_dd_bp<8, 8>};
+const DdBpFn kDdBp[] = {go_dd_bp<16, 16, 1>, go_dd_bp<8, 32, 1>, go_dd_bp<32, 8, 1>, go_dd_bp<8, 8, 1>,
+                       go_dd_bp<8, 8, 4>,   go_dd_bp<16, 16, 4>, go_dd_bp<8, 8, 8>, go_dd_bp<16, 16, 8>,
+                       go_dd_bp<8, 8, 16>};
+constexpr int kNDdBp = sizeof(kDdBp) / sizeof(kDdBp[0]);
+
+// The Siddon and DD gather kernels address one slice's image plane, ray table
+// and sinogram rows with 32-bit byte offsets.
+void require_offsets32(const FanK& g, int nviews, size_t elem)
+{
+    const long lim = 0x7fffffffL;
+    RN_REQUIRE((long)g.nx * g.ny * (long)elem < lim && (long)g.nview * g.nu * 16L < lim &&
+                   (long)nviews * g.nu * (long)elem < lim,
+               "image plane, ray table or sinogram of one slice exceeds 2 GiB");
+}
 
 } // namespace
 
@@ -454,7 +515,8 @@ void launch_siddon_fan_fp(const FanK& g, const float* Pyx, const float* Pxy, flo
                           int nslices, int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 6, [&](int i) {
+    require_offsets32(g, nviews, sizeof(float));
+    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, kNSidFp, [&](int i) {
         kSidFp[i](g, Pyx, Pxy, prj_vu, nslices, view0, nviews, nullptr, nullptr, st);
     }, st, true);
     kSidFp[c](g, Pyx, Pxy, prj_vu, nslices, view0, nviews, nullptr, nullptr, st);
@@ -466,7 +528,8 @@ void launch_siddon_fan_fp_residual(const FanK& g, const float* Pyx, const float*
                                    hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, 6, [&](int i) {
+    require_offsets32(g, nviews, sizeof(float));
+    const int c = tune_choose({TK_SIDDON_FP, g.nx, g.ny, g.nu, nviews, nslices}, kNSidFp, [&](int i) {
         kSidFp[i](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
     }, st, true);
     kSidFp[c](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
@@ -477,9 +540,10 @@ void launch_siddon_fan_fp_f64(const FanK& g, const double* Pyx, const double* Px
                               int nslices, int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
+    require_offsets32(g, nviews, sizeof(double));
     dim3 blk(128), grd(cdiv(g.nu, 128), nviews, nslices);
-    hipLaunchKernelGGL((k_siddon_fan_fp<double, 128, 0>), grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
-                       (const float*)nullptr, (const float*)nullptr, 0);
+    hipLaunchKernelGGL((k_siddon_fan_fp<double, 128, 0, 1>), grd, blk, 0, st, g, Pyx, Pxy, prj_vu, view0, nviews,
+                       nslices, (const float*)nullptr, (const float*)nullptr, 0);
     check_launch("siddon_fan_fp_f64");
 }
 
@@ -487,8 +551,10 @@ void launch_siddon_fan_bp_f64(const FanK& g, const double* prj_vu, double* img_x
                               int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0) return;
+    require_offsets32(g, nviews, sizeof(double));
     dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL((k_siddon_fan_bp<double, 16, 16, 4>), grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews, 0);
+    hipLaunchKernelGGL((k_siddon_fan_bp<double, 16, 16, 4, 1>), grd, blk, 0, st, g, prj_vu, img_xy, view0, nviews,
+                       nslices, 0);
     check_launch("siddon_fan_bp_f64");
 }
 
@@ -496,7 +562,8 @@ void launch_siddon_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int
                           int view0, int nviews, bool accumulate, hipStream_t st)
 {
     if (nslices <= 0) return;
-    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, 6, [&](int i) {
+    require_offsets32(g, nviews, sizeof(float));
+    const int c = tune_choose({TK_SIDDON_BP, g.nx, g.ny, g.nu, nviews, nslices}, kNSidBp, [&](int i) {
         kSidBp[i](g, prj_vu, img_xy, nslices, view0, nviews, 0, st);
     }, st, !accumulate);
     kSidBp[c](g, prj_vu, img_xy, nslices, view0, nviews, accumulate ? 1 : 0, st);
@@ -525,7 +592,8 @@ void launch_dd_preweight(const FanK& g, const float* prj_vu, float* wprj_vu, int
 void launch_dd_fan_bp(const FanK& g, const float* wprj_vu, float* img_xy, int nslices, bool fbp, hipStream_t st)
 {
     if (nslices <= 0) return;
-    const int c = tune_choose({fbp ? TK_DD_FBP : TK_DD_BP, g.nx, g.ny, g.nu, g.nview, nslices}, 4,
+    require_offsets32(g, g.nview, sizeof(float));
+    const int c = tune_choose({fbp ? TK_DD_FBP : TK_DD_BP, g.nx, g.ny, g.nu, g.nview, nslices}, kNDdBp,
                               [&](int i) { kDdBp[i](g, wprj_vu, img_xy, nslices, fbp ? 1 : 0, st); }, st, true);
     kDdBp[c](g, wprj_vu, img_xy, nslices, fbp ? 1 : 0, st);
     check_launch("dd_fan_bp");

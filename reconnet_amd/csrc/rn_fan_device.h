@@ -7,6 +7,16 @@
 namespace rn {
 namespace fan {
 
+// Element at byte offset `boff` (32-bit) from a wave-uniform base: written so
+// that the load is addressed as scalar base + 32-bit lane offset, with one
+// offset computation shared by loads from several bases.  Callers keep every
+// byte offset below 2^31 (checked on the host).
+template <class T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ base, unsigned boff)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+
 // ---------------------------------------------------------------------------
 // geometry helpers
 // ---------------------------------------------------------------------------
@@ -55,6 +65,37 @@ __device__ __forceinline__ T seg_avg(const T* __restrict__ line, int n, float lo
         s += line[m] * (T)(ph - pl);
     }
     return s / (T)w;
+}
+
+// seg_avg of S rows (row q `rboff` bytes after the wave-uniform base rows[q])
+// over one interval: the cells and overlaps are computed once; each row gets
+// the arithmetic of seg_avg.
+template <class T, int S>
+__device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned rboff, int n, float lo, float hi,
+                                             T (&val)[S])
+{
+    const float w = hi - lo;
+    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
+    #pragma unroll
+    for (int q = 0; q < S; ++q) val[q] = T(0);
+    if (!(h > l)) {
+        if (w > 0.0f || !(lo >= 0.0f && lo < (float)n)) return;
+        const unsigned boff = rboff + (unsigned)(int)lo * (unsigned)sizeof(T);
+        #pragma unroll
+        for (int q = 0; q < S; ++q) val[q] = ld_at(rows[q], boff);
+        return;
+    }
+    const int m0 = (int)l;
+    const int m1 = min((int)h, n - 1);
+    for (int m = m0; m <= m1; ++m) {
+        const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
+        const T wt = (T)(ph - pl);
+        const unsigned boff = rboff + (unsigned)m * (unsigned)sizeof(T);
+        #pragma unroll
+        for (int q = 0; q < S; ++q) val[q] += ld_at(rows[q], boff) * wt;
+    }
+    #pragma unroll
+    for (int q = 0; q < S; ++q) val[q] = val[q] / (T)w;
 }
 
 // Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).
@@ -152,26 +193,78 @@ __device__ __forceinline__ T siddon_line_sum(const T* __restrict__ line, int n, 
 // siddon_line_sum for hi - lo < 2 (at most three cells) without branches:
 // three independent loads (indices clamped into the line) and the weights
 // max(0, min(h, m + 1) - max(l, m)) of cells m0 .. m0 + 2, which vanish
-// outside [l, h]; a zero-width interval selects the cell holding lo.
-template <class T>
-__device__ __forceinline__ T siddon_line_sum3(const T* __restrict__ line, int n, float lo, float hi)
+// outside [l, h]; a zero-width interval selects the cell holding lo.  The
+// cells and weights (Line3) depend on the geometry only, so one Line3 serves
+// every slice of a batch.
+struct Line3 {
+    unsigned i0, i1, i2;
+    float w0, w1, w2, inv;
+};
+
+__device__ __forceinline__ Line3 line3(int n, float lo, float hi)
 {
     const float fn = (float)n;
     const float w = hi - lo;
     const float l = fminf(fmaxf(lo, 0.0f), fn), h = fminf(hi, fn);
     const int m0 = (int)l;                 // l in [m0, m0 + 1) unless l == n (then every weight is 0)
     const float f0 = (float)m0;
-    float w0 = fmaxf(fminf(h, f0 + 1.0f) - l, 0.0f);
-    const float w1 = fmaxf(fminf(h, f0 + 2.0f) - (f0 + 1.0f), 0.0f);
-    const float w2 = fmaxf(fminf(h, f0 + 3.0f) - (f0 + 2.0f), 0.0f);
+    Line3 c;
+    c.w0 = fmaxf(fminf(h, f0 + 1.0f) - l, 0.0f);
+    c.w1 = fmaxf(fminf(h, f0 + 2.0f) - (f0 + 1.0f), 0.0f);
+    c.w2 = fmaxf(fminf(h, f0 + 3.0f) - (f0 + 2.0f), 0.0f);
     const bool pos = w > 0.0f;
-    w0 = pos ? w0 : ((lo >= 0.0f && lo < fn) ? 1.0f : 0.0f);
-    const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
+    c.w0 = pos ? c.w0 : ((lo >= 0.0f && lo < fn) ? 1.0f : 0.0f);
+    c.inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
     const unsigned last = (unsigned)(n - 1);
-    const T v0 = line[min((unsigned)m0, last)];
-    const T v1 = line[min((unsigned)m0 + 1u, last)];
-    const T v2 = line[min((unsigned)m0 + 2u, last)];
-    return (v0 * (T)w0 + v1 * (T)w1 + v2 * (T)w2) * (T)inv;
+    c.i0 = min((unsigned)m0, last);
+    c.i1 = min((unsigned)m0 + 1u, last);
+    c.i2 = min((unsigned)m0 + 2u, last);
+    return c;
+}
+
+template <class T>
+__device__ __forceinline__ T line3_sum(const T* __restrict__ line, const Line3& c)
+{
+    return (line[c.i0] * (T)c.w0 + line[c.i1] * (T)c.w1 + line[c.i2] * (T)c.w2) * (T)c.inv;
+}
+
+// line3_sum of the line starting `lboff` bytes after a wave-uniform base,
+// over the first NC cells (NC = 2 when the interval is known to be narrower
+// than one cell: the third weight is then exactly 0 and its term is dropped)
+template <class T, int NC>
+__device__ __forceinline__ T line_sum_at(const T* __restrict__ base, unsigned lboff, const Line3& c)
+{
+    const T v0 = ld_at(base, lboff + c.i0 * (unsigned)sizeof(T));
+    const T v1 = ld_at(base, lboff + c.i1 * (unsigned)sizeof(T));
+    if (NC == 2) return (v0 * (T)c.w0 + v1 * (T)c.w1) * (T)c.inv;
+    const T v2 = ld_at(base, lboff + c.i2 * (unsigned)sizeof(T));
+    return (v0 * (T)c.w0 + v1 * (T)c.w1 + v2 * (T)c.w2) * (T)c.inv;
+}
+
+// The narrow-ray line loop of siddon_fp_ray_slices: lines wj0 .. wj1 of the S
+// slice bases; line j ends where line j + 1 starts (base + j + 1 is exact, so
+// the shared end point is the same FMA result for both lines)
+template <class T, int S, int NC>
+__device__ __forceinline__ void fp_narrow_lines(const T* const (&Pq)[S], int n, float4 r, float base, int wj0,
+                                                int wj1, T (&acc)[S])
+{
+    float ea = fmaf(r.y, base + (float)wj0, r.x);
+    constexpr int kUnroll = S > 4 ? 1 : (S > 2 ? 2 : 4);
+    #pragma unroll kUnroll
+    for (int j = wj0; j <= wj1; ++j) {
+        const float eb = fmaf(r.y, base + (float)(j + 1), r.x);
+        const Line3 c = line3(n, fminf(ea, eb), fmaxf(ea, eb));
+        const unsigned lboff = (unsigned)(j * n) * (unsigned)sizeof(T);
+        #pragma unroll
+        for (int q = 0; q < S; ++q) acc[q] += line_sum_at<T, NC>(Pq[q], lboff, c);
+        ea = eb;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T siddon_line_sum3(const T* __restrict__ line, int n, float lo, float hi)
+{
+    return line3_sum<T>(line, line3(n, lo, hi));
 }
 
 // wave-wide minimum / maximum (every lane of the wavefront must take part)
@@ -186,13 +279,18 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
-// One ray (bin iu of `view`); every lane of the wavefront calls it (inactive
-// lanes with active = false and any valid iu).  The lines are walked over the
-// union of the lanes' line ranges, so the loop and the line addresses are
-// wave-uniform; lanes outside their own range add exact zeros.
-template <class T>
-__device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ Pyx, const T* __restrict__ Pxy,
-                                           int view, int iu, bool active = true)
+// One ray (bin iu of `view`) through S slices of one geometry: slice q reads
+// planes Pyx + q * plane / Pxy + q * plane (slices q >= nsl read slice nsl - 1;
+// the caller discards their sums).  Every lane of the wavefront calls it
+// (inactive lanes with active = false and any valid iu).  The lines are walked
+// over the union of the lanes' line ranges, so the loop and the line
+// addresses are wave-uniform; lanes outside their own range add exact zeros.
+// The cells met and their weights are computed once per line and applied to
+// every slice, each slice with the arithmetic of a single-slice call.
+template <class T, int S>
+__device__ __forceinline__ void siddon_fp_ray_slices(const FanK& g, const T* __restrict__ Pyx,
+                                                     const T* __restrict__ Pxy, long plane, int nsl, int view,
+                                                     int iu, bool active, T (&acc)[S])
 {
     const float4 r = g.sray[(long)view * g.nu + iu];
     const bool xv = g.vcls[view] == 0;
@@ -216,27 +314,39 @@ __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ 
     if (!active) { j0 = nl; j1 = -1; }
     const int wj0 = __builtin_amdgcn_readfirstlane(wave_min(j0));
     const int wj1 = __builtin_amdgcn_readfirstlane(wave_max(j1));
-    // [lo, hi] (rounded) spans at most three cells on every lane
+    const T* Pq[S];                      // wave-uniform slice bases
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        Pq[q] = P + (long)min(q, nsl - 1) * plane;
+        acc[q] = T(0);
+    }
+    // [lo, hi] (rounded) spans at most three cells on every lane (two when
+    // |B| < 1: the rounding error of the end points is far below 1e-3)
     const bool narrow = __all(!active || fabsf(r.y) <= 1.999f);
-    T acc = T(0);
     if (narrow) {
-        // line j ends where line j + 1 starts: base + j + 1 is exact, so the
-        // shared end point is the same FMA result for both lines
-        float ea = fmaf(r.y, base + (float)wj0, r.x);
-        #pragma unroll 4
-        for (int j = wj0; j <= wj1; ++j) {
-            const float eb = fmaf(r.y, base + (float)(j + 1), r.x);
-            acc += siddon_line_sum3<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
-            ea = eb;
-        }
+        if (__all(!active || fabsf(r.y) <= 0.999f)) fp_narrow_lines<T, S, 2>(Pq, n, r, base, wj0, wj1, acc);
+        else fp_narrow_lines<T, S, 3>(Pq, n, r, base, wj0, wj1, acc);
     } else {
         for (int j = wj0; j <= wj1; ++j) {
             const float lr = base + (float)j;
             const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
-            acc += siddon_line_sum<T>(P + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+            #pragma unroll
+            for (int q = 0; q < S; ++q)
+                acc[q] += siddon_line_sum<T>(Pq[q] + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
         }
     }
-    return acc * (T)r.z;
+    #pragma unroll
+    for (int q = 0; q < S; ++q) acc[q] *= (T)r.z;
+}
+
+// One ray of one slice (siddon_fp_ray_slices with S = 1).
+template <class T>
+__device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ Pyx, const T* __restrict__ Pxy,
+                                           int view, int iu, bool active = true)
+{
+    T acc[1];
+    siddon_fp_ray_slices<T, 1>(g, Pyx, Pxy, 0, 1, view, iu, active, acc);
+    return acc[0];
 }
 
 // atan from an odd polynomial on [-1, 1] (coefficients fitted here by
@@ -299,39 +409,56 @@ __device__ __forceinline__ float siddon_cell_weight(const float4 r, float lr, fl
     return (pos ? ov * inv : pt) * r.z;
 }
 
-// the bins k0 .. k1 of one view: the first NB without branches (clamped,
-// zero-weighted when past k1), the rest in a loop
-template <class T, int NB, bool WIDE>
-__device__ __forceinline__ T siddon_gather_bins(const float4* __restrict__ R, const T* __restrict__ p, int nu, int k0,
-                                                int k1, float lr, float cell)
+// the bins k0 .. k1 of one view for S slices (slice q's row `rboff` bytes
+// after the wave-uniform base pq[q]; the view's ray table row `rowoff` bytes
+// after g.sray): the first NB without branches (clamped, zero-weighted when
+// past k1, so an empty range k1 < k0 adds exact zeros), the rest in a loop;
+// each weight is computed once and applied to every slice.
+template <class T, int NB, int S>
+__device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sray, unsigned rowoff,
+                                                   const T* const (&pq)[S], unsigned rboff, int nu, int k0, int k1,
+                                                   float lr, float cell, T (&acc)[S])
 {
-    T acc = T(0);
     #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const int kk = min(k0 + j, nu - 1);
-        const float4 r = R[kk];
-        const T pv = p[kk];
+        const unsigned kk = (unsigned)min(k0 + j, nu - 1);
+        const float4 r = ld_at(sray, rowoff + kk * 16u);
         const float valid = (k0 + j <= k1) ? 1.0f : 0.0f;
-        acc += pv * (T)(siddon_cell_weight<WIDE>(r, lr, cell) * valid);
+        const T wt = (T)(siddon_cell_weight<false>(r, lr, cell) * valid);
+        const unsigned boff = rboff + kk * (unsigned)sizeof(T);
+        #pragma unroll
+        for (int q = 0; q < S; ++q) acc[q] += ld_at(pq[q], boff) * wt;
     }
-    for (int k = k0 + NB; k <= k1; ++k) acc += p[k] * (T)siddon_cell_weight<WIDE>(R[k], lr, cell);
-    return acc;
+    for (int k = k0 + NB; k <= k1; ++k) {
+        const T wt = (T)siddon_cell_weight<false>(ld_at(sray, rowoff + (unsigned)k * 16u), lr, cell);
+        const unsigned boff = rboff + (unsigned)k * (unsigned)sizeof(T);
+        #pragma unroll
+        for (int q = 0; q < S; ++q) acc[q] += ld_at(pq[q], boff) * wt;
+    }
 }
 
 // Exact transpose of siddon_fp_ray: pixel (ix, iy) gathers, over views
-// view0 .. view0+nviews-1 of prj ([nviews][nu]), the rays that cross it, with
-// the weights siddon_fp_ray gives it.  The first NB bins of a pixel's range
-// are taken without branches, the rest in a loop (NB changes only the speed).
-template <class T, int NB, bool UNIT>
-__device__ __forceinline__ T siddon_bp_views(const FanK& g, const T* __restrict__ prj, int view0, int nviews, int ix,
-                                             int iy)
+// view0 .. view0+nviews-1 of prj ([nviews][nu] per slice, slices sstride
+// elements apart; slices q >= nsl read slice nsl - 1 and are discarded by the
+// caller), the rays that cross it, with the weights siddon_fp_ray gives it.
+// The bin range and the weights are computed once per view for all S slices.
+// The first NB bins of a pixel's range are taken without branches, the rest
+// in a loop (NB changes only the speed).
+template <class T, int NB, bool UNIT, int S>
+__device__ __forceinline__ void siddon_bp_views(const FanK& g, const T* __restrict__ prj, long sstride, int nsl,
+                                                int view0, int nviews, int ix, int iy, T (&acc)[S])
 {
     const float xc = (ix - 0.5f * (g.nx - 1)) * g.dx, yc = (iy - 0.5f * (g.ny - 1)) * g.dy;
     const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
     const float inv_da = 1.0f / g.da;
     const float yr = (float)iy - 0.5f * g.ny;
     const float xr = (float)ix - 0.5f * g.nx;
-    T acc = T(0);
+    const T* pq[S];                             // wave-uniform slice bases
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        pq[q] = prj + (long)min(q, nsl - 1) * sstride;
+        acc[q] = T(0);
+    }
     for (int vi = 0; vi < nviews; ++vi) {
         const int view = view0 + vi;
         int k0, k1;
@@ -339,21 +466,28 @@ __device__ __forceinline__ T siddon_bp_views(const FanK& g, const T* __restrict_
         const bool xv = g.vcls[view] == 0;
         const float lr = xv ? yr : xr;          // this pixel's line
         const float cell = xv ? (float)ix : (float)iy;
-        const float4* __restrict__ R = g.sray + (long)view * g.nu;
-        const T* __restrict__ p = prj + (long)vi * g.nu;
-        if (k1 < k0) continue;                  // the pixel is outside this view's fan
-        if (g.vwide[view]) acc += siddon_gather_bins<T, NB, true>(R, p, g.nu, k0, k1, lr, cell);
-        else acc += siddon_gather_bins<T, NB, false>(R, p, g.nu, k0, k1, lr, cell);
+        siddon_gather_bins<T, NB, S>(g.sray, (unsigned)(view * g.nu) * 16u, pq,
+                                     (unsigned)(vi * g.nu) * (unsigned)sizeof(T), g.nu, k0, k1, lr, cell, acc);
     }
-    return acc;
 }
 
+template <class T, int NB, int S>
+__device__ __forceinline__ void siddon_bp_pixel_slices(const FanK& g, const T* __restrict__ prj, long sstride,
+                                                       int nsl, int view0, int nviews, int ix, int iy,
+                                                       T (&acc)[S])
+{
+    if (g.unit_angle) siddon_bp_views<T, NB, true, S>(g, prj, sstride, nsl, view0, nviews, ix, iy, acc);
+    else siddon_bp_views<T, NB, false, S>(g, prj, sstride, nsl, view0, nviews, ix, iy, acc);
+}
+
+// one pixel of one slice
 template <class T, int NB = 4>
 __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict__ prj, int view0, int nviews,
                                              int ix, int iy)
 {
-    return g.unit_angle ? siddon_bp_views<T, NB, true>(g, prj, view0, nviews, ix, iy)
-                        : siddon_bp_views<T, NB, false>(g, prj, view0, nviews, ix, iy);
+    T acc[1];
+    siddon_bp_pixel_slices<T, NB, 1>(g, prj, 0, 1, view0, nviews, ix, iy, acc);
+    return acc[0];
 }
 
 // Exact transpose of dd_fp_ray (no reference counterpart: the reference DD BP

@@ -119,6 +119,10 @@ void RaggedBatch::setup(const std::vector<FanGeo>& geos, const std::vector<const
     long io = 0, so = 0;
     for (int s = 0; s < n; s++) {
         RN_REQUIRE(geos[s].nz == 1 && geos[s].nv == 1, "ragged batches hold 2D slices");
+        // the per-slice gathers address a plane, a ray table and a sinogram with 32-bit byte offsets
+        RN_REQUIRE((long)geos[s].nx * geos[s].ny * 4L < 0x7fffffffL &&
+                       (long)geos[s].nu * geos[s].nview * 16L < 0x7fffffffL,
+                   "a ragged slice's image, ray table or sinogram exceeds 2 GiB");
         p_->tables[s].build(geos[s], angles[s], st);
         p_->img_off[s] = io;
         p_->sino_off[s] = so;

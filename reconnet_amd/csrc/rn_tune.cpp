@@ -25,6 +25,7 @@ struct Entry {
 struct State {
     std::mutex mu;
     std::map<TuneKey, Entry> cache;
+    std::map<int, int> pinned;   // kernel -> candidate forced for every key
     int enabled = -1;   // -1: read RECONNET_AUTOTUNE on first use
 };
 
@@ -65,6 +66,13 @@ int tune_choose(const TuneKey& key, int ncand, const std::function<void(int)>& l
     State& s = state();
     {
         std::lock_guard<std::mutex> lk(s.mu);
+        auto pin = s.pinned.find(key.kernel);
+        if (pin != s.pinned.end()) {
+            RN_REQUIRE(pin->second < ncand, "pinned launch candidate " + std::to_string(pin->second) +
+                                                " out of range: kernel " + std::to_string(key.kernel) + " has " +
+                                                std::to_string(ncand));
+            return pin->second;
+        }
         auto it = s.cache.find(key);
         if (it != s.cache.end()) return it->second.choice;
         if (ncand <= 1 || !may_tune || !enabled_locked(s)) return 0;
@@ -136,6 +144,14 @@ void tune_reset()
     State& s = state();
     std::lock_guard<std::mutex> lk(s.mu);
     s.cache.clear();
+}
+
+void tune_pin(int kernel, int candidate)
+{
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    if (candidate < 0) s.pinned.erase(kernel);
+    else s.pinned[kernel] = candidate;
 }
 
 } // namespace rn

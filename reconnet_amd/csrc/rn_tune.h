@@ -48,5 +48,9 @@ bool tune_enabled();
 // "kernel a b c d e -> choice (ms per candidate ...)" lines for every tuned key
 std::string tune_report();
 void tune_reset();
+// Force candidate `candidate` for every key of `kernel` (a TuneKernel; a
+// negative candidate removes the pin).  A launch whose kernel has fewer
+// candidates fails with an error.  For tests of every launch shape.
+void tune_pin(int kernel, int candidate);
 
 } // namespace rn

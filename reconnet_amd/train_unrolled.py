@@ -68,12 +68,12 @@ def main(argv=None):
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--subsets", type=int, default=12)
     ap.add_argument("--i0", type=float, default=2.5e4)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=16)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--scale", type=float, default=50.0)
-    ap.add_argument("--out", default="gpurun_out/unrolled_fan512.npz")
-    ap.add_argument("--log", default="gpurun_out/train_log.json")
+    ap.add_argument("--out", default="unrolled_fan512.npz")
+    ap.add_argument("--log", default="train_log_fan512.json")
     args = ap.parse_args(argv)
 
     torch.cuda.set_device(0)

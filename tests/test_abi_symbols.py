@@ -60,6 +60,8 @@ def test_library_loads_with_ctypes_by_file_name():
     # a call that needs no device succeeds and clears the thread's last error
     assert lib.rn_autotune_reset() == 0
     assert lib.rn_last_error() == b""
+    # pinning a launch candidate is host state only
+    assert lib.rn_autotune_pin(1, 3) == 0 and lib.rn_autotune_pin(1, -1) == 0
 
 
 PFS_HEADER = os.path.join(ROOT, "include", "priorfunctionsolver.h")

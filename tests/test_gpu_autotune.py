@@ -36,3 +36,52 @@ def test_tuned_shapes_give_identical_results(fp, bp):
         assert 0 <= choice < len(ms) and len(ms) >= 3
         assert all(m > 0 for m in ms)
         assert ms[choice] == min(ms)
+
+
+@pytest.mark.parametrize("ns", [1, 3, 11])
+def test_every_launch_shape_agrees(ns):
+    # every candidate of the Siddon FP / BP and DD BP / FBP launches (block
+    # shapes, XCD order, slices per lane S = 1, 2, 4, 8) on slice counts that
+    # leave partial groups of S, against candidate 0 (one slice per lane)
+    from reconnet_amd._lib import autotune, autotune_pin
+    from reconnet_amd.torch_ops import FanPlan
+
+    geo = small_fan(nx=96, nu=140, nview=60, dx=1.3)
+    plan = FanPlan(geo)
+    rng = np.random.default_rng(11 + ns)
+    x = torch.from_numpy(rng.uniform(0, 1, (ns, geo.nx, geo.ny)).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.uniform(0, 1, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
+    counts = {}
+    try:
+        autotune(enable=True, reset=True)
+        plan.forward(x, "siddon")
+        plan.backward(y, "siddon")
+        plan.backward(y, "dd")
+        plan.backward(y, "dd_fbp")
+        for (kernel, *_), (_, ms) in autotune().items():
+            counts[kernel] = len(ms)
+        assert counts.get(1, 0) >= 13 and counts.get(2, 0) >= 13
+        assert counts.get(5, 0) >= 8 and counts.get(7, 0) >= 8
+        refs = {}
+        for kname, kid, run in (("siddon_fp", 1, lambda: plan.forward(x, "siddon")),
+                                ("siddon_bp", 2, lambda: plan.backward(y, "siddon")),
+                                ("dd_bp", 5, lambda: plan.backward(y, "dd")),
+                                ("dd_fbp", 7, lambda: plan.backward(y, "dd_fbp"))):
+            for c in range(counts[kid]):
+                autotune_pin(kname, c)
+                out = run()
+                torch.cuda.synchronize()
+                if c == 0:
+                    refs[kname] = out
+                    continue
+                ref = refs[kname]
+                err = float((out - ref).abs().max() / ref.abs().max())
+                assert err <= 1e-6, (kname, c, err)
+            autotune_pin(kname, None)
+        autotune_pin("siddon_fp", counts[1])
+        with pytest.raises(RuntimeError, match="out of range"):
+            plan.forward(x, "siddon")
+    finally:
+        for k in ("siddon_fp", "siddon_bp", "dd_bp", "dd_fbp"):
+            autotune_pin(k, None)
+        autotune(enable=True, reset=True)
