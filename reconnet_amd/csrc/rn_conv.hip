@@ -75,16 +75,54 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
 // ---------------------------------------------------------------------------
 // 32 -> 32 layer on MFMA.  wt: [9][32 co][32 ci] bf16 (tap t = 3*dy + dx).
 // mode 0: out = relu?(acc + bias); mode 1: out = acc * (mask > 0)
+//
+// One wave walks down a column of 16 pixels x 32 channels of one image over L
+// output rows; the four waves of a workgroup take adjacent 16-pixel columns,
+// so the grid is (W / 64, H / L, B) and needs no index division.  Each input
+// row is read once as three column-shifted B fragments (16 pixels x 32
+// channels, 16 bytes per lane) and feeds the three output rows that use it
+// (taps dy = 2, 1, 0 of rows y - 1, y, y + 1) through three rolling
+// accumulators; the oldest is complete after its third row and is stored.
+// The weight fragments of all 9 taps and both 16-channel output tiles stay in
+// registers for the L rows.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void conv_mid_store(const f32x4 (&acc)[2], const float (&bv)[2][4],
+                                               const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
+                                               long pbase, int g4, int relu, int mode)
+{
+    #pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int c0 = 16 * mt + 4 * g4;
+        float z[4];
+        if (mode == 0) {
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) { z[i] = acc[mt][i] + bv[mt][i]; if (relu) z[i] = fmaxf(z[i], 0.0f); }
+        } else {
+            const uint2 m = *reinterpret_cast<const uint2*>(mask + pbase + c0);
+            const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16),
+                                    (uint16_t)(m.y & 0xffff), (uint16_t)(m.y >> 16)};
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
+        }
+        uint2 ov;
+        ov.x = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+        ov.y = (uint32_t)f2bf(z[2]) | ((uint32_t)f2bf(z[3]) << 16);
+        *reinterpret_cast<uint2*>(out + pbase + c0) = ov;
+    }
+}
+
+template <int L>
 __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
                                                   const float* __restrict__ bias, const uint16_t* __restrict__ mask,
-                                                  uint16_t* __restrict__ out, int B, int H, int W, int relu,
-                                                  int mode)
+                                                  uint16_t* __restrict__ out, int H, int W, int relu, int mode)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int wv = threadIdx.x >> 6;
     const int r16 = lane & 15, g4 = lane >> 4;
+    const int xpix = (blockIdx.x * 4 + wv) * 16 + r16;   // this lane's B column (pixel)
+    const bool store = xpix < W;
+    const int y0 = blockIdx.y * L;
+    const long img = (long)blockIdx.z * H * W;
     // weight fragments: A[m = co][k = ci], lane holds co = r16 (+16 mt), ci = 8 g4 .. 8 g4 + 7
     bf16x8 wa[9][2];
     #pragma unroll
@@ -92,53 +130,42 @@ __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ i
         #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
             wa[t][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((t * 32 + 16 * mt + r16) * 32 + 8 * g4)));
+    // D[m = co][n = pixel]: lane holds co = 16 mt + 4 g4 + i for pixel r16
     float bv[2][4];
     #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
         #pragma unroll
         for (int i = 0; i < 4; ++i) bv[mt][i] = (mode == 0) ? bias[16 * mt + 4 * g4 + i] : 0.0f;
-    const int segs_per_row = (W + 15) / 16;
-    const long nseg = (long)B * H * segs_per_row;
-    for (long sg = wave; sg < nseg; sg += nwaves) {
-        const int sx = (int)(sg % segs_per_row);
-        const long by = sg / segs_per_row;
-        const int yh = (int)(by % H);
-        const long bimg = by / H;
-        const int xpix = sx * 16 + r16;                // this lane's B column (pixel)
-        f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    // a0: output row yy - 1 (gets tap dy = 2 and completes), a1: row yy (dy = 1), a2: row yy + 1 (dy = 0)
+    f32x4 a0[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, a1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    f32x4 a2[2];
+    #pragma unroll 3
+    for (int ir = 0; ir < L + 2; ++ir) {
+        const int yy = y0 + ir - 1;                      // input row
+        bf16x8 f[3];
         #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int yy = yh + t / 3 - 1, xx = xpix + t % 3 - 1;
+        for (int dx = 0; dx < 3; ++dx) {
+            const int xx = xpix + dx - 1;
             uint4 v = make_uint4(0, 0, 0, 0);
             if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = *reinterpret_cast<const uint4*>(in + (((bimg * H + yy) * W + xx) * 32 + 8 * g4));
-            const bf16x8 bfrag = as_bf16x8(v);
-            acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][0], bfrag, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t][1], bfrag, acc[1], 0, 0, 0);
+                v = *reinterpret_cast<const uint4*>(in + ((img + (long)yy * W + xx) * 32 + 8 * g4));
+            f[dx] = as_bf16x8(v);
         }
-        // D[m = co][n = pixel]: lane holds co = 16 mt + 4 g4 + i for pixel r16
-        if (xpix < W) {
-            const long pbase = ((bimg * H + yh) * W + xpix) * 32;
+        a2[0] = a2[1] = f32x4{0, 0, 0, 0};
+        #pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
             #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
-                const int c0 = 16 * mt + 4 * g4;
-                float z[4];
-                if (mode == 0) {
-                    #pragma unroll
-                    for (int i = 0; i < 4; ++i) { z[i] = acc[mt][i] + bv[mt][i]; if (relu) z[i] = fmaxf(z[i], 0.0f); }
-                } else {
-                    const uint2 m = *reinterpret_cast<const uint2*>(mask + pbase + c0);
-                    const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16),
-                                            (uint16_t)(m.y & 0xffff), (uint16_t)(m.y >> 16)};
-                    #pragma unroll
-                    for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
-                }
-                uint2 o;
-                o.x = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
-                o.y = (uint32_t)f2bf(z[2]) | ((uint32_t)f2bf(z[3]) << 16);
-                *reinterpret_cast<uint2*>(out + pbase + c0) = o;
+                a0[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx][mt], f[dx], a0[mt], 0, 0, 0);
+                a1[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx][mt], f[dx], a1[mt], 0, 0, 0);
+                a2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx][mt], f[dx], a2[mt], 0, 0, 0);
             }
         }
+        const int yo = yy - 1;                           // the row a0 completes
+        if (ir >= 2 && yo < H && store) conv_mid_store(a0, bv, mask, out, (img + (long)yo * W + xpix) * 32, g4,
+                                                       relu, mode);
+        a0[0] = a1[0]; a0[1] = a1[1];
+        a1[0] = a2[0]; a1[1] = a2[1];
     }
 }
 
@@ -415,9 +442,10 @@ void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const u
               int B, int H, int W, bool relu, hipStream_t st)
 {
     RN_REQUIRE(mask != nullptr || bias != nullptr, "conv_mid needs a bias (forward) or a mask (data gradient)");
-    const long nseg = (long)B * H * ((W + 15) / 16);
-    const int blocks = grid_for(nseg, 4 * 4, 4096);
-    hipLaunchKernelGGL(k_conv_mid, dim3(blocks), dim3(256), 0, st, in, wt, bias, mask, out, B, H, W, relu ? 1 : 0,
+    RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "conv_mid: bad batch or image size");
+    constexpr int kRows = 16;
+    const dim3 grd((unsigned)cdiv(W, 64), (unsigned)cdiv(H, kRows), (unsigned)B);
+    hipLaunchKernelGGL(k_conv_mid<kRows>, grd, dim3(256), 0, st, in, wt, bias, mask, out, H, W, relu ? 1 : 0,
                        mask ? 1 : 0);
     check_launch("conv_mid");
 }

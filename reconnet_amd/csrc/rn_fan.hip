@@ -120,6 +120,77 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp(FanK g, const T* __re
     }
 }
 
+// Quad-layout Siddon FP + residual: each lane carries G groups of 4 slices
+// (one float4 load per cell and group); the grid's slice index counts groups
+// of G quads.  res = w * (Ax - meas) per component.
+template <int BLK, int XCD, int G>
+__global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* __restrict__ Qyx,
+                                                         const float4* __restrict__ Qxy, float4* __restrict__ res,
+                                                         int view0, int nviews, int nquads,
+                                                         const float4* __restrict__ meas,
+                                                         const float4* __restrict__ wgt, int meas_nviews)
+{
+    int tile, vi, sg;
+    if (XCD > 0) {
+        const int per = gridDim.x / XCD;
+        const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
+        const int tiles = (g.nu + BLK - 1) / BLK;
+        tile = l % tiles;
+        vi = (l / tiles) % nviews;
+        sg = l / (tiles * nviews);
+    } else {
+        tile = blockIdx.x;
+        vi = blockIdx.y;
+        sg = blockIdx.z;
+    }
+    const int iu = tile * BLK + threadIdx.x;
+    const bool active = iu < g.nu;
+    const int view = view0 + vi;
+    const long plane = (long)g.nx * g.ny;
+    const int q0 = sg * G;
+    const int nq = min(G, nquads - q0);
+    float4 acc[G];
+    siddon_fp_ray_slices<float4, G>(g, Qyx + q0 * plane, Qxy + q0 * plane, plane, nq, view,
+                                    active ? iu : g.nu - 1, active, acc);
+    if (!active) return;
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= nq) break;
+        const long mi = ((long)(q0 + k) * meas_nviews + view) * g.nu + iu;
+        const float4 m = meas[mi], w = wgt[mi];
+        res[((long)(q0 + k) * nviews + vi) * g.nu + iu] =
+            make_float4(w.x * (acc[k].x - m.x), w.y * (acc[k].y - m.y), w.z * (acc[k].z - m.z),
+                        w.w * (acc[k].w - m.w));
+    }
+}
+
+// Quad-layout Siddon BP: one lane per pixel of G groups of 4 slices (one
+// float4 load per bin and group); writes the plain images of the real slices.
+template <int BX, int BY, int NB, int G>
+__global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp_q(FanK g, const float4* __restrict__ prj,
+                                                         float* __restrict__ img, int view0, int nviews, int ns)
+{
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
+    const int nquads = (ns + 3) / 4;
+    const int q0 = blockIdx.z * G;
+    const int nq = min(G, nquads - q0);
+    if (ix >= g.nx || iy >= g.ny) return;
+    const long sstride = (long)nviews * g.nu;
+    float4 acc[G];
+    siddon_bp_pixel_slices<float4, NB, G>(g, prj + q0 * sstride, sstride, nq, view0, nviews, ix, iy, acc);
+    const long plane = (long)g.nx * g.ny, pix = (long)ix * g.ny + iy;
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= nq) break;
+        const int s = 4 * (q0 + k);
+        const float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+        #pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (s + c < ns) img[(long)(s + c) * plane + pix] = v[c];
+    }
+}
+
 template <int BX, int BY>
 __global__ __launch_bounds__(BX * BY) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
                                                     float* __restrict__ img, int view0, int nviews, int accumulate)
@@ -227,6 +298,37 @@ __global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* _
 // ---------------------------------------------------------------------------
 // layout helpers
 // ---------------------------------------------------------------------------
+
+// q_same[q][i][j] = (in[4q][i][j], .., in[4q+3][i][j]) (zero past nslices) and,
+// if q_tr, q_tr[q][j][i] = the same value; 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void k_pack_quads(const float* __restrict__ in, float4* __restrict__ q_same,
+                                                    float4* __restrict__ q_tr, int n1, int n2, int ns)
+{
+    __shared__ float4 tile[32][33];
+    const int j0 = blockIdx.x * 32, i0 = blockIdx.y * 32, q = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const long plane = (long)n1 * n2;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < n1 && j < n2) {
+            float v[4];
+            #pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int s = 4 * q + c;
+                v[c] = s < ns ? in[(long)s * plane + (long)i * n2 + j] : 0.0f;
+            }
+            const float4 f = make_float4(v[0], v[1], v[2], v[3]);
+            tile[r][tx] = f;
+            q_same[(long)q * plane + (long)i * n2 + j] = f;
+        }
+    }
+    if (q_tr == nullptr) return;
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + r, i = i0 + tx;
+        if (i < n1 && j < n2) q_tr[(long)q * plane + (long)j * n1 + i] = tile[tx][r];
+    }
+}
 
 __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, float* __restrict__ out,
                                                    int rows, int cols)
@@ -499,6 +601,42 @@ void require_offsets32(const FanK& g, int nviews, size_t elem)
                "image plane, ray table or sinogram of one slice exceeds 2 GiB");
 }
 
+template <int BLK, int XCD, int G>
+void go_siddon_fp_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res, int nquads, int view0,
+                    int nviews, const float4* meas, const float4* w, hipStream_t st)
+{
+    const int ngroups = cdiv(nquads, G);
+    const long nblk = (long)cdiv(g.nu, BLK) * nviews * ngroups;
+    if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
+        hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, XCD, G>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Qyx, Qxy,
+                           res, view0, nviews, nquads, meas, w, g.nview);
+    } else {
+        hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, 0, G>), dim3(cdiv(g.nu, BLK), nviews, ngroups), dim3(BLK), 0, st,
+                           g, Qyx, Qxy, res, view0, nviews, nquads, meas, w, g.nview);
+    }
+}
+
+template <int BX, int BY, int NB, int G>
+void go_siddon_bp_q(const FanK& g, const float4* prj, float* img, int ns, int view0, int nviews, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_siddon_fan_bp_q<BX, BY, NB, G>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
+                       dim3(BX, BY), 0, st, g, prj, img, view0, nviews, ns);
+}
+
+using SidFpQFn = void (*)(const FanK&, const float4*, const float4*, float4*, int, int, int, const float4*,
+                          const float4*, hipStream_t);
+using SidBpQFn = void (*)(const FanK&, const float4*, float*, int, int, int, hipStream_t);
+// G: groups of 4 slices per lane
+const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<128, kXcds, 1>, go_siddon_fp_q<64, kXcds, 1>,
+                            go_siddon_fp_q<256, kXcds, 1>, go_siddon_fp_q<128, 0, 1>,
+                            go_siddon_fp_q<64, 0, 1>,      go_siddon_fp_q<128, kXcds, 2>,
+                            go_siddon_fp_q<128, 0, 2>};
+const SidBpQFn kSidBpQ[] = {go_siddon_bp_q<8, 8, 2, 2>,   go_siddon_bp_q<8, 8, 2, 1>, go_siddon_bp_q<16, 16, 2, 1>,
+                            go_siddon_bp_q<16, 16, 2, 2>, go_siddon_bp_q<8, 8, 2, 4>, go_siddon_bp_q<16, 16, 2, 4>,
+                            go_siddon_bp_q<8, 8, 3, 2>,   go_siddon_bp_q<8, 8, 3, 4>};
+constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
+constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);
+
 } // namespace
 
 void launch_dd_fan_fp(const FanK& g, const float* RX, const float* RY, float* prj_vu,
@@ -534,6 +672,32 @@ void launch_siddon_fan_fp_residual(const FanK& g, const float* Pyx, const float*
     }, st, true);
     kSidFp[c](g, Pyx, Pxy, res_vu, nslices, view0, nviews, meas_vu, w_vu, st);
     check_launch("siddon_fan_fp_residual");
+}
+
+void launch_siddon_fan_fp_residual_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res_q,
+                                     int nslices, int view0, int nviews, const float4* meas_q, const float4* w_q,
+                                     hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    require_offsets32(g, nviews, sizeof(float4));
+    const int nq = cdiv(nslices, 4);
+    const int c = tune_choose({TK_SIDDON_FP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidFpQ, [&](int i) {
+        kSidFpQ[i](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, st);
+    }, st, true);
+    kSidFpQ[c](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, st);
+    check_launch("siddon_fan_fp_residual_q");
+}
+
+void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, int nslices, int view0,
+                            int nviews, hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    require_offsets32(g, nviews, sizeof(float4));
+    const int c = tune_choose({TK_SIDDON_BP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidBpQ, [&](int i) {
+        kSidBpQ[i](g, prj_q, img_xy, nslices, view0, nviews, st);
+    }, st, true);
+    kSidBpQ[c](g, prj_q, img_xy, nslices, view0, nviews, st);
+    check_launch("siddon_fan_bp_q");
 }
 
 void launch_siddon_fan_fp_f64(const FanK& g, const double* Pyx, const double* Pxy, double* prj_vu,
@@ -605,6 +769,14 @@ void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int 
     dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
     hipLaunchKernelGGL((k_pixel_fan_bp<16, 16>), grd, blk, 0, st, g, prj_vu, img_xy);
     check_launch("pixel_fan_bp");
+}
+
+void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, hipStream_t st)
+{
+    if (nslices <= 0 || n1 <= 0 || n2 <= 0) return;
+    dim3 grd(cdiv(n2, 32), cdiv(n1, 32), cdiv(nslices, 4));
+    hipLaunchKernelGGL(k_pack_quads, grd, dim3(256), 0, st, in, q_same, q_tr, n1, n2, nslices);
+    check_launch("pack_quads");
 }
 
 void launch_transpose(const float* in, float* out, int rows, int cols, int batch, hipStream_t st)

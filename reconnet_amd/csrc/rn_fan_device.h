@@ -17,6 +17,27 @@ __device__ __forceinline__ T ld_at(const T* __restrict__ base, unsigned boff)
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
 }
 
+// Values a lane carries: one slice (float, or double for the FP64
+// normaliser) or four interleaved slices (float4, each component with the
+// arithmetic of the float case).
+template <class T>
+__device__ __forceinline__ T vzero() { return T(0); }
+template <>
+__device__ __forceinline__ float4 vzero<float4>() { return make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+__device__ __forceinline__ void vadd(float& a, float b) { a += b; }
+__device__ __forceinline__ void vadd(double& a, double b) { a += b; }
+__device__ __forceinline__ void vadd(float4& a, float4 b) { a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w; }
+// a += v * w (w converted to the value type first)
+__device__ __forceinline__ void vmadd(float& a, float v, float w) { a += v * w; }
+__device__ __forceinline__ void vmadd(double& a, double v, float w) { a += v * (double)w; }
+__device__ __forceinline__ void vmadd(float4& a, float4 v, float w)
+{
+    a.x += v.x * w; a.y += v.y * w; a.z += v.z * w; a.w += v.w * w;
+}
+__device__ __forceinline__ float vscale(float v, float f) { return v * f; }
+__device__ __forceinline__ double vscale(double v, float f) { return v * (double)f; }
+__device__ __forceinline__ float4 vscale(float4 v, float f) { return make_float4(v.x * f, v.y * f, v.z * f, v.w * f); }
+
 // ---------------------------------------------------------------------------
 // geometry helpers
 // ---------------------------------------------------------------------------
@@ -160,13 +181,13 @@ template <class T>
 __device__ __forceinline__ T seg_sum(const T* __restrict__ line, int n, float lo, float hi)
 {
     const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
-    if (!(h > l)) return T(0);
+    T s = vzero<T>();
+    if (!(h > l)) return s;
     const int m0 = (int)l;
     const int m1 = min((int)h, n - 1);
-    T s = T(0);
     for (int m = m0; m <= m1; ++m) {
         const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
-        s += line[m] * (T)(ph - pl);
+        vmadd(s, line[m], ph - pl);
     }
     return s;
 }
@@ -186,8 +207,8 @@ template <class T>
 __device__ __forceinline__ T siddon_line_sum(const T* __restrict__ line, int n, float lo, float hi)
 {
     const float w = hi - lo;
-    if (w > 0.0f) return seg_sum<T>(line, n, lo, hi) * (T)__builtin_amdgcn_rcpf(w);
-    return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : T(0);
+    if (w > 0.0f) return vscale(seg_sum<T>(line, n, lo, hi), __builtin_amdgcn_rcpf(w));
+    return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : vzero<T>();
 }
 
 // siddon_line_sum for hi - lo < 2 (at most three cells) without branches:
@@ -228,6 +249,22 @@ __device__ __forceinline__ T line3_sum(const T* __restrict__ line, const Line3& 
     return (line[c.i0] * (T)c.w0 + line[c.i1] * (T)c.w1 + line[c.i2] * (T)c.w2) * (T)c.inv;
 }
 
+// (v0 w0 + v1 w1 [+ v2 w2]) / width of one line, per value component
+__device__ __forceinline__ float lcomb(float v0, float v1, float v2, const Line3& c, bool three)
+{
+    return three ? (v0 * c.w0 + v1 * c.w1 + v2 * c.w2) * c.inv : (v0 * c.w0 + v1 * c.w1) * c.inv;
+}
+__device__ __forceinline__ double lcomb(double v0, double v1, double v2, const Line3& c, bool three)
+{
+    return three ? (v0 * (double)c.w0 + v1 * (double)c.w1 + v2 * (double)c.w2) * (double)c.inv
+                 : (v0 * (double)c.w0 + v1 * (double)c.w1) * (double)c.inv;
+}
+__device__ __forceinline__ float4 lcomb(float4 v0, float4 v1, float4 v2, const Line3& c, bool three)
+{
+    return make_float4(lcomb(v0.x, v1.x, v2.x, c, three), lcomb(v0.y, v1.y, v2.y, c, three),
+                       lcomb(v0.z, v1.z, v2.z, c, three), lcomb(v0.w, v1.w, v2.w, c, three));
+}
+
 // line3_sum of the line starting `lboff` bytes after a wave-uniform base,
 // over the first NC cells (NC = 2 when the interval is known to be narrower
 // than one cell: the third weight is then exactly 0 and its term is dropped)
@@ -236,9 +273,8 @@ __device__ __forceinline__ T line_sum_at(const T* __restrict__ base, unsigned lb
 {
     const T v0 = ld_at(base, lboff + c.i0 * (unsigned)sizeof(T));
     const T v1 = ld_at(base, lboff + c.i1 * (unsigned)sizeof(T));
-    if (NC == 2) return (v0 * (T)c.w0 + v1 * (T)c.w1) * (T)c.inv;
-    const T v2 = ld_at(base, lboff + c.i2 * (unsigned)sizeof(T));
-    return (v0 * (T)c.w0 + v1 * (T)c.w1 + v2 * (T)c.w2) * (T)c.inv;
+    const T v2 = NC == 2 ? v1 : ld_at(base, lboff + c.i2 * (unsigned)sizeof(T));
+    return lcomb(v0, v1, v2, c, NC == 3);
 }
 
 // The narrow-ray line loop of siddon_fp_ray_slices: lines wj0 .. wj1 of the S
@@ -256,7 +292,7 @@ __device__ __forceinline__ void fp_narrow_lines(const T* const (&Pq)[S], int n, 
         const Line3 c = line3(n, fminf(ea, eb), fmaxf(ea, eb));
         const unsigned lboff = (unsigned)(j * n) * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) acc[q] += line_sum_at<T, NC>(Pq[q], lboff, c);
+        for (int q = 0; q < S; ++q) vadd(acc[q], line_sum_at<T, NC>(Pq[q], lboff, c));
         ea = eb;
     }
 }
@@ -318,7 +354,7 @@ __device__ __forceinline__ void siddon_fp_ray_slices(const FanK& g, const T* __r
     #pragma unroll
     for (int q = 0; q < S; ++q) {
         Pq[q] = P + (long)min(q, nsl - 1) * plane;
-        acc[q] = T(0);
+        acc[q] = vzero<T>();
     }
     // [lo, hi] (rounded) spans at most three cells on every lane (two when
     // |B| < 1: the rounding error of the end points is far below 1e-3)
@@ -332,11 +368,11 @@ __device__ __forceinline__ void siddon_fp_ray_slices(const FanK& g, const T* __r
             const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
             #pragma unroll
             for (int q = 0; q < S; ++q)
-                acc[q] += siddon_line_sum<T>(Pq[q] + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb));
+                vadd(acc[q], siddon_line_sum<T>(Pq[q] + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb)));
         }
     }
     #pragma unroll
-    for (int q = 0; q < S; ++q) acc[q] *= (T)r.z;
+    for (int q = 0; q < S; ++q) acc[q] = vscale(acc[q], r.z);
 }
 
 // One ray of one slice (siddon_fp_ray_slices with S = 1).
@@ -424,16 +460,16 @@ __device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sr
         const unsigned kk = (unsigned)min(k0 + j, nu - 1);
         const float4 r = ld_at(sray, rowoff + kk * 16u);
         const float valid = (k0 + j <= k1) ? 1.0f : 0.0f;
-        const T wt = (T)(siddon_cell_weight<false>(r, lr, cell) * valid);
+        const float wt = siddon_cell_weight<false>(r, lr, cell) * valid;
         const unsigned boff = rboff + kk * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) acc[q] += ld_at(pq[q], boff) * wt;
+        for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(pq[q], boff), wt);
     }
     for (int k = k0 + NB; k <= k1; ++k) {
-        const T wt = (T)siddon_cell_weight<false>(ld_at(sray, rowoff + (unsigned)k * 16u), lr, cell);
+        const float wt = siddon_cell_weight<false>(ld_at(sray, rowoff + (unsigned)k * 16u), lr, cell);
         const unsigned boff = rboff + (unsigned)k * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) acc[q] += ld_at(pq[q], boff) * wt;
+        for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(pq[q], boff), wt);
     }
 }
 
@@ -457,7 +493,7 @@ __device__ __forceinline__ void siddon_bp_views(const FanK& g, const T* __restri
     #pragma unroll
     for (int q = 0; q < S; ++q) {
         pq[q] = prj + (long)min(q, nsl - 1) * sstride;
-        acc[q] = T(0);
+        acc[q] = vzero<T>();
     }
     for (int vi = 0; vi < nviews; ++vi) {
         const int view = view0 + vi;

@@ -51,6 +51,11 @@ private:
     FanTables t_;
     int ns_ = 0, npers_ = 0, nsub_ = 0;
     DevBuf B_, W_, D_, X_[2], Xyx_, R_, G_, red_;
+    // interleaved layout (groups of 4 slices as float4), used for 4 or more
+    // slices unless RECONNET_SQS_INTERLEAVE=0: images in both orientations,
+    // measurement, weights and the subset residual
+    bool quads_ = false;
+    DevBuf Qxy_, Qyx_, B4_, W4_, R4_;
     int cur_ = 0;
     bool have_w_ = false;
     bool prof_ = false;

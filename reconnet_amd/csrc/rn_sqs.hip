@@ -13,6 +13,8 @@
 
 #include "rn_sqs.h"
 
+#include <cstdlib>
+
 #include <cstring>
 
 namespace rn {
@@ -195,9 +197,20 @@ void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nview
     D_.ensure(ns_ * plane * sizeof(float));
     X_[0].ensure(ns_ * plane * sizeof(float));
     X_[1].ensure(ns_ * plane * sizeof(float));
-    Xyx_.ensure(ns_ * plane * sizeof(float));
-    R_.ensure(ns_ * (size_t)g.nu * npers_ * sizeof(float));
     G_.ensure(ns_ * plane * sizeof(float));
+    const char* env = std::getenv("RECONNET_SQS_INTERLEAVE");
+    quads_ = ns_ >= 4 && !(env && env[0] == '0');
+    if (quads_) {
+        const size_t nq = (size_t)(ns_ + 3) / 4;
+        Qxy_.ensure(nq * plane * sizeof(float4));
+        Qyx_.ensure(nq * plane * sizeof(float4));
+        B4_.ensure(nq * sino * sizeof(float4));
+        W4_.ensure(nq * sino * sizeof(float4));
+        R4_.ensure(nq * (size_t)g.nu * npers_ * sizeof(float4));
+    } else {
+        Xyx_.ensure(ns_ * plane * sizeof(float));
+        R_.ensure(ns_ * (size_t)g.nu * npers_ * sizeof(float));
+    }
 }
 
 void OsSqs::set_data(const float* prj_ref, const float* w_ref, hipStream_t st)
@@ -206,6 +219,10 @@ void OsSqs::set_data(const float* prj_ref, const float* w_ref, hipStream_t st)
     launch_transpose(prj_ref, B_.as<float>(), g.nu, g.nview, ns_, st);
     if (w_ref) launch_transpose(w_ref, W_.as<float>(), g.nu, g.nview, ns_, st);
     else launch_fill(W_.as<float>(), 1.0f, (size_t)ns_ * g.nu * g.nview, st);
+    if (quads_) {
+        launch_pack_quads(B_.as<float>(), B4_.as<float4>(), nullptr, g.nview, g.nu, ns_, st);
+        launch_pack_quads(W_.as<float>(), W4_.as<float4>(), nullptr, g.nview, g.nu, ns_, st);
+    }
     have_w_ = true;
 }
 
@@ -238,7 +255,9 @@ void OsSqs::set_norm(const float* norm_xy, hipStream_t st)
 
 void OsSqs::sync_transposed(hipStream_t st)
 {
-    launch_transpose(X_[cur_].as<float>(), Xyx_.as<float>(), t_.geo.nx, t_.geo.ny, ns_, st);
+    if (quads_) launch_pack_quads(X_[cur_].as<float>(), Qxy_.as<float4>(), Qyx_.as<float4>(), t_.geo.nx, t_.geo.ny,
+                                  ns_, st);
+    else launch_transpose(X_[cur_].as<float>(), Xyx_.as<float>(), t_.geo.nx, t_.geo.ny, ns_, st);
 }
 
 OsSqs::~OsSqs()
@@ -298,15 +317,22 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         float* X = X_[cur_].as<float>();
         float* Xn = X_[1 - cur_].as<float>();
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k], st));
-        launch_siddon_fan_fp_residual(t_.k, Xyx_.as<float>(), X, R_.as<float>(), ns_, start, cur,
-                                      B_.as<float>(), W_.as<float>(), st);
+        if (quads_) {
+            launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
+                                            start, cur, B4_.as<float4>(), W4_.as<float4>(), st);
+        } else {
+            launch_siddon_fan_fp_residual(t_.k, Xyx_.as<float>(), X, R_.as<float>(), ns_, start, cur,
+                                          B_.as<float>(), W_.as<float>(), st);
+        }
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 1], st));
-        launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
+        if (quads_) launch_siddon_fan_bp_q(t_.k, R4_.as<float4>(), G_.as<float>(), ns_, start, cur, st);
+        else launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 2], st));
         hipLaunchKernelGGL(k_sqs_update, grd, blk, 0, st, X, Xn, G_.as<float>(), D_.as<float>(), g.nx, g.ny,
                            o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
         check_launch("sqs_update");
-        launch_transpose(Xn, Xyx_.as<float>(), g.nx, g.ny, ns_, st);
+        if (quads_) launch_pack_quads(Xn, Qxy_.as<float4>(), Qyx_.as<float4>(), g.nx, g.ny, ns_, st);
+        else launch_transpose(Xn, Xyx_.as<float>(), g.nx, g.ny, ns_, st);
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 3], st));
         cur_ = 1 - cur_;
         start += cur;

@@ -35,6 +35,8 @@ enum TuneKernel {
     TK_DD_BP = 5,
     TK_SQS_UPDATE = 6,
     TK_DD_FBP = 7,
+    TK_SIDDON_FP_Q = 8,
+    TK_SIDDON_BP_Q = 9,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

@@ -85,3 +85,67 @@ def test_every_launch_shape_agrees(ns):
         for k in ("siddon_fp", "siddon_bp", "dd_bp", "dd_fbp"):
             autotune_pin(k, None)
         autotune(enable=True, reset=True)
+
+
+def _engine(ns, seed=3):
+    import torch
+    from reconnet_amd.sqs import OsSqsTV
+
+    geo = small_fan(nx=96, nu=140, nview=60, dx=1.3)
+    rng = np.random.default_rng(seed)
+    b = torch.from_numpy(rng.uniform(0, 2, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
+    w = torch.from_numpy(rng.uniform(0.5, 1, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
+    x0 = torch.from_numpy(rng.uniform(0, 0.05, (ns, geo.nx, geo.ny)).astype(np.float32)).cuda()
+    eng = OsSqsTV(geo, 6, ns)
+    eng.set_data(b, w)
+    eng.compute_norm()
+    return eng, x0
+
+
+def test_every_interleaved_launch_shape_agrees():
+    # the OS-SQS engine's 4-slice interleaved FP / BP: every candidate gives
+    # the pass of candidate 0 (6 slices: one full and one padded group)
+    from reconnet_amd._lib import autotune, autotune_pin
+
+    eng, x0 = _engine(6)
+    try:
+        autotune(enable=True, reset=True)
+        eng.set_image(x0)
+        eng.run_pass(beta=1e-3, eps=1e-4)
+        counts = {k[0]: len(ms) for k, (_, ms) in autotune().items()}
+        assert counts.get(8, 0) >= 7 and counts.get(9, 0) >= 8, counts
+        for kname, kid in (("siddon_fp_q", 8), ("siddon_bp_q", 9)):
+            ref = None
+            for c in range(counts[kid]):
+                autotune_pin(kname, c)
+                eng.set_image(x0)
+                eng.run_pass(beta=1e-3, eps=1e-4)
+                out = eng.image()
+                if ref is None:
+                    ref = out
+                    continue
+                err = float((out - ref).abs().max() / ref.abs().max())
+                assert err <= 1e-6, (kname, c, err)
+            autotune_pin(kname, None)
+    finally:
+        autotune_pin("siddon_fp_q", None)
+        autotune_pin("siddon_bp_q", None)
+        autotune(enable=True, reset=True)
+        eng.close()
+
+
+@pytest.mark.parametrize("ns", [4, 7])
+def test_interleaved_engine_matches_plain_layout(ns, monkeypatch):
+    # the same passes with the engine's plain one-slice layout
+    # (RECONNET_SQS_INTERLEAVE=0) and with the interleaved 4-slice layout
+    outs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("RECONNET_SQS_INTERLEAVE", flag)
+        eng, x0 = _engine(ns)
+        eng.set_image(x0)
+        for _ in range(2):
+            eng.run_pass(beta=1e-3, eps=1e-4, lb=0.0)
+        outs.append(eng.image())
+        eng.close()
+    err = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
+    assert err <= 1e-6, err

@@ -232,6 +232,23 @@ def test_ossqs_parity_small():
     assert rel_l2(out, ref) <= TOL
 
 
+def test_ossqs_parity_batch_of_slices():
+    # six slices under one engine: the interleaved 4-slice layout with a
+    # padded second group, against the oracle's pass of every slice
+    geo = small_fan(nx=64, nu=128, nview=96)
+    deg = fan_angles(geo.nview)
+    truth = np.concatenate([_img5(phantom(20 + s, 64, 64, 1.0, 1.0)) for s in range(6)])
+    b = orc.siddon_fan_fp(truth, deg, geo.oracle_geo(), geo.nu).astype(np.float32)
+    w = np.random.default_rng(3).uniform(0.5, 1.0, b.shape).astype(np.float32)
+    fp1 = orc.siddon_fan_fp(np.ones_like(truth[:1]), deg, geo.oracle_geo(), geo.nu)
+    D = orc.siddon_fan_bp((fp1 * w).astype(np.float32), deg, geo.oracle_geo(), (64, 64, 1)).astype(np.float32)
+    x0 = np.random.default_rng(4).uniform(0, 0.02, truth.shape).astype(np.float32)
+    out = recon_for(geo).cSiddonFanOSSQS4d(8, x0, b, D, w, lb=0.0, ub=1.0)
+    ref = orc.os_sqs(x0, b, D, w, deg, geo.oracle_geo(), 8, lb=0.0, ub=1.0)
+    for s in range(6):
+        assert rel_l2(out[s], ref[s]) <= TOL, s
+
+
 def test_sqs_normaliser_fp64_matches_oracle():
     import torch
     from reconnet_amd.sqs import OsSqsTV
