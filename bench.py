@@ -310,7 +310,8 @@ def main():
     # ---- roofline of the dominant kernel (HIP events on the SQS stream) ----
     npers = eng.npers
     bytes_proj = 4.0 * (geo.nx * geo.ny + geo.nu * npers) * ns
-    bytes_upd = 4.0 * 6 * geo.nx * geo.ny * ns   # x, g, D read, x written; transposed copy read + written
+    # update (x, g, D read, x written) + interleaved copies (x read, two 4-slice layouts written)
+    bytes_upd = 4.0 * 7 * geo.nx * geo.ny * ns
     steps_k = max(kt["steps"], 1)
     kernels = {
         "siddon_fan_bp": (kt["bp"], bytes_proj),

@@ -23,12 +23,18 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ uint16_t f2bf(float f)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> two bf16 (round to nearest even, NaN stays NaN) packed low | high << 16,
+// one hardware conversion (v_cvt_pk_bf16_f32) for the pair
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    const f32x2 f = {lo, hi};
+    const bf16x2 h = __builtin_convertvector(f, bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &h, sizeof(u));
+    return u;
 }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
@@ -45,9 +51,7 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
                                                  const float* __restrict__ b, uint16_t* __restrict__ out,
                                                  int B, int H, int W)
 {
-    __shared__ float sw[9 * 32 + 32];
-    for (int i = threadIdx.x; i < 9 * 32 + 32; i += blockDim.x) sw[i] = i < 288 ? w[i] : b[i - 288];
-    __syncthreads();
+    // weights and biases are read at wave-uniform addresses (scalar loads)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long npx = (long)B * H * W;
     if (p >= npx) return;
@@ -62,10 +66,10 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
     uint32_t packed[16];
     #pragma unroll
     for (int c = 0; c < 32; c += 2) {
-        float z0 = sw[288 + c], z1 = sw[288 + c + 1];
+        float z0 = b[c], z1 = b[c + 1];
         #pragma unroll
-        for (int t = 0; t < 9; ++t) { z0 = fmaf(sw[t * 32 + c], v[t], z0); z1 = fmaf(sw[t * 32 + c + 1], v[t], z1); }
-        packed[c / 2] = (uint32_t)f2bf(fmaxf(z0, 0.0f)) | ((uint32_t)f2bf(fmaxf(z1, 0.0f)) << 16);
+        for (int t = 0; t < 9; ++t) { z0 = fmaf(w[t * 32 + c], v[t], z0); z1 = fmaf(w[t * 32 + c + 1], v[t], z1); }
+        packed[c / 2] = pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f));
     }
     uint4* o = reinterpret_cast<uint4*>(out + p * 32);
     #pragma unroll
@@ -105,8 +109,8 @@ __device__ __forceinline__ void conv_mid_store(const f32x4 (&acc)[2], const floa
             for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
         }
         uint2 ov;
-        ov.x = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
-        ov.y = (uint32_t)f2bf(z[2]) | ((uint32_t)f2bf(z[3]) << 16);
+        ov.x = pack_bf16(z[0], z[1]);
+        ov.y = pack_bf16(z[2], z[3]);
         *reinterpret_cast<uint2*>(out + pbase + c0) = ov;
     }
 }
@@ -171,37 +175,71 @@ __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ i
 
 // ---------------------------------------------------------------------------
 // layer 5: y = xres + b + sum_t sum_ci a4[p + t][ci] w5[t][ci]   (fp32 out)
+//
+// On MFMA with the layout and rolling row walk of k_conv_mid, one 16-channel
+// output tile of which two rows are used: the FP32 weights are split into
+// w5 = hi + lo with hi = bf16(w5) and lo = bf16(w5 - hi), held as output
+// channels 0 and 1 of the A fragment, so y = xres + b + (acc[0] + acc[1])
+// keeps the weights to about 16 significant bits.
 // ---------------------------------------------------------------------------
+template <int L>
 __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a, const float* __restrict__ w,
                                                   float b5, const float* __restrict__ xres, float* __restrict__ y,
-                                                  int B, int H, int W)
+                                                  int H, int W)
 {
-    __shared__ float sw[288];
-    for (int i = threadIdx.x; i < 288; i += blockDim.x) sw[i] = w[i];
-    __syncthreads();
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long npx = (long)B * H * W;
-    if (p >= npx) return;
-    const int xw = (int)(p % W), yh = (int)((p / W) % H);
-    const long base = p - (long)yh * W - xw;
-    float s = b5;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const int xpix = (blockIdx.x * 4 + wv) * 16 + r16;
+    const int y0 = blockIdx.y * L;
+    const long img = (long)blockIdx.z * H * W;
+    // A[m = co][k = ci]: row 0 = hi, row 1 = lo, rows 2..15 = 0
+    bf16x8 wa[9];
+    #pragma unroll
     for (int t = 0; t < 9; ++t) {
-        const int yy = yh + t / 3 - 1, xx = xw + t % 3 - 1;
-        if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
-        const uint4* src = reinterpret_cast<const uint4*>(a + (base + (long)yy * W + xx) * 32);
+        uint32_t u[4];
         #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint4 v = src[q];
-            const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+        for (int e = 0; e < 4; ++e) {
+            float v[2];
             #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int c = 8 * q + 2 * e;
-                s = fmaf(sw[t * 32 + c], bf2f((uint16_t)(u[e] & 0xffff)), s);
-                s = fmaf(sw[t * 32 + c + 1], bf2f((uint16_t)(u[e] >> 16)), s);
+            for (int h = 0; h < 2; ++h) {
+                const float wf = w[t * 32 + 8 * g4 + 2 * e + h];
+                const float hi = (float)(__bf16)wf;
+                v[h] = r16 == 0 ? wf : (r16 == 1 ? wf - hi : 0.0f);
             }
+            u[e] = pack_bf16(v[0], v[1]);
         }
+        wa[t] = as_bf16x8(make_uint4(u[0], u[1], u[2], u[3]));
     }
-    y[p] = (xres ? xres[p] : 0.0f) + s;
+    f32x4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0}, a2;
+    #pragma unroll 3
+    for (int ir = 0; ir < L + 2; ++ir) {
+        const int yy = y0 + ir - 1;
+        bf16x8 f[3];
+        #pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const int xx = xpix + dx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *reinterpret_cast<const uint4*>(a + ((img + (long)yy * W + xx) * 32 + 8 * g4));
+            f[dx] = as_bf16x8(v);
+        }
+        a2 = f32x4{0, 0, 0, 0};
+        #pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx], f[dx], a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx], f[dx], a1, 0, 0, 0);
+            a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx], f[dx], a2, 0, 0, 0);
+        }
+        // D[m = co][n = pixel]: lanes 0..15 (g4 = 0) hold co 0..3 of pixel r16
+        const int yo = yy - 1;
+        if (ir >= 2 && yo < H && xpix < W && g4 == 0) {
+            const long p = img + (long)yo * W + xpix;
+            y[p] = (xres ? xres[p] : 0.0f) + (b5 + (a0[0] + a0[1]));
+        }
+        a0 = a1;
+        a1 = a2;
+    }
 }
 
 // layer-5 data gradient: dz4[p][ci] = (a4[p][ci] > 0) * sum_t dy[p - off_t] w5[t][ci]
@@ -243,7 +281,7 @@ __global__ __launch_bounds__(256) void k_out_dgrad(const float* __restrict__ dy,
                 const uint16_t mk = h ? (uint16_t)(mu[e] >> 16) : (uint16_t)(mu[e] & 0xffff);
                 z[h] = bf2f(mk) > 0.0f ? s : 0.0f;
             }
-            r[e] = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+            r[e] = pack_bf16(z[0], z[1]);
         }
         o[q] = make_uint4(r[0], r[1], r[2], r[3]);
     }
@@ -453,8 +491,10 @@ void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const u
 void conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, float* y, int B, int H, int W,
               hipStream_t st)
 {
-    const long npx = (long)B * H * W;
-    hipLaunchKernelGGL(k_conv_out, dim3(cdiv(npx, 256)), dim3(256), 0, st, a4, w5, b5, xres, y, B, H, W);
+    RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "conv_out: bad batch or image size");
+    constexpr int kRows = 16;
+    const dim3 grd((unsigned)cdiv(W, 64), (unsigned)cdiv(H, kRows), (unsigned)B);
+    hipLaunchKernelGGL(k_conv_out<kRows>, grd, dim3(256), 0, st, a4, w5, b5, xres, y, H, W);
     check_launch("conv_out");
 }
 

@@ -99,13 +99,20 @@ void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int 
 // ---- interleaved ("quad") layout: groups of 4 slices stored as one float4 per
 // cell (slice 4 q + c in component c; slices past nslices are zero) ----
 // Siddon FP + weighted residual on quad images Qyx [q][ny][nx], Qxy [q][nx][ny];
-// meas, w, res: quad sinograms [q][view][nu] (res over the subset's views)
+// meas, w: quad sinograms [q][view][nu]; view view0 + i goes to row res_voff + i
+// of res [q][res_vstride][nu]
 void launch_siddon_fan_fp_residual_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res_q,
                                      int nslices, int view0, int nviews, const float4* meas_q, const float4* w_q,
-                                     hipStream_t st);
+                                     int res_vstride, int res_voff, hipStream_t st);
+// out[b][j][:] = in[b][perm[j]][:] for a batch of (rows x cols) matrices (perm: device, rows entries)
+void launch_permute_rows(const float* in, float* out, const int* perm, int rows, int cols, int batch,
+                         hipStream_t st);
 // Siddon BP gather of a quad subset sinogram [q][view][nu] into plain images [s][nx][ny]
 void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, int nslices, int view0,
                             int nviews, hipStream_t st);
+// reference-parity DD BP of a quad pre-weighted sinogram [q][view][nu] into plain images
+void launch_dd_fan_bp_q(const FanK& g, const float4* wprj_q, float* img_xy, int nslices, bool fbp,
+                        hipStream_t st);
 // plain [s][n1][n2] -> quads [q][n1][n2] and (if qtr) the transposed quads [q][n2][n1]
 void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, hipStream_t st);
 

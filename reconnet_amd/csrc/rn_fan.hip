@@ -128,7 +128,8 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* _
                                                          const float4* __restrict__ Qxy, float4* __restrict__ res,
                                                          int view0, int nviews, int nquads,
                                                          const float4* __restrict__ meas,
-                                                         const float4* __restrict__ wgt, int meas_nviews)
+                                                         const float4* __restrict__ wgt, int meas_nviews,
+                                                         int res_vstride, int res_voff)
 {
     int tile, vi, sg;
     if (XCD > 0) {
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* _
         if (k >= nq) break;
         const long mi = ((long)(q0 + k) * meas_nviews + view) * g.nu + iu;
         const float4 m = meas[mi], w = wgt[mi];
-        res[((long)(q0 + k) * nviews + vi) * g.nu + iu] =
+        res[((long)(q0 + k) * res_vstride + res_voff + vi) * g.nu + iu] =
             make_float4(w.x * (acc[k].x - m.x), w.y * (acc[k].y - m.y), w.z * (acc[k].z - m.z),
                         w.w * (acc[k].w - m.w));
     }
@@ -223,6 +224,39 @@ __global__ void k_dd_preweight(FanK g, const float* __restrict__ prj, float* __r
 // atan, average of the (pre-weighted) detector row over the footprint.  One
 // lane per pixel of S consecutive slices (grid z counts groups of S): the
 // footprint and the FBP weight are computed once for the S slices.
+// The pixel's S slices (value type T: float, or float4 for 4 interleaved
+// slices) over every view: rows[q] are the wave-uniform bases of the S
+// pre-weighted [view][nu] sinograms.
+template <class T, int S>
+__device__ __forceinline__ void dd_bp_pixel(const FanK& g, const T* const (&rows)[S], int ix, int iy, int fbp,
+                                            T (&acc)[S])
+{
+    const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
+    const float inv_da = 1.0f / g.da;
+    const float boff = 0.5f * g.nu - g.off_a;
+    T val[S];
+    #pragma unroll
+    for (int q = 0; q < S; ++q) acc[q] = vzero<T>();
+    for (int view = 0; view < g.nview; ++view) {
+        const float ct = g.vcos[view], st = g.vsin[view];
+        const float rx = x * ct + y * st, ry = -x * st + y * ct;
+        float ex, ey;   // edge offset (rotated frame) of +-half a pixel
+        if (g.vcls[view] == 0) { ex = 0.5f * g.dx * ct; ey = -0.5f * g.dx * st; }
+        else                   { ex = 0.5f * g.dy * st; ey = 0.5f * g.dy * ct; }
+        const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
+        const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
+        seg_avg_rows<T, S>(rows, (unsigned)(view * g.nu) * (unsigned)sizeof(T), g.nu, fminf(b1, b2),
+                           fmaxf(b1, b2), val);
+        float fw = 1.0f;
+        if (fbp) {
+            const float d = g.dso + ry;
+            fw = g.dso * g.dso / (rx * rx + d * d);
+        }
+        #pragma unroll
+        for (int q = 0; q < S; ++q) vadd(acc[q], fbp ? vscale(val[q], fw) : val[q]);
+    }
+}
+
 template <int BX, int BY, int S>
 __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp(FanK g, const float* __restrict__ wprj,
                                                    float* __restrict__ img, int ns, int fbp)
@@ -232,38 +266,46 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp(FanK g, const float* __re
     const int s0 = blockIdx.z * S;
     const int nsl = min(S, ns - s0);
     if (ix >= g.nx || iy >= g.ny) return;
-    const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
-    const float inv_da = 1.0f / g.da;
-    const float boff = 0.5f * g.nu - g.off_a;
     const long sstride = (long)g.nview * g.nu;
     const float* rows[S];   // wave-uniform slice bases
-    float acc[S], val[S];
     #pragma unroll
-    for (int q = 0; q < S; ++q) {
-        rows[q] = wprj + (long)(s0 + min(q, nsl - 1)) * sstride;
-        acc[q] = 0.0f;
-    }
-    for (int view = 0; view < g.nview; ++view) {
-        const float ct = g.vcos[view], st = g.vsin[view];
-        const float rx = x * ct + y * st, ry = -x * st + y * ct;
-        float ex, ey;   // edge offset (rotated frame) of +-half a pixel
-        if (g.vcls[view] == 0) { ex = 0.5f * g.dx * ct; ey = -0.5f * g.dx * st; }
-        else                   { ex = 0.5f * g.dy * st; ey = 0.5f * g.dy * ct; }
-        const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
-        const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
-        seg_avg_rows<float, S>(rows, (unsigned)(view * g.nu) * 4u, g.nu, fminf(b1, b2), fmaxf(b1, b2), val);
-        float fw = 1.0f;
-        if (fbp) {
-            const float d = g.dso + ry;
-            fw = g.dso * g.dso / (rx * rx + d * d);
-        }
-        #pragma unroll
-        for (int q = 0; q < S; ++q) acc[q] += fbp ? val[q] * fw : val[q];
-    }
+    for (int q = 0; q < S; ++q) rows[q] = wprj + (long)(s0 + min(q, nsl - 1)) * sstride;
+    float acc[S];
+    dd_bp_pixel<float, S>(g, rows, ix, iy, fbp, acc);
     #pragma unroll
     for (int q = 0; q < S; ++q) {
         if (q >= nsl) break;
         img[(long)(s0 + q) * g.nx * g.ny + (long)ix * g.ny + iy] = acc[q];
+    }
+}
+
+// The same on 4-slice interleaved sinograms [q][view][nu] (float4): G groups
+// per lane; writes the plain images of the real slices.
+template <int BX, int BY, int G>
+__global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_q(FanK g, const float4* __restrict__ wprj,
+                                                     float* __restrict__ img, int ns, int fbp)
+{
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
+    const int nquads = (ns + 3) / 4;
+    const int q0 = blockIdx.z * G;
+    const int nq = min(G, nquads - q0);
+    if (ix >= g.nx || iy >= g.ny) return;
+    const long sstride = (long)g.nview * g.nu;
+    const float4* rows[G];
+    #pragma unroll
+    for (int k = 0; k < G; ++k) rows[k] = wprj + (long)(q0 + min(k, nq - 1)) * sstride;
+    float4 acc[G];
+    dd_bp_pixel<float4, G>(g, rows, ix, iy, fbp, acc);
+    const long plane = (long)g.nx * g.ny, pix = (long)ix * g.ny + iy;
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= nq) break;
+        const int s = 4 * (q0 + k);
+        const float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+        #pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (s + c < ns) img[(long)(s + c) * plane + pix] = v[c];
     }
 }
 
@@ -328,6 +370,15 @@ __global__ __launch_bounds__(256) void k_pack_quads(const float* __restrict__ in
         const int j = j0 + r, i = i0 + tx;
         if (i < n1 && j < n2) q_tr[(long)q * plane + (long)j * n1 + i] = tile[tx][r];
     }
+}
+
+__global__ __launch_bounds__(256) void k_permute_rows(const float* __restrict__ in, float* __restrict__ out,
+                                                      const int* __restrict__ perm, int rows, int cols)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, b = blockIdx.z;
+    if (c >= cols) return;
+    out[((long)b * rows + j) * cols + c] = in[((long)b * rows + perm[j]) * cols + c];
 }
 
 __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ in, float* __restrict__ out,
@@ -591,6 +642,17 @@ const DdBpFn kDdBp[] = {go_dd_bp<16, 16, 1>, go_dd_bp<8, 32, 1>, go_dd_bp<32, 8,
                        go_dd_bp<8, 8, 16>};
 constexpr int kNDdBp = sizeof(kDdBp) / sizeof(kDdBp[0]);
 
+template <int BX, int BY, int G>
+void go_dd_bp_q(const FanK& g, const float4* wprj, float* img, int ns, int fbp, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dd_fan_bp_q<BX, BY, G>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
+                       dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
+}
+using DdBpQFn = void (*)(const FanK&, const float4*, float*, int, int, hipStream_t);
+const DdBpQFn kDdBpQ[] = {go_dd_bp_q<8, 8, 1>, go_dd_bp_q<16, 16, 1>, go_dd_bp_q<8, 8, 2>, go_dd_bp_q<16, 16, 2>,
+                          go_dd_bp_q<8, 8, 4>};
+constexpr int kNDdBpQ = sizeof(kDdBpQ) / sizeof(kDdBpQ[0]);
+
 // The Siddon and DD gather kernels address one slice's image plane, ray table
 // and sinogram rows with 32-bit byte offsets.
 void require_offsets32(const FanK& g, int nviews, size_t elem)
@@ -603,16 +665,16 @@ void require_offsets32(const FanK& g, int nviews, size_t elem)
 
 template <int BLK, int XCD, int G>
 void go_siddon_fp_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res, int nquads, int view0,
-                    int nviews, const float4* meas, const float4* w, hipStream_t st)
+                    int nviews, const float4* meas, const float4* w, int res_vstride, int res_voff, hipStream_t st)
 {
     const int ngroups = cdiv(nquads, G);
     const long nblk = (long)cdiv(g.nu, BLK) * nviews * ngroups;
     if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
         hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, XCD, G>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Qyx, Qxy,
-                           res, view0, nviews, nquads, meas, w, g.nview);
+                           res, view0, nviews, nquads, meas, w, g.nview, res_vstride, res_voff);
     } else {
         hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, 0, G>), dim3(cdiv(g.nu, BLK), nviews, ngroups), dim3(BLK), 0, st,
-                           g, Qyx, Qxy, res, view0, nviews, nquads, meas, w, g.nview);
+                           g, Qyx, Qxy, res, view0, nviews, nquads, meas, w, g.nview, res_vstride, res_voff);
     }
 }
 
@@ -624,7 +686,7 @@ void go_siddon_bp_q(const FanK& g, const float4* prj, float* img, int ns, int vi
 }
 
 using SidFpQFn = void (*)(const FanK&, const float4*, const float4*, float4*, int, int, int, const float4*,
-                          const float4*, hipStream_t);
+                          const float4*, int, int, hipStream_t);
 using SidBpQFn = void (*)(const FanK&, const float4*, float*, int, int, int, hipStream_t);
 // G: groups of 4 slices per lane
 const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<128, kXcds, 1>, go_siddon_fp_q<64, kXcds, 1>,
@@ -633,7 +695,8 @@ const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<128, kXcds, 1>, go_siddon_fp_q<64, kX
                             go_siddon_fp_q<128, 0, 2>};
 const SidBpQFn kSidBpQ[] = {go_siddon_bp_q<8, 8, 2, 2>,   go_siddon_bp_q<8, 8, 2, 1>, go_siddon_bp_q<16, 16, 2, 1>,
                             go_siddon_bp_q<16, 16, 2, 2>, go_siddon_bp_q<8, 8, 2, 4>, go_siddon_bp_q<16, 16, 2, 4>,
-                            go_siddon_bp_q<8, 8, 3, 2>,   go_siddon_bp_q<8, 8, 3, 4>};
+                            go_siddon_bp_q<8, 8, 3, 2>,   go_siddon_bp_q<8, 8, 3, 4>, go_siddon_bp_q<8, 8, 2, 8>,
+                            go_siddon_bp_q<16, 16, 2, 8>};
 constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
 constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);
 
@@ -676,15 +739,16 @@ void launch_siddon_fan_fp_residual(const FanK& g, const float* Pyx, const float*
 
 void launch_siddon_fan_fp_residual_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res_q,
                                      int nslices, int view0, int nviews, const float4* meas_q, const float4* w_q,
-                                     hipStream_t st)
+                                     int res_vstride, int res_voff, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    require_offsets32(g, nviews, sizeof(float4));
+    RN_REQUIRE(res_voff >= 0 && res_voff + nviews <= res_vstride, "residual view range out of its rows");
+    require_offsets32(g, res_vstride, sizeof(float4));
     const int nq = cdiv(nslices, 4);
     const int c = tune_choose({TK_SIDDON_FP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidFpQ, [&](int i) {
-        kSidFpQ[i](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, st);
+        kSidFpQ[i](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, res_vstride, res_voff, st);
     }, st, true);
-    kSidFpQ[c](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, st);
+    kSidFpQ[c](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, res_vstride, res_voff, st);
     check_launch("siddon_fan_fp_residual_q");
 }
 
@@ -763,6 +827,16 @@ void launch_dd_fan_bp(const FanK& g, const float* wprj_vu, float* img_xy, int ns
     check_launch("dd_fan_bp");
 }
 
+void launch_dd_fan_bp_q(const FanK& g, const float4* wprj_q, float* img_xy, int nslices, bool fbp, hipStream_t st)
+{
+    if (nslices <= 0) return;
+    require_offsets32(g, g.nview, sizeof(float4));
+    const int c = tune_choose({fbp ? TK_DD_FBP_Q : TK_DD_BP_Q, g.nx, g.ny, g.nu, g.nview, nslices}, kNDdBpQ,
+                              [&](int i) { kDdBpQ[i](g, wprj_q, img_xy, nslices, fbp ? 1 : 0, st); }, st, true);
+    kDdBpQ[c](g, wprj_q, img_xy, nslices, fbp ? 1 : 0, st);
+    check_launch("dd_fan_bp_q");
+}
+
 void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int nslices, hipStream_t st)
 {
     if (nslices <= 0) return;
@@ -777,6 +851,15 @@ void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, in
     dim3 grd(cdiv(n2, 32), cdiv(n1, 32), cdiv(nslices, 4));
     hipLaunchKernelGGL(k_pack_quads, grd, dim3(256), 0, st, in, q_same, q_tr, n1, n2, nslices);
     check_launch("pack_quads");
+}
+
+void launch_permute_rows(const float* in, float* out, const int* perm, int rows, int cols, int batch,
+                         hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0 || batch <= 0) return;
+    hipLaunchKernelGGL(k_permute_rows, dim3(cdiv(cols, 256), rows, batch), dim3(256), 0, st, in, out, perm, rows,
+                       cols);
+    check_launch("permute_rows");
 }
 
 void launch_transpose(const float* in, float* out, int rows, int cols, int batch, hipStream_t st)

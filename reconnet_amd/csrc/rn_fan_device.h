@@ -37,6 +37,8 @@ __device__ __forceinline__ void vmadd(float4& a, float4 v, float w)
 __device__ __forceinline__ float vscale(float v, float f) { return v * f; }
 __device__ __forceinline__ double vscale(double v, float f) { return v * (double)f; }
 __device__ __forceinline__ float4 vscale(float4 v, float f) { return make_float4(v.x * f, v.y * f, v.z * f, v.w * f); }
+__device__ __forceinline__ float vdiv(float v, float d) { return v / d; }
+__device__ __forceinline__ float4 vdiv(float4 v, float d) { return make_float4(v.x / d, v.y / d, v.z / d, v.w / d); }
 
 // ---------------------------------------------------------------------------
 // geometry helpers
@@ -98,7 +100,7 @@ __device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned
     const float w = hi - lo;
     const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
     #pragma unroll
-    for (int q = 0; q < S; ++q) val[q] = T(0);
+    for (int q = 0; q < S; ++q) val[q] = vzero<T>();
     if (!(h > l)) {
         if (w > 0.0f || !(lo >= 0.0f && lo < (float)n)) return;
         const unsigned boff = rboff + (unsigned)(int)lo * (unsigned)sizeof(T);
@@ -110,13 +112,12 @@ __device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned
     const int m1 = min((int)h, n - 1);
     for (int m = m0; m <= m1; ++m) {
         const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
-        const T wt = (T)(ph - pl);
         const unsigned boff = rboff + (unsigned)m * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) val[q] += ld_at(rows[q], boff) * wt;
+        for (int q = 0; q < S; ++q) vmadd(val[q], ld_at(rows[q], boff), ph - pl);
     }
     #pragma unroll
-    for (int q = 0; q < S; ++q) val[q] = val[q] / (T)w;
+    for (int q = 0; q < S; ++q) val[q] = vdiv(val[q], w);
 }
 
 // Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).
@@ -211,15 +212,14 @@ __device__ __forceinline__ T siddon_line_sum(const T* __restrict__ line, int n, 
     return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : vzero<T>();
 }
 
-// siddon_line_sum for hi - lo < 2 (at most three cells) without branches:
-// three independent loads (indices clamped into the line) and the weights
-// max(0, min(h, m + 1) - max(l, m)) of cells m0 .. m0 + 2, which vanish
-// outside [l, h]; a zero-width interval selects the cell holding lo.  The
-// cells and weights (Line3) depend on the geometry only, so one Line3 serves
-// every slice of a batch.
+// A line of at most three cells for hi - lo < 2: the cells m0 .. m0 + 2 (indices
+// clamped into the line) and their shares max(0, min(h, m + 1) - max(l, m))
+// / (hi - lo), which vanish outside [l, h]; a zero-width interval gives the
+// cell holding lo the whole share.  The cells and shares depend on the
+// geometry only, so one Line3 serves every slice of a batch.
 struct Line3 {
     unsigned i0, i1, i2;
-    float w0, w1, w2, inv;
+    float w0, w1, w2;
 };
 
 __device__ __forceinline__ Line3 line3(int n, float lo, float hi)
@@ -229,13 +229,13 @@ __device__ __forceinline__ Line3 line3(int n, float lo, float hi)
     const float l = fminf(fmaxf(lo, 0.0f), fn), h = fminf(hi, fn);
     const int m0 = (int)l;                 // l in [m0, m0 + 1) unless l == n (then every weight is 0)
     const float f0 = (float)m0;
-    Line3 c;
-    c.w0 = fmaxf(fminf(h, f0 + 1.0f) - l, 0.0f);
-    c.w1 = fmaxf(fminf(h, f0 + 2.0f) - (f0 + 1.0f), 0.0f);
-    c.w2 = fmaxf(fminf(h, f0 + 3.0f) - (f0 + 2.0f), 0.0f);
     const bool pos = w > 0.0f;
-    c.w0 = pos ? c.w0 : ((lo >= 0.0f && lo < fn) ? 1.0f : 0.0f);
-    c.inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
+    const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
+    Line3 c;
+    const float o0 = fmaxf(fminf(h, f0 + 1.0f) - l, 0.0f);
+    c.w0 = (pos ? o0 : ((lo >= 0.0f && lo < fn) ? 1.0f : 0.0f)) * inv;
+    c.w1 = fmaxf(fminf(h, f0 + 2.0f) - (f0 + 1.0f), 0.0f) * inv;
+    c.w2 = fmaxf(fminf(h, f0 + 3.0f) - (f0 + 2.0f), 0.0f) * inv;
     const unsigned last = (unsigned)(n - 1);
     c.i0 = min((unsigned)m0, last);
     c.i1 = min((unsigned)m0 + 1u, last);
@@ -243,38 +243,37 @@ __device__ __forceinline__ Line3 line3(int n, float lo, float hi)
     return c;
 }
 
-template <class T>
-__device__ __forceinline__ T line3_sum(const T* __restrict__ line, const Line3& c)
+// acc += v0 w0 + v1 w1 [+ v2 w2] per value component (fused multiply-adds)
+__device__ __forceinline__ void lmadd(float& a, float v0, float v1, float v2, const Line3& c, bool three)
 {
-    return (line[c.i0] * (T)c.w0 + line[c.i1] * (T)c.w1 + line[c.i2] * (T)c.w2) * (T)c.inv;
+    a = fmaf(v0, c.w0, a);
+    a = fmaf(v1, c.w1, a);
+    if (three) a = fmaf(v2, c.w2, a);
+}
+__device__ __forceinline__ void lmadd(double& a, double v0, double v1, double v2, const Line3& c, bool three)
+{
+    a = fma(v0, (double)c.w0, a);
+    a = fma(v1, (double)c.w1, a);
+    if (three) a = fma(v2, (double)c.w2, a);
+}
+__device__ __forceinline__ void lmadd(float4& a, float4 v0, float4 v1, float4 v2, const Line3& c, bool three)
+{
+    lmadd(a.x, v0.x, v1.x, v2.x, c, three);
+    lmadd(a.y, v0.y, v1.y, v2.y, c, three);
+    lmadd(a.z, v0.z, v1.z, v2.z, c, three);
+    lmadd(a.w, v0.w, v1.w, v2.w, c, three);
 }
 
-// (v0 w0 + v1 w1 [+ v2 w2]) / width of one line, per value component
-__device__ __forceinline__ float lcomb(float v0, float v1, float v2, const Line3& c, bool three)
-{
-    return three ? (v0 * c.w0 + v1 * c.w1 + v2 * c.w2) * c.inv : (v0 * c.w0 + v1 * c.w1) * c.inv;
-}
-__device__ __forceinline__ double lcomb(double v0, double v1, double v2, const Line3& c, bool three)
-{
-    return three ? (v0 * (double)c.w0 + v1 * (double)c.w1 + v2 * (double)c.w2) * (double)c.inv
-                 : (v0 * (double)c.w0 + v1 * (double)c.w1) * (double)c.inv;
-}
-__device__ __forceinline__ float4 lcomb(float4 v0, float4 v1, float4 v2, const Line3& c, bool three)
-{
-    return make_float4(lcomb(v0.x, v1.x, v2.x, c, three), lcomb(v0.y, v1.y, v2.y, c, three),
-                       lcomb(v0.z, v1.z, v2.z, c, three), lcomb(v0.w, v1.w, v2.w, c, three));
-}
-
-// line3_sum of the line starting `lboff` bytes after a wave-uniform base,
-// over the first NC cells (NC = 2 when the interval is known to be narrower
-// than one cell: the third weight is then exactly 0 and its term is dropped)
+// acc += the line starting `lboff` bytes after a wave-uniform base, over the
+// first NC cells (NC = 2 when the interval is known to be narrower than one
+// cell: the third share is then exactly 0 and its term is dropped)
 template <class T, int NC>
-__device__ __forceinline__ T line_sum_at(const T* __restrict__ base, unsigned lboff, const Line3& c)
+__device__ __forceinline__ void line_madd_at(T& acc, const T* __restrict__ base, unsigned lboff, const Line3& c)
 {
     const T v0 = ld_at(base, lboff + c.i0 * (unsigned)sizeof(T));
     const T v1 = ld_at(base, lboff + c.i1 * (unsigned)sizeof(T));
     const T v2 = NC == 2 ? v1 : ld_at(base, lboff + c.i2 * (unsigned)sizeof(T));
-    return lcomb(v0, v1, v2, c, NC == 3);
+    lmadd(acc, v0, v1, v2, c, NC == 3);
 }
 
 // The narrow-ray line loop of siddon_fp_ray_slices: lines wj0 .. wj1 of the S
@@ -292,15 +291,9 @@ __device__ __forceinline__ void fp_narrow_lines(const T* const (&Pq)[S], int n, 
         const Line3 c = line3(n, fminf(ea, eb), fmaxf(ea, eb));
         const unsigned lboff = (unsigned)(j * n) * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) vadd(acc[q], line_sum_at<T, NC>(Pq[q], lboff, c));
+        for (int q = 0; q < S; ++q) line_madd_at<T, NC>(acc[q], Pq[q], lboff, c);
         ea = eb;
     }
-}
-
-template <class T>
-__device__ __forceinline__ T siddon_line_sum3(const T* __restrict__ line, int n, float lo, float hi)
-{
-    return line3_sum<T>(line, line3(n, lo, hi));
 }
 
 // wave-wide minimum / maximum (every lane of the wavefront must take part)

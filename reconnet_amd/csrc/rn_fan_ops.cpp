@@ -158,12 +158,25 @@ void fan_backward(const FanTables& t, FanBP kind, int nB, int nC, const float* p
     float* out = direct_out ? img_ref : w.rx.as<float>();
     switch (kind) {
     case FanBP::DD:
-        launch_dd_preweight(t.k, W, w.ry.as<float>(), ns, st);
-        launch_dd_fan_bp(t.k, w.ry.as<float>(), out, ns, false, st);
+    case FanBP::DDFBP: {
+        // the FBP variant backprojects the sinogram as given; the iterative
+        // one pre-weights it by the line length factor first
+        const bool fbp = kind == FanBP::DDFBP;
+        const float* P = W;
+        if (!fbp) {
+            launch_dd_preweight(t.k, W, w.ry.as<float>(), ns, st);
+            P = w.ry.as<float>();
+        }
+        if (ns >= 4) {
+            // groups of 4 slices interleaved: one float4 load per bin and group
+            w.quad.ensure((size_t)cdiv(ns, 4) * sino * sizeof(float4));
+            launch_pack_quads(P, w.quad.as<float4>(), nullptr, g.nview, g.nu, ns, st);
+            launch_dd_fan_bp_q(t.k, w.quad.as<float4>(), out, ns, fbp, st);
+        } else {
+            launch_dd_fan_bp(t.k, P, out, ns, fbp, st);
+        }
         break;
-    case FanBP::DDFBP:
-        launch_dd_fan_bp(t.k, W, out, ns, true, st);
-        break;
+    }
     case FanBP::DDAdjoint:
         launch_dd_fan_adj(t.k, W, out, ns, 0, g.nview, false, st);
         break;

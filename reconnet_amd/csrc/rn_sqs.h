@@ -56,6 +56,13 @@ private:
     // measurement, weights and the subset residual
     bool quads_ = false;
     DevBuf Qxy_, Qyx_, B4_, W4_, R4_;
+    // Within each subset the engine orders the views X class first (a view
+    // with |cos| > |sin| walks image rows, the others columns), so that the
+    // interleaved forward projection runs as one launch per class, each
+    // reading one image orientation.  perm_[j] = caller's view index of engine
+    // view j; sub_nx_[k] = number of X views of subset k.
+    std::vector<int> perm_, sub_nx_;
+    DevBuf perm_dev_;
     int cur_ = 0;
     bool have_w_ = false;
     bool prof_ = false;

@@ -13,6 +13,8 @@
 
 #include "rn_sqs.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 #include <cstring>
@@ -189,8 +191,27 @@ void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nview
     RN_REQUIRE(nslices > 0, "number of slices must be positive");
     RN_REQUIRE(nviewPerSubset > 0 && nSubsets > 0, "subset sizes must be positive");
     RN_REQUIRE(g.nz == 1 && g.nv == 1, "OS-SQS engine handles 2D slices (nz = nv = 1)");
-    if (!t_.matches(g, deg_host)) t_.build(g, deg_host, st);
     ns_ = nslices; npers_ = nviewPerSubset; nsub_ = nSubsets;
+    // X-class views first within each subset (stable); the same float32 class
+    // decision as the trig tables
+    perm_.resize(g.nview);
+    sub_nx_.assign(nSubsets, 0);
+    std::vector<float> deg(g.nview);
+    for (int k = 0, start = 0; k < nSubsets && start < g.nview; k++) {
+        const int end = std::min(start + nviewPerSubset, g.nview);
+        int j = start;
+        for (int v = start; v < end; v++)
+            if (std::fabs(cosf(deg_host[v])) > std::fabs(sinf(deg_host[v]))) perm_[j++] = v;
+        sub_nx_[k] = j - start;
+        for (int v = start; v < end; v++)
+            if (!(std::fabs(cosf(deg_host[v])) > std::fabs(sinf(deg_host[v])))) perm_[j++] = v;
+        start = end;
+    }
+    for (int j = 0; j < g.nview; j++) deg[j] = deg_host[perm_[j]];
+    if (!t_.matches(g, deg.data())) t_.build(g, deg.data(), st);
+    perm_dev_.ensure(g.nview * sizeof(int));
+    RN_HIP_CHECK(hipMemcpyAsync(perm_dev_.get(), perm_.data(), g.nview * sizeof(int), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipStreamSynchronize(st));
     const size_t plane = (size_t)g.nx * g.ny, sino = (size_t)g.nu * g.nview;
     B_.ensure(ns_ * sino * sizeof(float));
     W_.ensure(ns_ * sino * sizeof(float));
@@ -216,9 +237,16 @@ void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nview
 void OsSqs::set_data(const float* prj_ref, const float* w_ref, hipStream_t st)
 {
     const FanGeo& g = t_.geo;
-    launch_transpose(prj_ref, B_.as<float>(), g.nu, g.nview, ns_, st);
-    if (w_ref) launch_transpose(w_ref, W_.as<float>(), g.nu, g.nview, ns_, st);
-    else launch_fill(W_.as<float>(), 1.0f, (size_t)ns_ * g.nu * g.nview, st);
+    // view-major, then the engine's view order
+    DevBuf tmp((size_t)ns_ * g.nu * g.nview * sizeof(float));
+    launch_transpose(prj_ref, tmp.as<float>(), g.nu, g.nview, ns_, st);
+    launch_permute_rows(tmp.as<float>(), B_.as<float>(), perm_dev_.as<int>(), g.nview, g.nu, ns_, st);
+    if (w_ref) {
+        launch_transpose(w_ref, tmp.as<float>(), g.nu, g.nview, ns_, st);
+        launch_permute_rows(tmp.as<float>(), W_.as<float>(), perm_dev_.as<int>(), g.nview, g.nu, ns_, st);
+    } else {
+        launch_fill(W_.as<float>(), 1.0f, (size_t)ns_ * g.nu * g.nview, st);
+    }
     if (quads_) {
         launch_pack_quads(B_.as<float>(), B4_.as<float4>(), nullptr, g.nview, g.nu, ns_, st);
         launch_pack_quads(W_.as<float>(), W4_.as<float4>(), nullptr, g.nview, g.nu, ns_, st);
@@ -318,8 +346,15 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         float* Xn = X_[1 - cur_].as<float>();
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k], st));
         if (quads_) {
-            launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
-                                            start, cur, B4_.as<float4>(), W4_.as<float4>(), st);
+            // one launch per view class (each reads one image orientation)
+            const int nxv = sub_nx_[k];
+            if (nxv > 0)
+                launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
+                                                start, nxv, B4_.as<float4>(), W4_.as<float4>(), cur, 0, st);
+            if (nxv < cur)
+                launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
+                                                start + nxv, cur - nxv, B4_.as<float4>(), W4_.as<float4>(), cur, nxv,
+                                                st);
         } else {
             launch_siddon_fan_fp_residual(t_.k, Xyx_.as<float>(), X, R_.as<float>(), ns_, start, cur,
                                           B_.as<float>(), W_.as<float>(), st);

@@ -37,6 +37,8 @@ enum TuneKernel {
     TK_DD_FBP = 7,
     TK_SIDDON_FP_Q = 8,
     TK_SIDDON_BP_Q = 9,
+    TK_DD_BP_Q = 10,
+    TK_DD_FBP_Q = 11,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

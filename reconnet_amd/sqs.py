@@ -21,7 +21,7 @@ from .torch_ops import FanPlan, _ptr, _stream
 __all__ = ["OsSqsTV", "PassGraph", "fbp_initial_image"]
 
 
-def fbp_initial_image(plan, sino, filter_type=2, chunk=16, out=None):
+def fbp_initial_image(plan, sino, filter_type=2, chunk=32, out=None):
     """Non-negative FBP images [B, nx, ny] of sinograms [B, nu, nview] (natural
     view order): windowed ramp filter (2 = Hann) and the FBP-weighted DD
     backprojection, the usual starting image of the iterative methods."""

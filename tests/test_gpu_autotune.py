@@ -61,12 +61,14 @@ def test_every_launch_shape_agrees(ns):
         for (kernel, *_), (_, ms) in autotune().items():
             counts[kernel] = len(ms)
         assert counts.get(1, 0) >= 13 and counts.get(2, 0) >= 13
-        assert counts.get(5, 0) >= 8 and counts.get(7, 0) >= 8
+        # 4 or more slices take the DD BP on the interleaved 4-slice layout
+        dd_bp, dd_fbp = (("dd_bp", 5), ("dd_fbp", 7)) if ns < 4 else (("dd_bp_q", 10), ("dd_fbp_q", 11))
+        assert counts.get(dd_bp[1], 0) >= 5 and counts.get(dd_fbp[1], 0) >= 5, counts
         refs = {}
         for kname, kid, run in (("siddon_fp", 1, lambda: plan.forward(x, "siddon")),
                                 ("siddon_bp", 2, lambda: plan.backward(y, "siddon")),
-                                ("dd_bp", 5, lambda: plan.backward(y, "dd")),
-                                ("dd_fbp", 7, lambda: plan.backward(y, "dd_fbp"))):
+                                (dd_bp[0], dd_bp[1], lambda: plan.backward(y, "dd")),
+                                (dd_fbp[0], dd_fbp[1], lambda: plan.backward(y, "dd_fbp"))):
             for c in range(counts[kid]):
                 autotune_pin(kname, c)
                 out = run()
@@ -82,7 +84,7 @@ def test_every_launch_shape_agrees(ns):
         with pytest.raises(RuntimeError, match="out of range"):
             plan.forward(x, "siddon")
     finally:
-        for k in ("siddon_fp", "siddon_bp", "dd_bp", "dd_fbp"):
+        for k in ("siddon_fp", "siddon_bp", "dd_bp", "dd_fbp", "dd_bp_q", "dd_fbp_q"):
             autotune_pin(k, None)
         autotune(enable=True, reset=True)
 

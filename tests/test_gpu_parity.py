@@ -56,6 +56,19 @@ def test_dd_bp_config0(c0, type_projector):
     assert rel_l2(out, ref) <= TOL
 
 
+@pytest.mark.parametrize("type_projector", [0, 1])
+def test_dd_bp_batch_interleaved(type_projector):
+    # 5 slices in one call take the interleaved 4-slice DD BP (one full and
+    # one padded group): each slice against the oracle
+    geo = small_fan(nx=72, nu=130, nview=90)
+    deg = fan_angles(geo.nview)
+    sino = np.random.default_rng(8).uniform(0, 1, (5, geo.nu, geo.nview, 1, 1)).astype(np.float32)
+    out = recon_for(geo).cDDFanBackprojection4d(sino, type_projector=type_projector)
+    ref = orc.dd_fan_bp(sino, deg, geo.oracle_geo(), (geo.nx, geo.ny, 1), type_projector=type_projector)
+    for s in range(5):
+        assert rel_l2(out[s], ref[s]) <= TOL, s
+
+
 def test_siddon_bp_config0(c0):
     geo, _, sino, deg = c0
     out = recon_for(geo).cSiddonFanBackprojection4d(sino)
