@@ -41,6 +41,23 @@ __device__ __forceinline__ void tv_terms(const float* __restrict__ X, int ix, in
     var = (t0 + t1 + t2 + t3) / 2;
 }
 
+// tv_terms with reciprocal square roots in place of the square roots and
+// divisions (about 1 ulp each), for the SQS-TV iteration's update
+__device__ __forceinline__ void tv_terms_rsq(const float* __restrict__ X, int ix, int iy, int nx, int ny, float eps,
+                                             float& gtv, float& ntv)
+{
+    const long ind = (long)ix * ny + iy;
+    const float x = X[ind];
+    const float dx0 = ix == 0 ? x : x - X[ind - ny];
+    const float dx1 = ix == nx - 1 ? x : x - X[ind + ny];
+    const float dy0 = iy == 0 ? x : x - X[ind - 1];
+    const float dy1 = iy == ny - 1 ? x : x - X[ind + 1];
+    const float r0 = __builtin_amdgcn_rsqf(fmaf(dx0, dx0, eps)), r1 = __builtin_amdgcn_rsqf(fmaf(dx1, dx1, eps));
+    const float r2 = __builtin_amdgcn_rsqf(fmaf(dy0, dy0, eps)), r3 = __builtin_amdgcn_rsqf(fmaf(dy1, dy1, eps));
+    gtv = 0.5f * (dx0 * r0 + dx1 * r1 + dy0 * r2 + dy1 * r3);
+    ntv = r0 + r1 + r2 + r3;
+}
+
 // block-wide FP64 sum of up to two values; thread 0 writes the partials
 __device__ __forceinline__ void block_sum2(double a, double b, double* out)
 {
@@ -90,14 +107,63 @@ __global__ __launch_bounds__(256) void k_sqs_update(const float* __restrict__ X,
     float num = step * G[i];
     float den = D[i];
     if (beta2 != 0.0f) {
-        float g, n, var;
-        tv_terms(X + s * plane, ix, iy, nx, ny, eps, g, n, var);
+        float g, n;
+        tv_terms_rsq(X + s * plane, ix, iy, nx, ny, eps, g, n);
         num = fmaf(beta2, g, num);
         den = fmaf(beta2, n, den);
     }
     float xn = x - num / (den + 1e-6f);
     if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
     Xn[i] = xn;
+}
+
+// k_sqs_update for the interleaved layout, fused with the layout copies: a
+// 32 x 32 pixel tile of one group of 4 slices; each slice's new image goes to
+// Xn and, as the float4 of the 4 slices, to Qxy [q][nx][ny] and (through LDS)
+// to the transposed Qyx [q][ny][nx].  Slices past ns hold 0.
+__global__ __launch_bounds__(256) void k_sqs_update_q(const float* __restrict__ X, float* __restrict__ Xn,
+                                                      const float* __restrict__ G, const float* __restrict__ D,
+                                                      float4* __restrict__ Qxy, float4* __restrict__ Qyx, int nx,
+                                                      int ny, int ns, float step, float beta2, float eps, float lb,
+                                                      float ub, int crop)
+{
+    __shared__ float4 tile[32][33];
+    const int iy0 = blockIdx.x * 32, ix0 = blockIdx.y * 32, q = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const long plane = (long)nx * ny;
+    for (int r = ty; r < 32; r += 8) {
+        const int ix = ix0 + r, iy = iy0 + tx;
+        if (ix >= nx || iy >= ny) continue;
+        const long pix = (long)ix * ny + iy;
+        float v[4];
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int s = 4 * q + c;
+            v[c] = 0.0f;
+            if (s >= ns) continue;
+            const long i = s * plane + pix;
+            float num = step * G[i];
+            float den = D[i];
+            if (beta2 != 0.0f) {
+                float gt, nt;
+                tv_terms_rsq(X + s * plane, ix, iy, nx, ny, eps, gt, nt);
+                num = fmaf(beta2, gt, num);
+                den = fmaf(beta2, nt, den);
+            }
+            float xn = X[i] - num / (den + 1e-6f);
+            if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
+            Xn[i] = xn;
+            v[c] = xn;
+        }
+        const float4 f = make_float4(v[0], v[1], v[2], v[3]);
+        tile[r][tx] = f;
+        Qxy[(long)q * plane + pix] = f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int iy = iy0 + r, ix = ix0 + tx;
+        if (ix < nx && iy < ny) Qyx[(long)q * plane + (long)iy * nx + ix] = tile[tx][r];
+    }
 }
 
 // per-block FP64 partials of sum (b - a)^2 and sum b^2
@@ -363,11 +429,18 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         if (quads_) launch_siddon_fan_bp_q(t_.k, R4_.as<float4>(), G_.as<float>(), ns_, start, cur, st);
         else launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 2], st));
-        hipLaunchKernelGGL(k_sqs_update, grd, blk, 0, st, X, Xn, G_.as<float>(), D_.as<float>(), g.nx, g.ny,
-                           o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
-        check_launch("sqs_update");
-        if (quads_) launch_pack_quads(Xn, Qxy_.as<float4>(), Qyx_.as<float4>(), g.nx, g.ny, ns_, st);
-        else launch_transpose(Xn, Xyx_.as<float>(), g.nx, g.ny, ns_, st);
+        if (quads_) {
+            const dim3 gq(cdiv(g.ny, 32), cdiv(g.nx, 32), cdiv(ns_, 4));
+            hipLaunchKernelGGL(k_sqs_update_q, gq, dim3(256), 0, st, X, Xn, G_.as<float>(), D_.as<float>(),
+                               Qxy_.as<float4>(), Qyx_.as<float4>(), g.nx, g.ny, ns_, o.lambda * nsub_,
+                               2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
+            check_launch("sqs_update_q");
+        } else {
+            hipLaunchKernelGGL(k_sqs_update, grd, blk, 0, st, X, Xn, G_.as<float>(), D_.as<float>(), g.nx, g.ny,
+                               o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
+            check_launch("sqs_update");
+            launch_transpose(Xn, Xyx_.as<float>(), g.nx, g.ny, ns_, st);
+        }
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 3], st));
         cur_ = 1 - cur_;
         start += cur;

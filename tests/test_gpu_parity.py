@@ -262,6 +262,48 @@ def test_ossqs_parity_batch_of_slices():
         assert rel_l2(out[s], ref[s]) <= TOL, s
 
 
+@pytest.mark.parametrize("ns", [1, 5])
+def test_sqs_tv_pass_matches_oracle_composition(ns):
+    # one SQS-TV pass of the engine (no reference counterpart) against its
+    # definition composed from oracle pieces: per subset s,
+    #   x <- clamp(x - (M A_s'W_s(A_s x - b_s) + 2 beta g_TV(x)) / (D + 2 beta n_TV(x) + 1e-6), 0)
+    # with the reference TVSQSKernel's g_TV, n_TV (TotalVariation.cu:9-68)
+    import torch
+    from reconnet_amd.sqs import OsSqsTV
+
+    geo = small_fan(nx=48, nu=96, nview=40)
+    nsub, beta = 4, 1e-3
+    deg = fan_angles(geo.nview)
+    rng = np.random.default_rng(5)
+    b = rng.uniform(0, 2, (ns, geo.nu, geo.nview)).astype(np.float32)
+    w = rng.uniform(0.5, 1, (ns, geo.nu, geo.nview)).astype(np.float32)
+    x = rng.uniform(0, 0.05, (ns, geo.nx, geo.ny)).astype(np.float32)
+    D = rng.uniform(5, 10, (ns, geo.nx, geo.ny)).astype(np.float32)
+    eng = OsSqsTV(geo, nsub, ns, order=None)
+    eng.set_data(torch.from_numpy(b).cuda(), torch.from_numpy(w).cuda())
+    eng.set_norm(torch.from_numpy(D).cuda())
+    eng.set_image(torch.from_numpy(x).cuda())
+    eng.run_pass(beta=beta, eps=1e-6, lb=0.0)
+    out = eng.image().cpu().numpy()
+    eng.close()
+    npers = geo.nview // nsub
+    ref = x.astype(np.float64)
+    g5 = geo.oracle_geo()
+    for k in range(nsub):
+        v = slice(k * npers, (k + 1) * npers)
+        xi = ref.astype(np.float32)[:, :, :, None, None]
+        fp = orc.siddon_fan_fp(xi, deg[v], g5, geo.nu)[:, :, :, 0, 0]
+        r = (w[:, :, v] * (fp - b[:, :, v]))[:, :, :, None, None].astype(np.float32)
+        gd = orc.siddon_fan_bp(r, deg[v], g5, (geo.nx, geo.ny, 1))[:, :, :, 0, 0]
+        for s in range(ns):
+            gt, nt, _ = orc.tv_sqs(ref[s].astype(np.float32))
+            num = nsub * gd[s] + 2 * beta * gt
+            den = D[s] + 2 * beta * nt
+            ref[s] = np.maximum(ref[s] - num / (den + 1e-6), 0.0)
+    for s in range(ns):
+        assert rel_l2(out[s], ref[s]) <= TOL, s
+
+
 def test_sqs_normaliser_fp64_matches_oracle():
     import torch
     from reconnet_amd.sqs import OsSqsTV
