@@ -76,6 +76,53 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
     for (int q = 0; q < 4; ++q) o[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
 }
 
+// The three column-shifted B fragments (pixels xpix - 1, xpix, xpix + 1 of
+// row yy, 8 channels per lane) of a 16-pixel column segment.  Only the centre
+// fragment is loaded by every lane; the shifted ones are the same registers
+// moved one lane along each 16-lane DPP row (a row is one channel group), and
+// the one pixel each shift brings in from outside the segment is loaded by the
+// 4 edge lanes alone.  Pixels outside the image are zero.
+__device__ __forceinline__ uint32_t dpp_from_lower(uint32_t src, uint32_t edge)
+{
+    // lane i takes lane i - 1 of its row; lane 0 of the row keeps `edge`
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)src, 0x111, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t dpp_from_upper(uint32_t src, uint32_t edge)
+{
+    // lane i takes lane i + 1 of its row; lane 15 of the row keeps `edge`
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)src, 0x101, 0xf, 0xf, false);
+}
+
+// raw loads of one input row: the centre fragment and the two edge pixels
+struct RowLoad {
+    uint4 c, el, er;
+};
+
+__device__ __forceinline__ RowLoad row_load(const uint16_t* __restrict__ in, long img, int yy, int H, int W, int xpix,
+                                            int r16, int g4)
+{
+    RowLoad r;
+    r.c = r.el = r.er = make_uint4(0, 0, 0, 0);
+    if (yy >= 0 && yy < H) {
+        const uint16_t* row = in + (img + (long)yy * W) * 32 + 8 * g4;
+        if (xpix < W) r.c = *reinterpret_cast<const uint4*>(row + (long)xpix * 32);
+        if (r16 == 0 && xpix >= 1 && xpix - 1 < W) r.el = *reinterpret_cast<const uint4*>(row + (long)(xpix - 1) * 32);
+        if (r16 == 15 && xpix + 1 < W) r.er = *reinterpret_cast<const uint4*>(row + (long)(xpix + 1) * 32);
+    }
+    return r;
+}
+
+__device__ __forceinline__ void row_fragments(const RowLoad& r, bf16x8 (&f)[3])
+{
+    const uint4 lo = make_uint4(dpp_from_lower(r.c.x, r.el.x), dpp_from_lower(r.c.y, r.el.y),
+                                dpp_from_lower(r.c.z, r.el.z), dpp_from_lower(r.c.w, r.el.w));
+    const uint4 hi = make_uint4(dpp_from_upper(r.c.x, r.er.x), dpp_from_upper(r.c.y, r.er.y),
+                                dpp_from_upper(r.c.z, r.er.z), dpp_from_upper(r.c.w, r.er.w));
+    f[0] = as_bf16x8(lo);
+    f[1] = as_bf16x8(r.c);
+    f[2] = as_bf16x8(hi);
+}
+
 // ---------------------------------------------------------------------------
 // 32 -> 32 layer on MFMA.  wt: [9][32 co][32 ci] bf16 (tap t = 3*dy + dx).
 // mode 0: out = relu?(acc + bias); mode 1: out = acc * (mask > 0)
@@ -83,17 +130,19 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
 // One wave walks down a column of 16 pixels x 32 channels of one image over L
 // output rows; the four waves of a workgroup take adjacent 16-pixel columns,
 // so the grid is (W / 64, H / L, B) and needs no index division.  Each input
-// row is read once as three column-shifted B fragments (16 pixels x 32
-// channels, 16 bytes per lane) and feeds the three output rows that use it
+// row is read once (row_fragments: 16 pixels x 32 channels, 16 bytes per lane,
+// the column-shifted copies by DPP) and feeds the three output rows that use it
 // (taps dy = 2, 1, 0 of rows y - 1, y, y + 1) through three rolling
 // accumulators; the oldest is complete after its third row and is stored.
 // The weight fragments of all 9 taps and both 16-channel output tiles stay in
 // registers for the L rows.
 // ---------------------------------------------------------------------------
+template <int MODE>
 __device__ __forceinline__ void conv_mid_store(const f32x4 (&acc)[2], const float (&bv)[2][4],
                                                const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
-                                               long pbase, int g4, int relu, int mode)
+                                               long pbase, int g4, int relu)
 {
+    constexpr int mode = MODE;
     #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
         const int c0 = 16 * mt + 4 * g4;
@@ -115,11 +164,12 @@ __device__ __forceinline__ void conv_mid_store(const f32x4 (&acc)[2], const floa
     }
 }
 
-template <int L>
+template <int L, int MODE>
 __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
                                                   const float* __restrict__ bias, const uint16_t* __restrict__ mask,
-                                                  uint16_t* __restrict__ out, int H, int W, int relu, int mode)
+                                                  uint16_t* __restrict__ out, int H, int W, int relu)
 {
+    constexpr int mode = MODE;
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int r16 = lane & 15, g4 = lane >> 4;
@@ -143,18 +193,15 @@ __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ i
     // a0: output row yy - 1 (gets tap dy = 2 and completes), a1: row yy (dy = 1), a2: row yy + 1 (dy = 0)
     f32x4 a0[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, a1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     f32x4 a2[2];
+    // the next input row's loads are issued before this row's MFMAs
+    RowLoad nxt = row_load(in, img, y0 - 1, H, W, xpix, r16, g4);
     #pragma unroll 3
     for (int ir = 0; ir < L + 2; ++ir) {
         const int yy = y0 + ir - 1;                      // input row
+        const RowLoad cur = nxt;
+        if (ir + 1 < L + 2) nxt = row_load(in, img, yy + 1, H, W, xpix, r16, g4);
         bf16x8 f[3];
-        #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            const int xx = xpix + dx - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = *reinterpret_cast<const uint4*>(in + ((img + (long)yy * W + xx) * 32 + 8 * g4));
-            f[dx] = as_bf16x8(v);
-        }
+        row_fragments(cur, f);
         a2[0] = a2[1] = f32x4{0, 0, 0, 0};
         #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
@@ -166,8 +213,8 @@ __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ i
             }
         }
         const int yo = yy - 1;                           // the row a0 completes
-        if (ir >= 2 && yo < H && store) conv_mid_store(a0, bv, mask, out, (img + (long)yo * W + xpix) * 32, g4,
-                                                       relu, mode);
+        if (ir >= 2 && yo < H && store)
+            conv_mid_store<MODE>(a0, bv, mask, out, (img + (long)yo * W + xpix) * 32, g4, relu);
         a0[0] = a1[0]; a0[1] = a1[1];
         a1[0] = a2[0]; a1[1] = a2[1];
     }
@@ -215,15 +262,9 @@ __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a
     #pragma unroll 3
     for (int ir = 0; ir < L + 2; ++ir) {
         const int yy = y0 + ir - 1;
+        const RowLoad cur = row_load(a, img, yy, H, W, xpix, r16, g4);
         bf16x8 f[3];
-        #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            const int xx = xpix + dx - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = *reinterpret_cast<const uint4*>(a + ((img + (long)yy * W + xx) * 32 + 8 * g4));
-            f[dx] = as_bf16x8(v);
-        }
+        row_fragments(cur, f);
         a2 = f32x4{0, 0, 0, 0};
         #pragma unroll
         for (int dx = 0; dx < 3; ++dx) {
@@ -483,8 +524,10 @@ void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const u
     RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "conv_mid: bad batch or image size");
     constexpr int kRows = 16;
     const dim3 grd((unsigned)cdiv(W, 64), (unsigned)cdiv(H, kRows), (unsigned)B);
-    hipLaunchKernelGGL(k_conv_mid<kRows>, grd, dim3(256), 0, st, in, wt, bias, mask, out, H, W, relu ? 1 : 0,
-                       mask ? 1 : 0);
+    if (mask)
+        hipLaunchKernelGGL((k_conv_mid<kRows, 1>), grd, dim3(256), 0, st, in, wt, bias, mask, out, H, W, relu ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_conv_mid<kRows, 0>), grd, dim3(256), 0, st, in, wt, bias, mask, out, H, W, relu ? 1 : 0);
     check_launch("conv_mid");
 }
 
