@@ -2,7 +2,7 @@
 
 The compute path is libReconNet4D_no_tf.so (hand-written HIP kernels behind a
 C-ABI that matches the reference library's exports).  ``recon.ReconNet`` is
-the host-array driver with the reference Python class's interface;
+the host-array front end with the reference Python class's interface;
 ``torch_ops`` and ``sqs`` drive the same kernels on device tensors.
 """
 

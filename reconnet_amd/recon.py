@@ -1,4 +1,4 @@
-"""Host-array driver over the C-ABI, mirroring python/ReconNet_no_tf.py.
+"""Host-array front end over the C-ABI, mirroring python/ReconNet_no_tf.py.
 
 ``ReconNet`` keeps the reference class's attribute names, defaults
 (ReconNet_no_tf.py:27-47), parameter-file semantics (:49-70), angle recipe
