@@ -98,7 +98,14 @@ void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int 
 
 // ---- interleaved ("quad") layout: groups of 4 slices stored as one float4 per
 // cell (slice 4 q + c in component c; slices past nslices are zero) ----
-// Siddon FP + weighted residual on quad images Qyx [q][ny][nx], Qxy [q][nx][ny];
+// Images in this layout have padded lines: kQPadL zero cells before and
+// kQPadR after the n cells of every line (pitch qpitch(n)), so that the
+// forward projector reads 2 or 3 cells around any crossing without clamping
+// its cell interval.
+constexpr int kQPadL = 3, kQPadR = 3;
+__host__ __device__ constexpr int qpitch(int n) { return n + kQPadL + kQPadR; }
+// Siddon FP + weighted residual on padded quad images Qyx [q][ny][qpitch(nx)],
+// Qxy [q][nx][qpitch(ny)];
 // meas, w: quad sinograms [q][view][nu]; view view0 + i goes to row res_voff + i
 // of res [q][res_vstride][nu]
 void launch_siddon_fan_fp_residual_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res_q,
@@ -113,8 +120,11 @@ void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, i
 // reference-parity DD BP of a quad pre-weighted sinogram [q][view][nu] into plain images
 void launch_dd_fan_bp_q(const FanK& g, const float4* wprj_q, float* img_xy, int nslices, bool fbp,
                         hipStream_t st);
-// plain [s][n1][n2] -> quads [q][n1][n2] and (if qtr) the transposed quads [q][n2][n1]
-void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, hipStream_t st);
+// plain [s][n1][n2] -> quads [q][n1][n2] and (if q_tr) the transposed quads [q][n2][n1];
+// padded: lines of the padded layout (pitch qpitch(n2), resp. qpitch(n1); the
+// pad cells are left as they are)
+void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, bool padded,
+                       hipStream_t st);
 
 // ---- layout helpers ----
 // out[b][c][r] = in[b][r][c] for a batch of (rows x cols) matrices

@@ -15,6 +15,7 @@
 #include "rn_fan_device.h"
 #include "rn_tune.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -147,12 +148,11 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* _
     const int iu = tile * BLK + threadIdx.x;
     const bool active = iu < g.nu;
     const int view = view0 + vi;
-    const long plane = (long)g.nx * g.ny;
     const int q0 = sg * G;
     const int nq = min(G, nquads - q0);
     float4 acc[G];
-    siddon_fp_ray_slices<float4, G>(g, Qyx + q0 * plane, Qxy + q0 * plane, plane, nq, view,
-                                    active ? iu : g.nu - 1, active, acc);
+    siddon_fp_ray_quads<G>(g, Qyx + (long)q0 * g.ny * qpitch(g.nx), Qxy + (long)q0 * g.nx * qpitch(g.ny), nq, view,
+                           active ? iu : g.nu - 1, active, acc);
     if (!active) return;
     #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -342,9 +342,11 @@ __global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* _
 // ---------------------------------------------------------------------------
 
 // q_same[q][i][j] = (in[4q][i][j], .., in[4q+3][i][j]) (zero past nslices) and,
-// if q_tr, q_tr[q][j][i] = the same value; 32 x 32 tiles through LDS.
+// if q_tr, q_tr[q][j][i] = the same value; 32 x 32 tiles through LDS.  Output
+// lines have pitch p_same / p_tr and start off cells into the line.
 __global__ __launch_bounds__(256) void k_pack_quads(const float* __restrict__ in, float4* __restrict__ q_same,
-                                                    float4* __restrict__ q_tr, int n1, int n2, int ns)
+                                                    float4* __restrict__ q_tr, int n1, int n2, int ns, int p_same,
+                                                    int p_tr, int off)
 {
     __shared__ float4 tile[32][33];
     const int j0 = blockIdx.x * 32, i0 = blockIdx.y * 32, q = blockIdx.z;
@@ -361,14 +363,14 @@ __global__ __launch_bounds__(256) void k_pack_quads(const float* __restrict__ in
             }
             const float4 f = make_float4(v[0], v[1], v[2], v[3]);
             tile[r][tx] = f;
-            q_same[(long)q * plane + (long)i * n2 + j] = f;
+            q_same[(long)q * n1 * p_same + (long)i * p_same + off + j] = f;
         }
     }
     if (q_tr == nullptr) return;
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         const int j = j0 + r, i = i0 + tx;
-        if (i < n1 && j < n2) q_tr[(long)q * plane + (long)j * n1 + i] = tile[tx][r];
+        if (i < n1 && j < n2) q_tr[(long)q * n2 * p_tr + (long)j * p_tr + off + i] = tile[tx][r];
     }
 }
 
@@ -744,6 +746,8 @@ void launch_siddon_fan_fp_residual_q(const FanK& g, const float4* Qyx, const flo
     if (nslices <= 0 || nviews <= 0) return;
     RN_REQUIRE(res_voff >= 0 && res_voff + nviews <= res_vstride, "residual view range out of its rows");
     require_offsets32(g, res_vstride, sizeof(float4));
+    RN_REQUIRE((long)std::max(g.nx * qpitch(g.ny), g.ny * qpitch(g.nx)) * 16L < 0x7fffffffL,
+               "padded image plane of one slice group exceeds 2 GiB");
     const int nq = cdiv(nslices, 4);
     const int c = tune_choose({TK_SIDDON_FP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidFpQ, [&](int i) {
         kSidFpQ[i](g, Qyx, Qxy, res_q, nq, view0, nviews, meas_q, w_q, res_vstride, res_voff, st);
@@ -845,11 +849,13 @@ void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int 
     check_launch("pixel_fan_bp");
 }
 
-void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, hipStream_t st)
+void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, bool padded,
+                       hipStream_t st)
 {
     if (nslices <= 0 || n1 <= 0 || n2 <= 0) return;
     dim3 grd(cdiv(n2, 32), cdiv(n1, 32), cdiv(nslices, 4));
-    hipLaunchKernelGGL(k_pack_quads, grd, dim3(256), 0, st, in, q_same, q_tr, n1, n2, nslices);
+    hipLaunchKernelGGL(k_pack_quads, grd, dim3(256), 0, st, in, q_same, q_tr, n1, n2, nslices,
+                       padded ? qpitch(n2) : n2, padded ? qpitch(n1) : n1, padded ? kQPadL : 0);
     check_launch("pack_quads");
 }
 

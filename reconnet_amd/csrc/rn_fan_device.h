@@ -368,6 +368,117 @@ __device__ __forceinline__ void siddon_fp_ray_slices(const FanK& g, const T* __r
     for (int q = 0; q < S; ++q) acc[q] = vscale(acc[q], r.z);
 }
 
+// The padded-line variant of the narrow line loop for G interleaved slice
+// groups (float4 cells; padded line j of group k starts at Pq[k] + j * pitch
+// with kQPadL zero cells, so cell m is at + j * pitch + kQPadL + m, and ends
+// with kQPadR zero cells; every offset from Pq[k] is non-negative).
+// The crossing interval [lo, hi] of a line spans the cells m0 = floor(lo) ..
+// m0 + NC - 1; the cell index is only clamped into the padded line, so an
+// interval outside the image reads pad zeros, and the shares are
+// (min(hi, m0 + 1) - lo, ..) / (hi - lo) without clamping to [0, n].  A
+// zero-width interval gives cell m0 the whole share.
+template <int G, int NC>
+__device__ __forceinline__ void fp_quad_lines_padded(const float4* const (&Pq)[G], int n, int pitch, float4 r,
+                                                     float base, int wj0, int wj1, float4 (&acc)[G])
+{
+    float ea = fmaf(r.y, base + (float)wj0, r.x);
+    float fj = base + (float)(wj0 + 1);
+    #pragma unroll 2
+    for (int j = wj0; j <= wj1; ++j) {
+        const float eb = fmaf(r.y, fj, r.x);
+        fj += 1.0f;
+        const float lo = fminf(ea, eb), hi = fmaxf(ea, eb);
+        const float w = hi - lo;
+        const float f0 = floorf(lo);
+        // clamped into the padded line: a clamped index only reads pad cells
+        // (kQPadL, kQPadR >= NC), whatever the shares
+        const int m0 = (int)__builtin_amdgcn_fmed3f(f0, -(float)kQPadL, (float)(n + kQPadR - NC));
+        const bool pos = w > 0.0f;
+        const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
+        const float o0 = fminf(hi, f0 + 1.0f) - lo;
+        float w0, w1, w2 = 0.0f;
+        if (NC == 2) {
+            w0 = (pos ? o0 : 1.0f) * inv;
+            w1 = (w - o0) * inv;
+        } else {
+            const float o1 = fmaxf(fminf(hi, f0 + 2.0f) - (f0 + 1.0f), 0.0f);
+            w0 = (pos ? o0 : 1.0f) * inv;
+            w1 = o1 * inv;
+            w2 = (w - o0 - o1) * inv;
+        }
+        const unsigned boff = (unsigned)(j * pitch + kQPadL + m0) * 16u;   // m0 >= -kQPadL
+        #pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const float4 v0 = ld_at(Pq[k], boff), v1 = ld_at(Pq[k], boff + 16u);
+            float4& a = acc[k];
+            a.x = fmaf(v0.x, w0, a.x); a.y = fmaf(v0.y, w0, a.y); a.z = fmaf(v0.z, w0, a.z); a.w = fmaf(v0.w, w0, a.w);
+            a.x = fmaf(v1.x, w1, a.x); a.y = fmaf(v1.y, w1, a.y); a.z = fmaf(v1.z, w1, a.z); a.w = fmaf(v1.w, w1, a.w);
+            if (NC == 3) {
+                const float4 v2 = ld_at(Pq[k], boff + 32u);
+                a.x = fmaf(v2.x, w2, a.x); a.y = fmaf(v2.y, w2, a.y);
+                a.z = fmaf(v2.z, w2, a.z); a.w = fmaf(v2.w, w2, a.w);
+            }
+        }
+        ea = eb;
+    }
+}
+
+// One ray through G interleaved slice groups of padded quad images (Qyx for
+// X views, Qxy for Y views; groups qstride cells apart, groups k >= nq read
+// group nq - 1): the line range, wave-uniform loop and final scaling of
+// siddon_fp_ray_slices with the padded line loop above (rays with |B| >= 2,
+// none in ordinary geometries, take the clamped general path).
+template <int G>
+__device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4* __restrict__ Qyx,
+                                                    const float4* __restrict__ Qxy, int nq, int view, int iu,
+                                                    bool active, float4 (&acc)[G])
+{
+    const float4 r = g.sray[(long)view * g.nu + iu];
+    const bool xv = g.vcls[view] == 0;
+    const float4* __restrict__ P = xv ? Qyx : Qxy;
+    const int nl = xv ? g.ny : g.nx;     // lines
+    const int n = xv ? g.nx : g.ny;      // cells per line
+    const int pitch = qpitch(n);
+    const long qstride = (long)nl * pitch;
+    const float base = -0.5f * (float)nl;
+    int j0 = 0, j1 = nl - 1;
+    if (r.y != 0.0f) {
+        const float ib = 1.0f / r.y;
+        const float ta = (-fabsf(r.y) - 1.0f - r.x) * ib - base, tb = ((float)n + 1.0f - r.x) * ib - base;
+        const float tlo = fminf(fmaxf(fminf(ta, tb) - 1.0f, 0.0f), (float)nl);
+        const float thi = fminf(fmaxf(ta, tb) + 1.0f, (float)(nl - 1));
+        j0 = (int)tlo;
+        j1 = thi >= tlo ? (int)thi : -1;
+    } else if (!(r.x >= 0.0f && r.x < (float)n)) {
+        j1 = -1;
+    }
+    if (!active) { j0 = nl; j1 = -1; }
+    const int wj0 = __builtin_amdgcn_readfirstlane(wave_min(j0));
+    const int wj1 = __builtin_amdgcn_readfirstlane(wave_max(j1));
+    const float4* Pq[G];                 // wave-uniform group bases at the start of padded line 0
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        Pq[k] = P + (long)min(k, nq - 1) * qstride;
+        acc[k] = vzero<float4>();
+    }
+    if (__all(!active || fabsf(r.y) <= 0.999f)) {
+        fp_quad_lines_padded<G, 2>(Pq, n, pitch, r, base, wj0, wj1, acc);
+    } else if (__all(!active || fabsf(r.y) <= 1.999f)) {
+        fp_quad_lines_padded<G, 3>(Pq, n, pitch, r, base, wj0, wj1, acc);
+    } else {
+        for (int j = wj0; j <= wj1; ++j) {
+            const float lr = base + (float)j;
+            const float ea = fmaf(r.y, lr, r.x), eb = fmaf(r.y, lr + 1.0f, r.x);
+            #pragma unroll
+            for (int k = 0; k < G; ++k)
+                vadd(acc[k], siddon_line_sum<float4>(Pq[k] + (long)j * pitch + kQPadL, n, fminf(ea, eb),
+                                                     fmaxf(ea, eb)));
+        }
+    }
+    #pragma unroll
+    for (int k = 0; k < G; ++k) acc[k] = vscale(acc[k], r.z);
+}
+
 // One ray of one slice (siddon_fp_ray_slices with S = 1).
 template <class T>
 __device__ __forceinline__ T siddon_fp_ray(const FanK& g, const T* __restrict__ Pyx, const T* __restrict__ Pxy,

@@ -170,7 +170,7 @@ void fan_backward(const FanTables& t, FanBP kind, int nB, int nC, const float* p
         if (ns >= 4) {
             // groups of 4 slices interleaved: one float4 load per bin and group
             w.quad.ensure((size_t)cdiv(ns, 4) * sino * sizeof(float4));
-            launch_pack_quads(P, w.quad.as<float4>(), nullptr, g.nview, g.nu, ns, st);
+            launch_pack_quads(P, w.quad.as<float4>(), nullptr, g.nview, g.nu, ns, false, st);
             launch_dd_fan_bp_q(t.k, w.quad.as<float4>(), out, ns, fbp, st);
         } else {
             launch_dd_fan_bp(t.k, P, out, ns, fbp, st);

@@ -119,8 +119,9 @@ __global__ __launch_bounds__(256) void k_sqs_update(const float* __restrict__ X,
 
 // k_sqs_update for the interleaved layout, fused with the layout copies: a
 // 32 x 32 pixel tile of one group of 4 slices; each slice's new image goes to
-// Xn and, as the float4 of the 4 slices, to Qxy [q][nx][ny] and (through LDS)
-// to the transposed Qyx [q][ny][nx].  Slices past ns hold 0.
+// Xn and, as the float4 of the 4 slices, to the padded Qxy [q][nx][qpitch(ny)]
+// and (through LDS) to the transposed Qyx [q][ny][qpitch(nx)].  Slices past
+// ns hold 0.
 __global__ __launch_bounds__(256) void k_sqs_update_q(const float* __restrict__ X, float* __restrict__ Xn,
                                                       const float* __restrict__ G, const float* __restrict__ D,
                                                       float4* __restrict__ Qxy, float4* __restrict__ Qyx, int nx,
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(256) void k_sqs_update_q(const float* __restrict__ 
     const int iy0 = blockIdx.x * 32, ix0 = blockIdx.y * 32, q = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const long plane = (long)nx * ny;
+    const int pxy = qpitch(ny), pyx = qpitch(nx);
     for (int r = ty; r < 32; r += 8) {
         const int ix = ix0 + r, iy = iy0 + tx;
         if (ix >= nx || iy >= ny) continue;
@@ -157,12 +159,12 @@ __global__ __launch_bounds__(256) void k_sqs_update_q(const float* __restrict__ 
         }
         const float4 f = make_float4(v[0], v[1], v[2], v[3]);
         tile[r][tx] = f;
-        Qxy[(long)q * plane + pix] = f;
+        Qxy[(long)q * nx * pxy + (long)ix * pxy + kQPadL + iy] = f;
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         const int iy = iy0 + r, ix = ix0 + tx;
-        if (ix < nx && iy < ny) Qyx[(long)q * plane + (long)iy * nx + ix] = tile[tx][r];
+        if (ix < nx && iy < ny) Qyx[(long)q * ny * pyx + (long)iy * pyx + kQPadL + ix] = tile[tx][r];
     }
 }
 
@@ -289,8 +291,12 @@ void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nview
     quads_ = ns_ >= 4 && !(env && env[0] == '0');
     if (quads_) {
         const size_t nq = (size_t)(ns_ + 3) / 4;
-        Qxy_.ensure(nq * plane * sizeof(float4));
-        Qyx_.ensure(nq * plane * sizeof(float4));
+        // padded lines; the pad cells stay zero (only line interiors are written)
+        const size_t nxy = nq * g.nx * (size_t)qpitch(g.ny), nyx = nq * g.ny * (size_t)qpitch(g.nx);
+        Qxy_.ensure(nxy * sizeof(float4));
+        Qyx_.ensure(nyx * sizeof(float4));
+        launch_fill(Qxy_.as<float>(), 0.0f, 4 * nxy, st);
+        launch_fill(Qyx_.as<float>(), 0.0f, 4 * nyx, st);
         B4_.ensure(nq * sino * sizeof(float4));
         W4_.ensure(nq * sino * sizeof(float4));
         R4_.ensure(nq * (size_t)g.nu * npers_ * sizeof(float4));
@@ -314,8 +320,8 @@ void OsSqs::set_data(const float* prj_ref, const float* w_ref, hipStream_t st)
         launch_fill(W_.as<float>(), 1.0f, (size_t)ns_ * g.nu * g.nview, st);
     }
     if (quads_) {
-        launch_pack_quads(B_.as<float>(), B4_.as<float4>(), nullptr, g.nview, g.nu, ns_, st);
-        launch_pack_quads(W_.as<float>(), W4_.as<float4>(), nullptr, g.nview, g.nu, ns_, st);
+        launch_pack_quads(B_.as<float>(), B4_.as<float4>(), nullptr, g.nview, g.nu, ns_, false, st);
+        launch_pack_quads(W_.as<float>(), W4_.as<float4>(), nullptr, g.nview, g.nu, ns_, false, st);
     }
     have_w_ = true;
 }
@@ -350,7 +356,7 @@ void OsSqs::set_norm(const float* norm_xy, hipStream_t st)
 void OsSqs::sync_transposed(hipStream_t st)
 {
     if (quads_) launch_pack_quads(X_[cur_].as<float>(), Qxy_.as<float4>(), Qyx_.as<float4>(), t_.geo.nx, t_.geo.ny,
-                                  ns_, st);
+                                  ns_, true, st);
     else launch_transpose(X_[cur_].as<float>(), Xyx_.as<float>(), t_.geo.nx, t_.geo.ny, ns_, st);
 }
 
