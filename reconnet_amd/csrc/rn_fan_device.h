@@ -37,8 +37,6 @@ __device__ __forceinline__ void vmadd(float4& a, float4 v, float w)
 __device__ __forceinline__ float vscale(float v, float f) { return v * f; }
 __device__ __forceinline__ double vscale(double v, float f) { return v * (double)f; }
 __device__ __forceinline__ float4 vscale(float4 v, float f) { return make_float4(v.x * f, v.y * f, v.z * f, v.w * f); }
-__device__ __forceinline__ float vdiv(float v, float d) { return v / d; }
-__device__ __forceinline__ float4 vdiv(float4 v, float d) { return make_float4(v.x / d, v.y / d, v.z / d, v.w / d); }
 
 // ---------------------------------------------------------------------------
 // geometry helpers
@@ -91,8 +89,8 @@ __device__ __forceinline__ T seg_avg(const T* __restrict__ line, int n, float lo
 }
 
 // seg_avg of S rows (row q `rboff` bytes after the wave-uniform base rows[q])
-// over one interval: the cells and overlaps are computed once; each row gets
-// the arithmetic of seg_avg.
+// over one interval: the cells, overlaps and the reciprocal of the width are
+// computed once for the S rows.
 template <class T, int S>
 __device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned rboff, int n, float lo, float hi,
                                              T (&val)[S])
@@ -116,8 +114,10 @@ __device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned
         #pragma unroll
         for (int q = 0; q < S; ++q) vmadd(val[q], ld_at(rows[q], boff), ph - pl);
     }
+    // one correctly rounded reciprocal for all S rows
+    const float inv = 1.0f / w;
     #pragma unroll
-    for (int q = 0; q < S; ++q) val[q] = vdiv(val[q], w);
+    for (int q = 0; q < S; ++q) val[q] = vscale(val[q], inv);
 }
 
 // Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).

@@ -5,7 +5,8 @@
 // (R2C, spectrum product, C2R).  Because of the padding that product is an
 // exact linear convolution with the (2nu-1)-tap kernel, so here it is computed
 // directly: one workgroup per detector row, the weighted row and the taps
-// staged in LDS, Kahan-compensated FP32 accumulation.  The taps themselves are
+// staged in LDS, FP32 accumulation with Kahan compensation over the large
+// near taps.  The taps themselves are
 // built on the host in FP64 (closed-form Ram-Lak taps, window applied through
 // an exact FP64 DFT round trip, Filter.cu:90-193).
 
@@ -30,17 +31,52 @@ __global__ __launch_bounds__(256) void k_ramp_filter(const float* __restrict__ i
     for (int j = threadIdx.x; j < nu; j += blockDim.x) xs[j] = src[j] * pw[j];
     for (int j = threadIdx.x; j < 2 * nu - 1; j += blockDim.x) hs[j] = taps[j];
     __syncthreads();
-    for (int iu = threadIdx.x; iu < nu; iu += blockDim.x) {
-        // out[iu] = sum_j x[j] h[iu - j + nu - 1]
-        const float* hp = hs + iu + nu - 1;
-        float s = 0.0f, c = 0.0f;
-        for (int j = 0; j < nu; ++j) {
-            const float y = fmaf(xs[j], hp[-j], -c);
+    // out[iu] = sum_j x[j] h[iu - j + nu - 1].  The taps fall off as 1/k^2:
+    // the terms within kNear of the wave's 64 outputs (a band uniform over the
+    // wave) are summed with Kahan compensation, the small far terms first with
+    // plain FMAs.
+    constexpr int kNear = 32;
+    const int lane = threadIdx.x & 63;
+    const int nu4 = nu & ~3;
+    const float4* xs4 = reinterpret_cast<const float4*>(xs);   // 4 samples per broadcast LDS read
+    for (int iu = threadIdx.x; iu - lane < nu; iu += blockDim.x) {
+        const int iu0 = iu - lane;                   // the wave's first output
+        const bool act = iu < nu;
+        const float* hp = hs + min(iu, nu - 1) + nu - 1;
+        // Kahan band [b0, b1) in whole groups of 4 samples (plus the < 4 samples past nu4)
+        const int b0 = max(iu0 - kNear, 0) & ~3;
+        const int b1 = min(((iu0 + 63 + kNear) | 3) + 1, nu4);
+        float f = 0.0f;
+        for (int j = 0; j < b0; j += 4) {
+            const float4 x = xs4[j >> 2];
+            f = fmaf(x.x, hp[-j], f);
+            f = fmaf(x.y, hp[-j - 1], f);
+            f = fmaf(x.z, hp[-j - 2], f);
+            f = fmaf(x.w, hp[-j - 3], f);
+        }
+        for (int j = b1; j < nu4; j += 4) {
+            const float4 x = xs4[j >> 2];
+            f = fmaf(x.x, hp[-j], f);
+            f = fmaf(x.y, hp[-j - 1], f);
+            f = fmaf(x.z, hp[-j - 2], f);
+            f = fmaf(x.w, hp[-j - 3], f);
+        }
+        float s = f, c = 0.0f;
+        auto kahan = [&](float x, float h) {
+            const float y = fmaf(x, h, -c);
             const float t = s + y;
             c = (t - s) - y;
             s = t;
+        };
+        for (int j = b0; j < b1; j += 4) {
+            const float4 x = xs4[j >> 2];
+            kahan(x.x, hp[-j]);
+            kahan(x.y, hp[-j - 1]);
+            kahan(x.z, hp[-j - 2]);
+            kahan(x.w, hp[-j - 3]);
         }
-        out_vu[row * nu + iu] = s;
+        for (int j = nu4; j < nu; ++j) kahan(xs[j], hp[-j]);
+        if (act) out_vu[row * nu + iu] = s;
     }
 }
 
