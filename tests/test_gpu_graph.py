@@ -31,8 +31,10 @@ def _noise_work():
     torch.cuda.synchronize()
 
 
-def test_pass_graph_replays_equal_eager():
-    eng = _engine()
+@pytest.mark.parametrize("nslices", [2, 9])
+def test_pass_graph_replays_equal_eager(nslices):
+    # 9 slices: the engine's interleaved layout (three 4-slice groups)
+    eng = _engine(nslices)
     x0 = torch.full((eng.nslices, eng.geo.nx, eng.geo.ny), 0.01, device="cuda")
     eng.set_image(x0)
     for _ in range(3):
@@ -55,11 +57,12 @@ def test_pass_graph_rejects_odd_step_count():
         eng.capture_passes(1)
 
 
-def test_unrolled_graph_replays_equal_eager():
+@pytest.mark.parametrize("nslices", [2, 9])
+def test_unrolled_graph_replays_equal_eager(nslices):
     from reconnet_amd.cnn import PriorCNN, init_params
     from reconnet_amd.unrolled import UnrolledGraph, UnrolledOSSQS
 
-    eng = _engine()
+    eng = _engine(nslices)
     nets = [PriorCNN(init_params(seed=k, last_scale=0.5), scale=50.0) for k in range(2)]
     model = UnrolledOSSQS(nets)
     ref = model.reconstruct(eng).clone()
