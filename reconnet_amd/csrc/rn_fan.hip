@@ -651,8 +651,8 @@ void go_dd_bp_q(const FanK& g, const float4* wprj, float* img, int ns, int fbp, 
                        dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
 }
 using DdBpQFn = void (*)(const FanK&, const float4*, float*, int, int, hipStream_t);
-const DdBpQFn kDdBpQ[] = {go_dd_bp_q<8, 8, 1>, go_dd_bp_q<16, 16, 1>, go_dd_bp_q<8, 8, 2>, go_dd_bp_q<16, 16, 2>,
-                          go_dd_bp_q<8, 8, 4>};
+const DdBpQFn kDdBpQ[] = {go_dd_bp_q<8, 8, 4>, go_dd_bp_q<16, 16, 2>, go_dd_bp_q<8, 8, 2>, go_dd_bp_q<16, 16, 1>,
+                          go_dd_bp_q<8, 8, 1>};
 constexpr int kNDdBpQ = sizeof(kDdBpQ) / sizeof(kDdBpQ[0]);
 
 // The Siddon and DD gather kernels address one slice's image plane, ray table
@@ -690,15 +690,16 @@ void go_siddon_bp_q(const FanK& g, const float4* prj, float* img, int ns, int vi
 using SidFpQFn = void (*)(const FanK&, const float4*, const float4*, float4*, int, int, int, const float4*,
                           const float4*, int, int, hipStream_t);
 using SidBpQFn = void (*)(const FanK&, const float4*, float*, int, int, int, hipStream_t);
-// G: groups of 4 slices per lane
+// G: groups of 4 slices per lane; candidates in order of preference (the tuner
+// keeps the first within 3 % of the fastest)
 const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<128, kXcds, 1>, go_siddon_fp_q<64, kXcds, 1>,
                             go_siddon_fp_q<256, kXcds, 1>, go_siddon_fp_q<128, 0, 1>,
                             go_siddon_fp_q<64, 0, 1>,      go_siddon_fp_q<128, kXcds, 2>,
                             go_siddon_fp_q<128, 0, 2>};
-const SidBpQFn kSidBpQ[] = {go_siddon_bp_q<8, 8, 2, 2>,   go_siddon_bp_q<8, 8, 2, 1>, go_siddon_bp_q<16, 16, 2, 1>,
-                            go_siddon_bp_q<16, 16, 2, 2>, go_siddon_bp_q<8, 8, 2, 4>, go_siddon_bp_q<16, 16, 2, 4>,
-                            go_siddon_bp_q<8, 8, 3, 2>,   go_siddon_bp_q<8, 8, 3, 4>, go_siddon_bp_q<8, 8, 2, 8>,
-                            go_siddon_bp_q<16, 16, 2, 8>};
+const SidBpQFn kSidBpQ[] = {go_siddon_bp_q<16, 16, 2, 8>, go_siddon_bp_q<8, 8, 2, 8>, go_siddon_bp_q<16, 16, 2, 4>,
+                            go_siddon_bp_q<8, 8, 2, 4>,   go_siddon_bp_q<8, 8, 2, 2>, go_siddon_bp_q<16, 16, 2, 2>,
+                            go_siddon_bp_q<8, 8, 2, 1>,   go_siddon_bp_q<16, 16, 2, 1>, go_siddon_bp_q<8, 8, 3, 4>,
+                            go_siddon_bp_q<8, 8, 3, 2>};
 constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
 constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);
 

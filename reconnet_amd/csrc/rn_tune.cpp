@@ -95,11 +95,15 @@ int tune_choose(const TuneKey& key, int ncand, const std::function<void(int)>& l
         float ms = 0.0f;
         RN_HIP_CHECK(hipEventElapsedTime(&ms, ev.a, ev.b));
         e.ms[i] = ms / reps;
-        if (i == 0 || e.ms[i] < best) {
-            best = e.ms[i];
-            e.choice = i;
-        }
+        if (i == 0 || e.ms[i] < best) best = e.ms[i];
     }
+    // the first candidate within 3 % of the fastest: timing noise does not
+    // move the choice away from the earlier-listed (preferred) shapes
+    for (int i = 0; i < ncand; i++)
+        if (e.ms[i] <= 1.03f * best) {
+            e.choice = i;
+            break;
+        }
     std::lock_guard<std::mutex> lk(s.mu);
     auto res = s.cache.emplace(key, e);
     return res.first->second.choice;

@@ -3,8 +3,9 @@
 // A tunable launch names its kernel, a shape key (the sizes that change the
 // best configuration) and a list of candidate configurations.  On the first
 // launch of a key the tuner times every candidate with HIP events on the
-// caller's stream (one warm-up and three timed runs each) and keeps the
-// fastest; later launches of the key reuse it.  Candidates differ only in the
+// caller's stream (one warm-up and three timed runs each) and keeps the first
+// candidate within 3 % of the fastest (candidates are listed in order of
+// preference); later launches of the key reuse it.  Candidates differ only in the
 // block / tile shape of the launch: every output element is computed by the
 // same per-element code, so the choice never changes the results.
 //

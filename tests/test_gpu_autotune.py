@@ -35,7 +35,9 @@ def test_tuned_shapes_give_identical_results(fp, bp):
     for key, (choice, ms) in rep.items():
         assert 0 <= choice < len(ms) and len(ms) >= 3
         assert all(m > 0 for m in ms)
-        assert ms[choice] == min(ms)
+        # the first candidate within 3 % of the fastest
+        assert ms[choice] <= 1.03 * min(ms)
+        assert all(m > 1.03 * min(ms) for m in ms[:choice])
 
 
 @pytest.mark.parametrize("ns", [1, 3, 11])
