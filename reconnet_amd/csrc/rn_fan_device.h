@@ -551,9 +551,9 @@ __device__ __forceinline__ float siddon_cell_weight(const float4 r, float lr, fl
 
 // the bins k0 .. k1 of one view for S slices (slice q's row `rboff` bytes
 // after the wave-uniform base pq[q]; the view's ray table row `rowoff` bytes
-// after g.sray): the first NB without branches (clamped, zero-weighted when
-// past k1, so an empty range k1 < k0 adds exact zeros), the rest in a loop;
-// each weight is computed once and applied to every slice.
+// after g.sray): the first NB unrolled (each under its own lane mask, so an
+// empty range k1 < k0 adds nothing), the rest in a loop; each weight is
+// computed once and applied to every slice.
 template <class T, int NB, int S>
 __device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sray, unsigned rowoff,
                                                    const T* const (&pq)[S], unsigned rboff, int nu, int k0, int k1,
@@ -561,13 +561,15 @@ __device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sr
 {
     #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const unsigned kk = (unsigned)min(k0 + j, nu - 1);
-        const float4 r = ld_at(sray, rowoff + kk * 16u);
-        const float valid = (k0 + j <= k1) ? 1.0f : 0.0f;
-        const float wt = siddon_cell_weight<false>(r, lr, cell) * valid;
-        const unsigned boff = rboff + kk * (unsigned)sizeof(T);
-        #pragma unroll
-        for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(pq[q], boff), wt);
+        // lanes past k1 take no part (their loads are masked off, so they
+        // cost no vector-memory bandwidth); the others add weight x value
+        if (k0 + j <= k1) {
+            const unsigned kk = (unsigned)(k0 + j);
+            const float wt = siddon_cell_weight<false>(ld_at(sray, rowoff + kk * 16u), lr, cell);
+            const unsigned boff = rboff + kk * (unsigned)sizeof(T);
+            #pragma unroll
+            for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(pq[q], boff), wt);
+        }
     }
     for (int k = k0 + NB; k <= k1; ++k) {
         const float wt = siddon_cell_weight<false>(ld_at(sray, rowoff + (unsigned)k * 16u), lr, cell);
