@@ -91,11 +91,15 @@ def test_every_launch_shape_agrees(ns):
         autotune(enable=True, reset=True)
 
 
-def _engine(ns, seed=3):
+def _engine(ns, seed=3, rect=False):
     import torch
+    from reconnet_amd.geometry import FanGeometry
     from reconnet_amd.sqs import OsSqsTV
 
-    geo = small_fan(nx=96, nu=140, nview=60, dx=1.3)
+    if rect:
+        geo = FanGeometry(nx=90, ny=61, dx=1.3, dy=0.95, nu=140, nview=60, da=1.2 / 1085.6, dsd=1085.6, dso=595.0)
+    else:
+        geo = small_fan(nx=96, nu=140, nview=60, dx=1.3)
     rng = np.random.default_rng(seed)
     b = torch.from_numpy(rng.uniform(0, 2, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
     w = torch.from_numpy(rng.uniform(0.5, 1, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
@@ -138,14 +142,15 @@ def test_every_interleaved_launch_shape_agrees():
         eng.close()
 
 
-@pytest.mark.parametrize("ns", [4, 7])
-def test_interleaved_engine_matches_plain_layout(ns, monkeypatch):
+@pytest.mark.parametrize("ns,rect", [(4, False), (7, False), (6, True)])
+def test_interleaved_engine_matches_plain_layout(ns, rect, monkeypatch):
     # the same passes with the engine's plain one-slice layout
-    # (RECONNET_SQS_INTERLEAVE=0) and with the interleaved 4-slice layout
+    # (RECONNET_SQS_INTERLEAVE=0) and with the interleaved 4-slice layout,
+    # also on a rectangular grid whose two orientations have different pitches
     outs = []
     for flag in ("0", "1"):
         monkeypatch.setenv("RECONNET_SQS_INTERLEAVE", flag)
-        eng, x0 = _engine(ns)
+        eng, x0 = _engine(ns, rect=rect)
         eng.set_image(x0)
         for _ in range(2):
             eng.run_pass(beta=1e-3, eps=1e-4, lb=0.0)

@@ -245,16 +245,22 @@ def test_ossqs_parity_small():
     assert rel_l2(out, ref) <= TOL
 
 
-def test_ossqs_parity_batch_of_slices():
+def _rect_fan(nx, ny, dx, dy, nu, nview):
+    return FanGeometry(nx=nx, ny=ny, dx=dx, dy=dy, nu=nu, nview=nview, da=1.2 / 1085.6, dsd=1085.6, dso=595.0)
+
+
+@pytest.mark.parametrize("nx,ny,dx,dy", [(64, 64, 1.0, 1.0), (72, 50, 1.1, 0.9)])
+def test_ossqs_parity_batch_of_slices(nx, ny, dx, dy):
     # six slices under one engine: the interleaved 4-slice layout with a
-    # padded second group, against the oracle's pass of every slice
-    geo = small_fan(nx=64, nu=128, nview=96)
+    # padded second group, against the oracle's pass of every slice; also on
+    # a rectangular, anisotropic grid (the two image orientations differ)
+    geo = _rect_fan(nx, ny, dx, dy, 128, 96)
     deg = fan_angles(geo.nview)
-    truth = np.concatenate([_img5(phantom(20 + s, 64, 64, 1.0, 1.0)) for s in range(6)])
+    truth = np.concatenate([_img5(phantom(20 + s, nx, ny, dx, dy)) for s in range(6)])
     b = orc.siddon_fan_fp(truth, deg, geo.oracle_geo(), geo.nu).astype(np.float32)
     w = np.random.default_rng(3).uniform(0.5, 1.0, b.shape).astype(np.float32)
     fp1 = orc.siddon_fan_fp(np.ones_like(truth[:1]), deg, geo.oracle_geo(), geo.nu)
-    D = orc.siddon_fan_bp((fp1 * w).astype(np.float32), deg, geo.oracle_geo(), (64, 64, 1)).astype(np.float32)
+    D = orc.siddon_fan_bp((fp1 * w).astype(np.float32), deg, geo.oracle_geo(), (nx, ny, 1)).astype(np.float32)
     x0 = np.random.default_rng(4).uniform(0, 0.02, truth.shape).astype(np.float32)
     out = recon_for(geo).cSiddonFanOSSQS4d(8, x0, b, D, w, lb=0.0, ub=1.0)
     ref = orc.os_sqs(x0, b, D, w, deg, geo.oracle_geo(), 8, lb=0.0, ub=1.0)
@@ -262,8 +268,8 @@ def test_ossqs_parity_batch_of_slices():
         assert rel_l2(out[s], ref[s]) <= TOL, s
 
 
-@pytest.mark.parametrize("ns", [1, 5])
-def test_sqs_tv_pass_matches_oracle_composition(ns):
+@pytest.mark.parametrize("ns,rect", [(1, False), (5, False), (5, True)])
+def test_sqs_tv_pass_matches_oracle_composition(ns, rect):
     # one SQS-TV pass of the engine (no reference counterpart) against its
     # definition composed from oracle pieces: per subset s,
     #   x <- clamp(x - (M A_s'W_s(A_s x - b_s) + 2 beta g_TV(x)) / (D + 2 beta n_TV(x) + 1e-6), 0)
@@ -271,7 +277,7 @@ def test_sqs_tv_pass_matches_oracle_composition(ns):
     import torch
     from reconnet_amd.sqs import OsSqsTV
 
-    geo = small_fan(nx=48, nu=96, nview=40)
+    geo = _rect_fan(56, 37, 1.2, 0.8, 96, 40) if rect else small_fan(nx=48, nu=96, nview=40)
     nsub, beta = 4, 1e-3
     deg = fan_angles(geo.nview)
     rng = np.random.default_rng(5)
