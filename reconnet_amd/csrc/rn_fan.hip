@@ -192,6 +192,113 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp_q(FanK g, const float
     }
 }
 
+// k_siddon_fan_bp_q with each wave's bin window staged in LDS.  Per view the
+// wave's 64 pixels need the bins [wlo, whi] (the lane minimum and maximum of
+// their ranges); the view's ray table and the G groups' values over the CAP
+// bins from wlo are loaded once per wave with coalesced dwordx4 loads, issued
+// one view ahead into registers, and written to the wave's own LDS rows, from
+// which every pixel gathers its bins: a fraction of the per-pixel gathers'
+// vector-memory traffic.  A wave touches only its own rows and the LDS
+// instructions of one wave execute in order, so no barrier is needed; the
+// fences keep the compiler from moving LDS accesses across the handover.
+// Views whose window is wider than CAP bins take the direct gathers.
+template <int BX, int BY, int NB, int G, int CAP, int OCC>
+__global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const float4* __restrict__ prj,
+                                                          float* __restrict__ img, int view0, int nviews, int ns)
+{
+    static_assert((BX * BY) % 64 == 0, "whole waves");
+    constexpr int kRows = G + 1, kLoads = (kRows * CAP + 63) / 64;
+    __shared__ float4 stage[(BX * BY / 64) * kRows * CAP];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int lane = tid & 63;
+    float4* st = stage + (tid >> 6) * kRows * CAP;
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
+    const bool inside = ix < g.nx && iy < g.ny;   // lanes outside stage and gather nothing
+    const int nquads = (ns + 3) / 4;
+    const int q0 = blockIdx.z * G;
+    const int nq = min(G, nquads - q0);
+    const long sstride = (long)nviews * g.nu;
+    const float4* P = prj + q0 * sstride;
+    const int cx = min(ix, g.nx - 1), cy = min(iy, g.ny - 1);
+    const float xc = (cx - 0.5f * (g.nx - 1)) * g.dx, yc = (cy - 0.5f * (g.ny - 1)) * g.dy;
+    const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
+    const float inv_da = 1.0f / g.da;
+    const float yr = (float)cy - 0.5f * g.ny, xr = (float)cx - 0.5f * g.nx;
+    // staging load j of this lane: element i = 64 j + lane of the rows, that
+    // is row i / CAP, bin i % CAP of the window
+    int bj[kLoads];
+    const float4* src[kLoads];
+    bool has[kLoads];
+    #pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+        const int i = j * 64 + lane, r = i / CAP;
+        bj[j] = i % CAP;
+        has[j] = r < kRows;
+        src[j] = r == 0 ? g.sray + (long)view0 * g.nu : P + (long)min(r - 1, nq - 1) * sstride;
+    }
+    const float4* pq[G];
+    float4 acc[G];
+    #pragma unroll
+    for (int q = 0; q < G; ++q) {
+        pq[q] = P + (long)min(q, nq - 1) * sstride;
+        acc[q] = vzero<float4>();
+    }
+    float4 rg[kLoads];
+    // the pixel's bins of view vi (empty outside the image or the detector)
+    auto range = [&](int vi, int& k0, int& k1) {
+        if (g.unit_angle) bin_range<true>(g, xc, yc, rpix, inv_da, view0 + vi, k0, k1);
+        else bin_range<false>(g, xc, yc, rpix, inv_da, view0 + vi, k0, k1);
+        if (!inside || k1 < k0) { k0 = g.nu; k1 = -1; }
+    };
+    auto issue = [&](int vi, int wlo) {
+        #pragma unroll
+        for (int j = 0; j < kLoads; ++j)
+            if (has[j]) rg[j] = ld_at(src[j], (unsigned)(vi * g.nu + min(wlo + bj[j], g.nu - 1)) * 16u);
+    };
+    int k0, k1;
+    range(0, k0, k1);
+    int wlo = wave_min(k0), whi = wave_max(k1);
+    if (whi >= wlo && whi - wlo < CAP) issue(0, wlo);
+    for (int vi = 0; vi < nviews; ++vi) {
+        const bool staged = whi - wlo < CAP;   // also when the window is empty
+        if (staged && whi >= wlo) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            #pragma unroll
+            for (int j = 0; j < kLoads; ++j)
+                if (has[j]) st[j * 64 + lane] = rg[j];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        int n0 = g.nu, n1 = -1, nlo = g.nu, nhi = -1;
+        if (vi + 1 < nviews) {
+            range(vi + 1, n0, n1);
+            nlo = wave_min(n0);
+            nhi = wave_max(n1);
+            if (nhi >= nlo && nhi - nlo < CAP) issue(vi + 1, nlo);
+        }
+        const int view = view0 + vi;
+        const bool xv = g.vcls[view] == 0;
+        const float lr = xv ? yr : xr;
+        const float cell = xv ? (float)cx : (float)cy;
+        if (staged) siddon_gather_staged<NB, G, CAP>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
+        else siddon_gather_bins<float4, NB, G>(g.sray, (unsigned)(view * g.nu) * 16u, pq,
+                                               (unsigned)(vi * g.nu) * 16u, g.nu, k0, k1, lr, cell, acc);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        k0 = n0; k1 = n1; wlo = nlo; whi = nhi;
+    }
+    if (!inside) return;
+    const long plane = (long)g.nx * g.ny, pix = (long)ix * g.ny + iy;
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= nq) break;
+        const int s = 4 * (q0 + k);
+        const float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+        #pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (s + c < ns) img[(long)(s + c) * plane + pix] = v[c];
+    }
+}
+
 template <int BX, int BY>
 __global__ __launch_bounds__(BX * BY) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
                                                     float* __restrict__ img, int view0, int nviews, int accumulate)
@@ -687,6 +794,14 @@ void go_siddon_bp_q(const FanK& g, const float4* prj, float* img, int ns, int vi
                        dim3(BX, BY), 0, st, g, prj, img, view0, nviews, ns);
 }
 
+template <int BX, int BY, int NB, int G, int CAP, int OCC = 1>
+void go_siddon_bp_ql(const FanK& g, const float4* prj, float* img, int ns, int view0, int nviews, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_siddon_fan_bp_ql<BX, BY, NB, G, CAP, OCC>),
+                       dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)), dim3(BX, BY), 0, st, g, prj, img,
+                       view0, nviews, ns);
+}
+
 using SidFpQFn = void (*)(const FanK&, const float4*, const float4*, float4*, int, int, int, const float4*,
                           const float4*, int, int, hipStream_t);
 using SidBpQFn = void (*)(const FanK&, const float4*, float*, int, int, int, hipStream_t);
@@ -696,9 +811,14 @@ const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<128, kXcds, 1>, go_siddon_fp_q<64, kX
                             go_siddon_fp_q<256, kXcds, 1>, go_siddon_fp_q<128, 0, 1>,
                             go_siddon_fp_q<64, 0, 1>,      go_siddon_fp_q<128, kXcds, 2>,
                             go_siddon_fp_q<128, 0, 2>};
-const SidBpQFn kSidBpQ[] = {go_siddon_bp_q<16, 16, 2, 8>, go_siddon_bp_q<8, 8, 2, 8>, go_siddon_bp_q<16, 16, 2, 4>,
-                            go_siddon_bp_q<8, 8, 2, 4>,   go_siddon_bp_q<8, 8, 2, 2>, go_siddon_bp_q<16, 16, 2, 2>,
-                            go_siddon_bp_q<8, 8, 2, 1>,   go_siddon_bp_q<16, 16, 2, 1>, go_siddon_bp_q<8, 8, 3, 4>,
+const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 21>,  go_siddon_bp_ql<8, 8, 2, 8, 16>,
+                            go_siddon_bp_ql<8, 32, 2, 8, 21>, go_siddon_bp_ql<8, 8, 3, 8, 21>,
+                            go_siddon_bp_ql<8, 8, 2, 8, 32>,  go_siddon_bp_ql<8, 8, 2, 4, 21>,
+                            go_siddon_bp_ql<8, 8, 2, 2, 32>,  go_siddon_bp_q<16, 16, 2, 8>,
+                            go_siddon_bp_q<8, 8, 2, 8>,       go_siddon_bp_q<16, 16, 2, 4>,
+                            go_siddon_bp_q<8, 8, 2, 4>,       go_siddon_bp_q<8, 8, 2, 2>,
+                            go_siddon_bp_q<16, 16, 2, 2>,     go_siddon_bp_q<8, 8, 2, 1>,
+                            go_siddon_bp_q<16, 16, 2, 1>,     go_siddon_bp_q<8, 8, 3, 4>,
                             go_siddon_bp_q<8, 8, 3, 2>};
 constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
 constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);

@@ -296,17 +296,19 @@ __device__ __forceinline__ void fp_narrow_lines(const T* const (&Pq)[S], int n, 
     }
 }
 
-// wave-wide minimum / maximum (every lane of the wavefront must take part)
+// Minimum and maximum of v over the 64 lanes of a wave (every lane active):
+// pair, quad, half-row and row exchanges by DPP, then the four rows' values
+// read into scalar registers, so the result is wave-uniform.
 __device__ __forceinline__ int wave_min(int v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-    return v;
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));    // quad_perm [1, 0, 3, 2]
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));    // quad_perm [2, 3, 0, 1]
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));   // row_half_mirror
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));   // row_mirror
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
-__device__ __forceinline__ int wave_max(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return v;
-}
+__device__ __forceinline__ int wave_max(int v) { return -wave_min(-v); }
 
 // One ray (bin iu of `view`) through S slices of one geometry: slice q reads
 // planes Pyx + q * plane / Pxy + q * plane (slices q >= nsl read slice nsl - 1;
@@ -341,8 +343,8 @@ __device__ __forceinline__ void siddon_fp_ray_slices(const FanK& g, const T* __r
         j1 = -1;
     }
     if (!active) { j0 = nl; j1 = -1; }
-    const int wj0 = __builtin_amdgcn_readfirstlane(wave_min(j0));
-    const int wj1 = __builtin_amdgcn_readfirstlane(wave_max(j1));
+    const int wj0 = wave_min(j0);
+    const int wj1 = wave_max(j1);
     const T* Pq[S];                      // wave-uniform slice bases
     #pragma unroll
     for (int q = 0; q < S; ++q) {
@@ -453,8 +455,8 @@ __device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4*
         j1 = -1;
     }
     if (!active) { j0 = nl; j1 = -1; }
-    const int wj0 = __builtin_amdgcn_readfirstlane(wave_min(j0));
-    const int wj1 = __builtin_amdgcn_readfirstlane(wave_max(j1));
+    const int wj0 = wave_min(j0);
+    const int wj1 = wave_max(j1);
     const float4* Pq[G];                 // wave-uniform group bases at the start of padded line 0
     #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -576,6 +578,30 @@ __device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sr
         const unsigned boff = rboff + (unsigned)k * (unsigned)sizeof(T);
         #pragma unroll
         for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(pq[q], boff), wt);
+    }
+}
+
+// siddon_gather_bins for G slice groups from a wave's staged window in LDS:
+// row 0 of `st` holds the view's ray table and row 1 + q group q's values,
+// CAP bins per row from the window's first bin; t0 .. t1 are the pixel's bins
+// relative to that first bin.
+template <int NB, int G, int CAP>
+__device__ __forceinline__ void siddon_gather_staged(const float4* st, int t0, int t1, float lr, float cell,
+                                                     float4 (&acc)[G])
+{
+    #pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (t0 + j <= t1) {
+            const int t = t0 + j;
+            const float wt = siddon_cell_weight<false>(st[t], lr, cell);
+            #pragma unroll
+            for (int q = 0; q < G; ++q) vmadd(acc[q], st[(1 + q) * CAP + t], wt);
+        }
+    }
+    for (int t = t0 + NB; t <= t1; ++t) {
+        const float wt = siddon_cell_weight<false>(st[t], lr, cell);
+        #pragma unroll
+        for (int q = 0; q < G; ++q) vmadd(acc[q], st[(1 + q) * CAP + t], wt);
     }
 }
 

@@ -135,7 +135,8 @@ std::string tune_report()
                               kv.second.choice);
         out.append(line, n);
         for (size_t i = 0; i < kv.second.ms.size(); i++) {
-            n = std::snprintf(line, sizeof line, "%s%.4f", i ? " " : "", kv.second.ms[i]);
+            // every digit of the float, so the report shows the values the choice compared
+            n = std::snprintf(line, sizeof line, "%s%.9g", i ? " " : "", (double)kv.second.ms[i]);
             out.append(line, n);
         }
         out += " ms)\n";
