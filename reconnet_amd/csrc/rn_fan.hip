@@ -416,6 +416,121 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_q(FanK g, const float4* _
     }
 }
 
+// k_dd_fan_bp_q with each wave's bin window staged in LDS (as
+// k_siddon_fan_bp_ql): per view the wave's 64 pixels read the bins [wlo, whi]
+// of the G groups' pre-weighted rows; those CAP bins from wlo are loaded once
+// per wave with coalesced dwordx4 loads, issued one view ahead into
+// registers, written to the wave's own LDS rows and averaged from there with
+// the arithmetic of dd_bp_pixel.  Views whose window is wider than CAP bins
+// read the rows directly.
+template <int BX, int BY, int G, int CAP>
+__global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_ql(FanK g, const float4* __restrict__ wprj,
+                                                      float* __restrict__ img, int ns, int fbp)
+{
+    static_assert((BX * BY) % 64 == 0, "whole waves");
+    constexpr int kLoads = (G * CAP + 63) / 64;
+    __shared__ float4 stage[(BX * BY / 64) * G * CAP];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int lane = tid & 63;
+    float4* st = stage + (tid >> 6) * G * CAP;
+    const int iy = blockIdx.x * BX + threadIdx.x;
+    const int ix = blockIdx.y * BY + threadIdx.y;
+    const bool inside = ix < g.nx && iy < g.ny;   // lanes outside stage but gather nothing
+    const int nquads = (ns + 3) / 4;
+    const int q0 = blockIdx.z * G;
+    const int nq = min(G, nquads - q0);
+    const long sstride = (long)g.nview * g.nu;
+    const float4* P = wprj + q0 * sstride;
+    const float4* rows[G];
+    float4 acc[G];
+    #pragma unroll
+    for (int q = 0; q < G; ++q) {
+        rows[q] = P + (long)min(q, nq - 1) * sstride;
+        acc[q] = vzero<float4>();
+    }
+    // staging load j of this lane: element i = 64 j + lane, row i / CAP, bin i % CAP
+    int bj[kLoads];
+    const float4* src[kLoads];
+    bool has[kLoads];
+    #pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+        const int i = j * 64 + lane, r = i / CAP;
+        bj[j] = i % CAP;
+        has[j] = r < G;
+        src[j] = P + (long)min(r, nq - 1) * sstride;
+    }
+    const int cx = min(ix, g.nx - 1), cy = min(iy, g.ny - 1);
+    const float x = (cx - 0.5f * (g.nx - 1)) * g.dx, y = (cy - 0.5f * (g.ny - 1)) * g.dy;
+    const float inv_da = 1.0f / g.da;
+    const float boff = 0.5f * g.nu - g.off_a;
+    // the footprint [lo, hi] of the pixel in view `view` (dd_bp_pixel), its
+    // bins and FBP weight
+    auto geom = [&](int view, float& lo, float& hi, float& fw, int& m0, int& m1) {
+        const float ct = g.vcos[view], sn = g.vsin[view];
+        const float rx = x * ct + y * sn, ry = -x * sn + y * ct;
+        float ex, ey;
+        if (g.vcls[view] == 0) { ex = 0.5f * g.dx * ct; ey = -0.5f * g.dx * sn; }
+        else                   { ex = 0.5f * g.dy * sn; ey = 0.5f * g.dy * ct; }
+        const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
+        const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
+        lo = fminf(b1, b2);
+        hi = fmaxf(b1, b2);
+        const float d = g.dso + ry;
+        fw = g.dso * g.dso / (rx * rx + d * d);
+        seg_avg_bins(g.nu, lo, hi, m0, m1);
+        if (!inside) { m0 = g.nu; m1 = -1; }
+    };
+    float4 rg[kLoads];
+    auto issue = [&](int view, int wlo) {
+        #pragma unroll
+        for (int j = 0; j < kLoads; ++j)
+            if (has[j]) rg[j] = ld_at(src[j], (unsigned)(view * g.nu + min(wlo + bj[j], g.nu - 1)) * 16u);
+    };
+    float lo, hi, fw;
+    int m0, m1;
+    geom(0, lo, hi, fw, m0, m1);
+    int wlo = wave_min(m0), whi = wave_max(m1);
+    if (whi >= wlo && whi - wlo < CAP) issue(0, wlo);
+    for (int view = 0; view < g.nview; ++view) {
+        const bool staged = whi - wlo < CAP;   // also when the window is empty
+        if (staged && whi >= wlo) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            #pragma unroll
+            for (int j = 0; j < kLoads; ++j)
+                if (has[j]) st[j * 64 + lane] = rg[j];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        float nlo_f = 0.0f, nhi_f = 0.0f, nfw = 1.0f;
+        int n0 = g.nu, n1 = -1, nlo = g.nu, nhi = -1;
+        if (view + 1 < g.nview) {
+            geom(view + 1, nlo_f, nhi_f, nfw, n0, n1);
+            nlo = wave_min(n0);
+            nhi = wave_max(n1);
+            if (nhi >= nlo && nhi - nlo < CAP) issue(view + 1, nlo);
+        }
+        if (inside) {
+            float4 val[G];
+            if (staged) seg_avg_staged<G, CAP>(st, wlo, g.nu, lo, hi, val);
+            else seg_avg_rows<float4, G>(rows, (unsigned)(view * g.nu) * 16u, g.nu, lo, hi, val);
+            #pragma unroll
+            for (int q = 0; q < G; ++q) vadd(acc[q], fbp ? vscale(val[q], fw) : val[q]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        lo = nlo_f; hi = nhi_f; fw = nfw; wlo = nlo; whi = nhi;
+    }
+    if (!inside) return;
+    const long plane = (long)g.nx * g.ny, pix = (long)ix * g.ny + iy;
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= nq) break;
+        const int s = 4 * (q0 + k);
+        const float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+        #pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (s + c < ns) img[(long)(s + c) * plane + pix] = v[c];
+    }
+}
+
 // bpFanKernel3D with isFBP = true (the only mode the reference launches).
 template <int BX, int BY>
 __global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* __restrict__ prj,
@@ -757,9 +872,17 @@ void go_dd_bp_q(const FanK& g, const float4* wprj, float* img, int ns, int fbp, 
     hipLaunchKernelGGL((k_dd_fan_bp_q<BX, BY, G>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
                        dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
 }
+template <int BX, int BY, int G, int CAP>
+void go_dd_bp_ql(const FanK& g, const float4* wprj, float* img, int ns, int fbp, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dd_fan_bp_ql<BX, BY, G, CAP>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
+                       dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
+}
 using DdBpQFn = void (*)(const FanK&, const float4*, float*, int, int, hipStream_t);
-const DdBpQFn kDdBpQ[] = {go_dd_bp_q<8, 8, 4>, go_dd_bp_q<16, 16, 2>, go_dd_bp_q<8, 8, 2>, go_dd_bp_q<16, 16, 1>,
-                          go_dd_bp_q<8, 8, 1>};
+const DdBpQFn kDdBpQ[] = {go_dd_bp_ql<8, 8, 8, 32>, go_dd_bp_ql<8, 32, 8, 32>, go_dd_bp_ql<8, 32, 8, 24>,
+                          go_dd_bp_ql<8, 8, 8, 24>, go_dd_bp_ql<8, 8, 4, 32>, go_dd_bp_ql<8, 8, 2, 32>,
+                          go_dd_bp_q<8, 8, 4>,      go_dd_bp_q<16, 16, 2>,     go_dd_bp_q<8, 8, 2>,
+                          go_dd_bp_q<16, 16, 1>,    go_dd_bp_q<8, 8, 1>};
 constexpr int kNDdBpQ = sizeof(kDdBpQ) / sizeof(kDdBpQ[0]);
 
 // The Siddon and DD gather kernels address one slice's image plane, ray table

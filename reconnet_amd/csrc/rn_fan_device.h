@@ -120,6 +120,53 @@ __device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned
     for (int q = 0; q < S; ++q) val[q] = vscale(val[q], inv);
 }
 
+// seg_avg_rows for G rows staged in LDS: row q of `st` holds bins wlo ..
+// wlo + CAP - 1 of group q's row; the arithmetic is that of seg_avg_rows
+// (the caller's window covers every bin it reads).
+template <int G, int CAP>
+__device__ __forceinline__ void seg_avg_staged(const float4* st, int wlo, int n, float lo, float hi, float4 (&val)[G])
+{
+    const float w = hi - lo;
+    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
+    #pragma unroll
+    for (int q = 0; q < G; ++q) val[q] = vzero<float4>();
+    if (!(h > l)) {
+        if (w > 0.0f || !(lo >= 0.0f && lo < (float)n)) return;
+        const int t = (int)lo - wlo;
+        #pragma unroll
+        for (int q = 0; q < G; ++q) val[q] = st[q * CAP + t];
+        return;
+    }
+    const int m0 = (int)l;
+    const int m1 = min((int)h, n - 1);
+    for (int m = m0; m <= m1; ++m) {
+        const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
+        const int t = m - wlo;
+        #pragma unroll
+        for (int q = 0; q < G; ++q) vmadd(val[q], st[q * CAP + t], ph - pl);
+    }
+    const float inv = 1.0f / w;
+    #pragma unroll
+    for (int q = 0; q < G; ++q) val[q] = vscale(val[q], inv);
+}
+
+// The bins seg_avg_rows reads for the interval [lo, hi] of a row of n bins:
+// m0 .. m1, or m0 = n, m1 = -1 when it reads none.
+__device__ __forceinline__ void seg_avg_bins(int n, float lo, float hi, int& m0, int& m1)
+{
+    const float w = hi - lo;
+    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
+    if (h > l) {
+        m0 = (int)l;
+        m1 = min((int)h, n - 1);
+    } else if (!(w > 0.0f) && lo >= 0.0f && lo < (float)n) {
+        m0 = m1 = (int)lo;
+    } else {
+        m0 = n;
+        m1 = -1;
+    }
+}
+
 // Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).
 __device__ __forceinline__ float seg_weight(int m, float lo, float hi)
 {

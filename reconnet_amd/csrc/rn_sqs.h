@@ -55,12 +55,16 @@ private:
     // slices unless RECONNET_SQS_INTERLEAVE=0: images in both orientations,
     // measurement, weights and the subset residual
     bool quads_ = false;
+    // one FP launch over the whole subset (default), or one per view class
+    // (RECONNET_SQS_FP_SPLIT=1)
+    bool fp_split_ = false;
     DevBuf Qxy_, Qyx_, B4_, W4_, R4_;
     // Within each subset the engine orders the views X class first (a view
-    // with |cos| > |sin| walks image rows, the others columns), so that the
-    // interleaved forward projection runs as one launch per class, each
-    // reading one image orientation.  perm_[j] = caller's view index of engine
-    // view j; sub_nx_[k] = number of X views of subset k.
+    // with |cos| > |sin| walks image rows, the others columns): in XCD order
+    // the interleaved forward projection's workgroups of one XCD then walk one
+    // image orientation at a time, and the projection can also run as one
+    // launch per class.  perm_[j] = caller's view index of engine view j;
+    // sub_nx_[k] = number of X views of subset k.
     std::vector<int> perm_, sub_nx_;
     DevBuf perm_dev_;
     int cur_ = 0;

@@ -289,6 +289,8 @@ void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nview
     G_.ensure(ns_ * plane * sizeof(float));
     const char* env = std::getenv("RECONNET_SQS_INTERLEAVE");
     quads_ = ns_ >= 4 && !(env && env[0] == '0');
+    const char* split = std::getenv("RECONNET_SQS_FP_SPLIT");
+    fp_split_ = split && split[0] == '1';
     if (quads_) {
         const size_t nq = (size_t)(ns_ + 3) / 4;
         // padded lines; the pad cells stay zero (only line interiors are written)
@@ -418,8 +420,10 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         float* Xn = X_[1 - cur_].as<float>();
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k], st));
         if (quads_) {
-            // one launch per view class (each reads one image orientation)
-            const int nxv = sub_nx_[k];
+            // one launch over the subset (each wave reads the orientation of
+            // its view's class; the views are ordered X class first), or one
+            // per class
+            const int nxv = fp_split_ ? sub_nx_[k] : cur;
             if (nxv > 0)
                 launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
                                                 start, nxv, B4_.as<float4>(), W4_.as<float4>(), cur, 0, st);

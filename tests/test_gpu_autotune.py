@@ -160,3 +160,20 @@ def test_interleaved_engine_matches_plain_layout(ns, rect, monkeypatch):
         eng.close()
     err = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
     assert err <= 1e-6, err
+
+
+@pytest.mark.parametrize("rect", [False, True])
+def test_fp_per_class_launches_match_one_launch(rect, monkeypatch):
+    # the interleaved engine's forward projection as one launch per subset
+    # (default) and as one launch per view class (RECONNET_SQS_FP_SPLIT=1)
+    outs = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("RECONNET_SQS_FP_SPLIT", flag)
+        eng, x0 = _engine(6, rect=rect)
+        eng.set_image(x0)
+        for _ in range(2):
+            eng.run_pass(beta=1e-3, eps=1e-4, lb=0.0)
+        outs.append(eng.image())
+        eng.close()
+    err = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
+    assert err <= 1e-6, err
