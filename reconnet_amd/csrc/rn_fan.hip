@@ -280,7 +280,8 @@ __global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const
         const bool xv = g.vcls[view] == 0;
         const float lr = xv ? yr : xr;
         const float cell = xv ? (float)cx : (float)cy;
-        if (staged) siddon_gather_staged<NB, G, CAP>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
+        if (staged && g.vwide[view]) siddon_gather_staged<NB, G, CAP, true>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
+        else if (staged) siddon_gather_staged<NB, G, CAP, false>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
         else siddon_gather_bins<float4, NB, G>(g.sray, (unsigned)(view * g.nu) * 16u, pq,
                                                (unsigned)(vi * g.nu) * 16u, g.nu, k0, k1, lr, cell, acc);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

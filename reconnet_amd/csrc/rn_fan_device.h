@@ -426,7 +426,7 @@ __device__ __forceinline__ void siddon_fp_ray_slices(const FanK& g, const T* __r
 // interval outside the image reads pad zeros, and the shares are
 // (min(hi, m0 + 1) - lo, ..) / (hi - lo) without clamping to [0, n].  A
 // zero-width interval gives cell m0 the whole share.
-template <int G, int NC>
+template <int G, int NC, bool WIDE>
 __device__ __forceinline__ void fp_quad_lines_padded(const float4* const (&Pq)[G], int n, int pitch, float4 r,
                                                      float base, int wj0, int wj1, float4 (&acc)[G])
 {
@@ -442,7 +442,7 @@ __device__ __forceinline__ void fp_quad_lines_padded(const float4* const (&Pq)[G
         // clamped into the padded line: a clamped index only reads pad cells
         // (kQPadL, kQPadR >= NC), whatever the shares
         const int m0 = (int)__builtin_amdgcn_fmed3f(f0, -(float)kQPadL, (float)(n + kQPadR - NC));
-        const bool pos = w > 0.0f;
+        const bool pos = WIDE || w > 0.0f;   // WIDE: every interval of the view has w > 0
         const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
         const float o0 = fminf(hi, f0 + 1.0f) - lo;
         float w0, w1, w2 = 0.0f;
@@ -511,9 +511,10 @@ __device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4*
         acc[k] = vzero<float4>();
     }
     if (__all(!active || fabsf(r.y) <= 0.999f)) {
-        fp_quad_lines_padded<G, 2>(Pq, n, pitch, r, base, wj0, wj1, acc);
+        if (g.vwide[view]) fp_quad_lines_padded<G, 2, true>(Pq, n, pitch, r, base, wj0, wj1, acc);
+        else fp_quad_lines_padded<G, 2, false>(Pq, n, pitch, r, base, wj0, wj1, acc);
     } else if (__all(!active || fabsf(r.y) <= 1.999f)) {
-        fp_quad_lines_padded<G, 3>(Pq, n, pitch, r, base, wj0, wj1, acc);
+        fp_quad_lines_padded<G, 3, false>(Pq, n, pitch, r, base, wj0, wj1, acc);
     } else {
         for (int j = wj0; j <= wj1; ++j) {
             const float lr = base + (float)j;
@@ -584,7 +585,8 @@ __device__ __forceinline__ void bin_range(const FanK& g, float xc, float yc, flo
 
 // Length of ray r in cell `cell` of line lr (the weight siddon_fp_ray gives
 // it), computed without branches.  WIDE: the interval's FP32 width is known to
-// be positive (no zero-width case to select).
+// be positive (no zero-width case to select; the same value as the general
+// form then).
 template <bool WIDE>
 __device__ __forceinline__ float siddon_cell_weight(const float4 r, float lr, float cell)
 {
@@ -632,7 +634,7 @@ __device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sr
 // row 0 of `st` holds the view's ray table and row 1 + q group q's values,
 // CAP bins per row from the window's first bin; t0 .. t1 are the pixel's bins
 // relative to that first bin.
-template <int NB, int G, int CAP>
+template <int NB, int G, int CAP, bool WIDE>
 __device__ __forceinline__ void siddon_gather_staged(const float4* st, int t0, int t1, float lr, float cell,
                                                      float4 (&acc)[G])
 {
@@ -640,13 +642,13 @@ __device__ __forceinline__ void siddon_gather_staged(const float4* st, int t0, i
     for (int j = 0; j < NB; ++j) {
         if (t0 + j <= t1) {
             const int t = t0 + j;
-            const float wt = siddon_cell_weight<false>(st[t], lr, cell);
+            const float wt = siddon_cell_weight<WIDE>(st[t], lr, cell);
             #pragma unroll
             for (int q = 0; q < G; ++q) vmadd(acc[q], st[(1 + q) * CAP + t], wt);
         }
     }
     for (int t = t0 + NB; t <= t1; ++t) {
-        const float wt = siddon_cell_weight<false>(st[t], lr, cell);
+        const float wt = siddon_cell_weight<WIDE>(st[t], lr, cell);
         #pragma unroll
         for (int q = 0; q < G; ++q) vmadd(acc[q], st[(1 + q) * CAP + t], wt);
     }
