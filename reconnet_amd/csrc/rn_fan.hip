@@ -124,7 +124,7 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp(FanK g, const T* __re
 // Quad-layout Siddon FP + residual: each lane carries G groups of 4 slices
 // (one float4 load per cell and group); the grid's slice index counts groups
 // of G quads.  res = w * (Ax - meas) per component.
-template <int BLK, int XCD, int G>
+template <int BLK, int XCD, int G, int H>
 __global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* __restrict__ Qyx,
                                                          const float4* __restrict__ Qxy, float4* __restrict__ res,
                                                          int view0, int nviews, int nquads,
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* _
     if (XCD > 0) {
         const int per = gridDim.x / XCD;
         const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
-        const int tiles = (g.nu + BLK - 1) / BLK;
+        const int tiles = (g.nu + BLK / H - 1) / (BLK / H);
         tile = l % tiles;
         vi = (l / tiles) % nviews;
         sg = l / (tiles * nviews);
@@ -145,15 +145,16 @@ __global__ __launch_bounds__(BLK) void k_siddon_fan_fp_q(FanK g, const float4* _
         vi = blockIdx.y;
         sg = blockIdx.z;
     }
-    const int iu = tile * BLK + threadIdx.x;
+    // H > 1: each wave carries 64 / H rays H times (one part of the lines each)
+    const int iu = tile * (BLK / H) + (int)(threadIdx.x >> 6) * (64 / H) + (int)(threadIdx.x & (64 / H - 1));
     const bool active = iu < g.nu;
     const int view = view0 + vi;
     const int q0 = sg * G;
     const int nq = min(G, nquads - q0);
     float4 acc[G];
-    siddon_fp_ray_quads<G>(g, Qyx + (long)q0 * g.ny * qpitch(g.nx), Qxy + (long)q0 * g.nx * qpitch(g.ny), nq, view,
+    siddon_fp_ray_quads<G, H>(g, Qyx + (long)q0 * g.ny * qpitch(g.nx), Qxy + (long)q0 * g.nx * qpitch(g.ny), nq, view,
                            active ? iu : g.nu - 1, active, acc);
-    if (!active) return;
+    if (!active || (threadIdx.x & 63) >= 64 / H) return;
     #pragma unroll
     for (int k = 0; k < G; ++k) {
         if (k >= nq) break;
@@ -896,17 +897,17 @@ void require_offsets32(const FanK& g, int nviews, size_t elem)
                "image plane, ray table or sinogram of one slice exceeds 2 GiB");
 }
 
-template <int BLK, int XCD, int G>
+template <int BLK, int XCD, int G, int H = 1>
 void go_siddon_fp_q(const FanK& g, const float4* Qyx, const float4* Qxy, float4* res, int nquads, int view0,
                     int nviews, const float4* meas, const float4* w, int res_vstride, int res_voff, hipStream_t st)
 {
     const int ngroups = cdiv(nquads, G);
-    const long nblk = (long)cdiv(g.nu, BLK) * nviews * ngroups;
+    const long nblk = (long)cdiv(g.nu, BLK / H) * nviews * ngroups;
     if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
-        hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, XCD, G>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Qyx, Qxy,
+        hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, XCD, G, H>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, Qyx, Qxy,
                            res, view0, nviews, nquads, meas, w, g.nview, res_vstride, res_voff);
     } else {
-        hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, 0, G>), dim3(cdiv(g.nu, BLK), nviews, ngroups), dim3(BLK), 0, st,
+        hipLaunchKernelGGL((k_siddon_fan_fp_q<BLK, 0, G, H>), dim3(cdiv(g.nu, BLK / H), nviews, ngroups), dim3(BLK), 0, st,
                            g, Qyx, Qxy, res, view0, nviews, nquads, meas, w, g.nview, res_vstride, res_voff);
     }
 }
@@ -929,11 +930,15 @@ void go_siddon_bp_ql(const FanK& g, const float4* prj, float* img, int ns, int v
 using SidFpQFn = void (*)(const FanK&, const float4*, const float4*, float4*, int, int, int, const float4*,
                           const float4*, int, int, hipStream_t);
 using SidBpQFn = void (*)(const FanK&, const float4*, float*, int, int, int, hipStream_t);
-// G: groups of 4 slices per lane; candidates in order of preference (the tuner
-// keeps the first within 3 % of the fastest)
-const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<128, kXcds, 1>, go_siddon_fp_q<64, kXcds, 1>,
-                            go_siddon_fp_q<256, kXcds, 1>, go_siddon_fp_q<128, 0, 1>,
-                            go_siddon_fp_q<64, 0, 1>,      go_siddon_fp_q<128, kXcds, 2>,
+// G: groups of 4 slices per lane; H: parts of the line range per ray (the
+// split shortens the waves of a launch that would leave a partial second
+// round on the chip); candidates in order of preference (the tuner keeps the
+// first within 3 % of the fastest)
+const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<64, kXcds, 1, 2>,  go_siddon_fp_q<128, kXcds, 1, 2>,
+                            go_siddon_fp_q<256, kXcds, 1, 2>, go_siddon_fp_q<64, kXcds, 1, 4>,
+                            go_siddon_fp_q<128, kXcds, 1>,    go_siddon_fp_q<64, kXcds, 1>,
+                            go_siddon_fp_q<256, kXcds, 1>,    go_siddon_fp_q<128, 0, 1>,
+                            go_siddon_fp_q<64, 0, 1>,         go_siddon_fp_q<128, kXcds, 2>,
                             go_siddon_fp_q<128, 0, 2>};
 const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 21>,  go_siddon_bp_ql<8, 8, 2, 8, 16>,
                             go_siddon_bp_ql<8, 32, 2, 8, 21>, go_siddon_bp_ql<8, 8, 3, 8, 21>,

@@ -476,8 +476,11 @@ __device__ __forceinline__ void fp_quad_lines_padded(const float4* const (&Pq)[G
 // X views, Qxy for Y views; groups qstride cells apart, groups k >= nq read
 // group nq - 1): the line range, wave-uniform loop and final scaling of
 // siddon_fp_ray_slices with the padded line loop above (rays with |B| >= 2,
-// none in ordinary geometries, take the clamped general path).
-template <int G>
+// none in ordinary geometries, take the clamped general path).  H = 2 or 4:
+// the H parts of 64 / H lanes of the wave carry the same rays and walk
+// consecutive parts of the wave's line range; their sums are added across the
+// parts (every lane then holds the ray's total).
+template <int G, int H = 1>
 __device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4* __restrict__ Qyx,
                                                     const float4* __restrict__ Qxy, int nq, int view, int iu,
                                                     bool active, float4 (&acc)[G])
@@ -502,8 +505,16 @@ __device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4*
         j1 = -1;
     }
     if (!active) { j0 = nl; j1 = -1; }
-    const int wj0 = wave_min(j0);
-    const int wj1 = wave_max(j1);
+    int wj0 = wave_min(j0);
+    int wj1 = wave_max(j1);
+    static_assert(H == 1 || H == 2 || H == 4, "parts of a wave");
+    const int len = wj1 - wj0 + 1;
+    if (H > 1 && len > 0) {
+        const int part = (int)(threadIdx.x & 63) / (64 / H);
+        const int a = wj0 + len * part / H, b = wj0 + len * (part + 1) / H - 1;
+        wj0 = a;
+        wj1 = b;
+    }
     const float4* Pq[G];                 // wave-uniform group bases at the start of padded line 0
     #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -523,6 +534,16 @@ __device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4*
             for (int k = 0; k < G; ++k)
                 vadd(acc[k], siddon_line_sum<float4>(Pq[k] + (long)j * pitch + kQPadL, n, fminf(ea, eb),
                                                      fmaxf(ea, eb)));
+        }
+    }
+    #pragma unroll
+    for (int o = 32; o >= 64 / H; o >>= 1) {
+        #pragma unroll
+        for (int k = 0; k < G; ++k) {
+            acc[k].x += __shfl_xor(acc[k].x, o);
+            acc[k].y += __shfl_xor(acc[k].y, o);
+            acc[k].z += __shfl_xor(acc[k].z, o);
+            acc[k].w += __shfl_xor(acc[k].w, o);
         }
     }
     #pragma unroll
