@@ -195,24 +195,32 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp_q(FanK g, const float
 
 // k_siddon_fan_bp_q with each wave's bin window staged in LDS.  Per view the
 // wave's 64 pixels need the bins [wlo, whi] (the lane minimum and maximum of
-// their ranges); the view's ray table and the G groups' values over the CAP
+// their ranges); the G groups' values and the view's ray table over the CAP
 // bins from wlo are loaded once per wave with coalesced dwordx4 loads, issued
 // one view ahead into registers, and written to the wave's own LDS rows, from
 // which every pixel gathers its bins: a fraction of the per-pixel gathers'
 // vector-memory traffic.  A wave touches only its own rows and the LDS
 // instructions of one wave execute in order, so no barrier is needed; the
 // fences keep the compiler from moving LDS accesses across the handover.
-// Views whose window is wider than CAP bins take the direct gathers.
-template <int BX, int BY, int NB, int G, int CAP, int OCC>
-__global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const float4* __restrict__ prj,
+// Views whose window is wider than CAP bins take the direct gathers.  A wave
+// clears its rows once at the start, so that a view with an empty window
+// (every lane then takes weight 0 from slot 0) reads zeros or an earlier
+// view's staged bins, never unwritten memory.  The group rows are addressed
+// as 32-bit byte offsets from the groups' base (the launcher checks that G
+// groups fit).
+template <int BX, int BY, int NB, int G, int CAP>
+__global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(FanK g, const float4* __restrict__ prj,
                                                           float* __restrict__ img, int view0, int nviews, int ns)
 {
-    static_assert((BX * BY) % 64 == 0, "whole waves");
-    constexpr int kRows = G + 1, kLoads = (kRows * CAP + 63) / 64;
+    static_assert((BX * BY) % 64 == 0 && CAP <= 64, "whole waves, one load for the ray table row");
+    // LDS rows 0 .. G - 1: the groups; row G: the ray table
+    constexpr int kRows = G + 1, kLoads = (G * CAP + 63) / 64;
     __shared__ float4 stage[(BX * BY / 64) * kRows * CAP];
     const int tid = threadIdx.y * BX + threadIdx.x;
     const int lane = tid & 63;
     float4* st = stage + (tid >> 6) * kRows * CAP;
+    for (int i = lane; i < kRows * CAP; i += 64) st[i] = vzero<float4>();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int iy = blockIdx.x * BX + threadIdx.x;
     const int ix = blockIdx.y * BY + threadIdx.y;
     const bool inside = ix < g.nx && iy < g.ny;   // lanes outside stage and gather nothing
@@ -226,18 +234,21 @@ __global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const
     const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
     const float inv_da = 1.0f / g.da;
     const float yr = (float)cy - 0.5f * g.ny, xr = (float)cx - 0.5f * g.nx;
-    // staging load j of this lane: element i = 64 j + lane of the rows, that
-    // is row i / CAP, bin i % CAP of the window
+    // group staging load j of this lane: element i = 64 j + lane of the group
+    // rows, that is group i / CAP, bin i % CAP of the window (LDS slot i), at
+    // byte offset roff[j] + (view row + bin) * 16 from P; the ray table row is
+    // one more load by the first CAP lanes
     int bj[kLoads];
-    const float4* src[kLoads];
+    unsigned roff[kLoads];
     bool has[kLoads];
     #pragma unroll
     for (int j = 0; j < kLoads; ++j) {
         const int i = j * 64 + lane, r = i / CAP;
         bj[j] = i % CAP;
-        has[j] = r < kRows;
-        src[j] = r == 0 ? g.sray + (long)view0 * g.nu : P + (long)min(r - 1, nq - 1) * sstride;
+        has[j] = r < G;
+        roff[j] = (unsigned)(min(r, nq - 1) * sstride) * 16u;
     }
+    const float4* const sray_v = g.sray + (long)view0 * g.nu;
     const float4* pq[G];
     float4 acc[G];
     #pragma unroll
@@ -245,7 +256,7 @@ __global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const
         pq[q] = P + (long)min(q, nq - 1) * sstride;
         acc[q] = vzero<float4>();
     }
-    float4 rg[kLoads];
+    float4 rg[kLoads], rr;
     // the pixel's bins of view vi (empty outside the image or the detector)
     auto range = [&](int vi, int& k0, int& k1) {
         if (g.unit_angle) bin_range<true>(g, xc, yc, rpix, inv_da, view0 + vi, k0, k1);
@@ -255,7 +266,8 @@ __global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const
     auto issue = [&](int vi, int wlo) {
         #pragma unroll
         for (int j = 0; j < kLoads; ++j)
-            if (has[j]) rg[j] = ld_at(src[j], (unsigned)(vi * g.nu + min(wlo + bj[j], g.nu - 1)) * 16u);
+            if (has[j]) rg[j] = ld_at(P, roff[j] + (unsigned)(vi * g.nu + min(wlo + bj[j], g.nu - 1)) * 16u);
+        if (lane < CAP) rr = ld_at(sray_v, (unsigned)(vi * g.nu + min(wlo + lane, g.nu - 1)) * 16u);
     };
     int k0, k1;
     range(0, k0, k1);
@@ -268,6 +280,7 @@ __global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const
             #pragma unroll
             for (int j = 0; j < kLoads; ++j)
                 if (has[j]) st[j * 64 + lane] = rg[j];
+            if (lane < CAP) st[G * CAP + lane] = rr;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         int n0 = g.nu, n1 = -1, nlo = g.nu, nhi = -1;
@@ -281,10 +294,15 @@ __global__ __launch_bounds__(BX * BY, OCC) void k_siddon_fan_bp_ql(FanK g, const
         const bool xv = g.vcls[view] == 0;
         const float lr = xv ? yr : xr;
         const float cell = xv ? (float)cx : (float)cy;
-        if (staged && g.vwide[view]) siddon_gather_staged<NB, G, CAP, true>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
-        else if (staged) siddon_gather_staged<NB, G, CAP, false>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
-        else siddon_gather_bins<float4, NB, G>(g.sray, (unsigned)(view * g.nu) * 16u, pq,
-                                               (unsigned)(vi * g.nu) * 16u, g.nu, k0, k1, lr, cell, acc);
+        if (staged && g.vwide[view]) {
+            siddon_gather_staged<NB, G, CAP, true>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
+        } else if (staged) {
+            siddon_gather_staged<NB, G, CAP, false>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
+        } else {
+            // (one bin without branches: fewer registers for the rare wide window)
+            siddon_gather_bins<float4, 1, G>(g.sray, (unsigned)(view * g.nu) * 16u, pq, (unsigned)(vi * g.nu) * 16u,
+                                             g.nu, k0, k1, lr, cell, acc);
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         k0 = n0; k1 = n1; wlo = nlo; whi = nhi;
     }
@@ -425,8 +443,8 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_q(FanK g, const float4* _
 // registers, written to the wave's own LDS rows and averaged from there with
 // the arithmetic of dd_bp_pixel.  Views whose window is wider than CAP bins
 // read the rows directly.
-template <int BX, int BY, int G, int CAP>
-__global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_ql(FanK g, const float4* __restrict__ wprj,
+template <int BX, int BY, int G, int CAP, int NB>
+__global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_dd_fan_bp_ql(FanK g, const float4* __restrict__ wprj,
                                                       float* __restrict__ img, int ns, int fbp)
 {
     static_assert((BX * BY) % 64 == 0, "whole waves");
@@ -450,7 +468,8 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_ql(FanK g, const float4* 
         rows[q] = P + (long)min(q, nq - 1) * sstride;
         acc[q] = vzero<float4>();
     }
-    // staging load j of this lane: element i = 64 j + lane, row i / CAP, bin i % CAP
+    // staging load j of this lane: element i = 64 j + lane, row i / CAP, bin
+    // i % CAP (LDS slot i)
     int bj[kLoads];
     const float4* src[kLoads];
     bool has[kLoads];
@@ -494,8 +513,9 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_ql(FanK g, const float4* 
     int wlo = wave_min(m0), whi = wave_max(m1);
     if (whi >= wlo && whi - wlo < CAP) issue(0, wlo);
     for (int view = 0; view < g.nview; ++view) {
-        const bool staged = whi - wlo < CAP;   // also when the window is empty
-        if (staged && whi >= wlo) {
+        const bool any = whi >= wlo;            // some lane reads a bin of this view
+        const bool staged = whi - wlo < CAP;
+        if (any && staged) {
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             #pragma unroll
             for (int j = 0; j < kLoads; ++j)
@@ -510,10 +530,15 @@ __global__ __launch_bounds__(BX * BY) void k_dd_fan_bp_ql(FanK g, const float4* 
             nhi = wave_max(n1);
             if (nhi >= nlo && nhi - nlo < CAP) issue(view + 1, nlo);
         }
-        if (inside) {
+        if (any) {
             float4 val[G];
-            if (staged) seg_avg_staged<G, CAP>(st, wlo, g.nu, lo, hi, val);
-            else seg_avg_rows<float4, G>(rows, (unsigned)(view * g.nu) * 16u, g.nu, lo, hi, val);
+            if (staged) {
+                // lanes outside the image read nothing: their interval is made empty
+                seg_avg_staged<G, CAP, NB>(st, wlo, g.nu, inside ? lo : -2.0f, inside ? hi : -1.0f, val);
+            } else {
+                if (inside) seg_avg_rows<float4, G>(rows, (unsigned)(view * g.nu) * 16u, g.nu, lo, hi, val);
+                else for (int q = 0; q < G; ++q) val[q] = vzero<float4>();
+            }
             #pragma unroll
             for (int q = 0; q < G; ++q) vadd(acc[q], fbp ? vscale(val[q], fw) : val[q]);
         }
@@ -874,17 +899,17 @@ void go_dd_bp_q(const FanK& g, const float4* wprj, float* img, int ns, int fbp, 
     hipLaunchKernelGGL((k_dd_fan_bp_q<BX, BY, G>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
                        dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
 }
-template <int BX, int BY, int G, int CAP>
+template <int BX, int BY, int G, int CAP, int NB = 2>
 void go_dd_bp_ql(const FanK& g, const float4* wprj, float* img, int ns, int fbp, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dd_fan_bp_ql<BX, BY, G, CAP>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
+    hipLaunchKernelGGL((k_dd_fan_bp_ql<BX, BY, G, CAP, NB>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)),
                        dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
 }
 using DdBpQFn = void (*)(const FanK&, const float4*, float*, int, int, hipStream_t);
-const DdBpQFn kDdBpQ[] = {go_dd_bp_ql<8, 8, 8, 32>, go_dd_bp_ql<8, 32, 8, 32>, go_dd_bp_ql<8, 32, 8, 24>,
-                          go_dd_bp_ql<8, 8, 8, 24>, go_dd_bp_ql<8, 8, 4, 32>, go_dd_bp_ql<8, 8, 2, 32>,
-                          go_dd_bp_q<8, 8, 4>,      go_dd_bp_q<16, 16, 2>,     go_dd_bp_q<8, 8, 2>,
-                          go_dd_bp_q<16, 16, 1>,    go_dd_bp_q<8, 8, 1>};
+const DdBpQFn kDdBpQ[] = {go_dd_bp_ql<8, 8, 8, 24>, go_dd_bp_ql<8, 8, 8, 32>, go_dd_bp_ql<8, 32, 8, 32>,
+                          go_dd_bp_ql<8, 32, 8, 24>, go_dd_bp_ql<8, 8, 4, 32>, go_dd_bp_ql<8, 8, 2, 32>,
+                          go_dd_bp_q<8, 8, 4>,       go_dd_bp_q<16, 16, 2>,     go_dd_bp_q<8, 8, 2>,
+                          go_dd_bp_q<16, 16, 1>,     go_dd_bp_q<8, 8, 1>};
 constexpr int kNDdBpQ = sizeof(kDdBpQ) / sizeof(kDdBpQ[0]);
 
 // The Siddon and DD gather kernels address one slice's image plane, ray table
@@ -919,10 +944,10 @@ void go_siddon_bp_q(const FanK& g, const float4* prj, float* img, int ns, int vi
                        dim3(BX, BY), 0, st, g, prj, img, view0, nviews, ns);
 }
 
-template <int BX, int BY, int NB, int G, int CAP, int OCC = 1>
+template <int BX, int BY, int NB, int G, int CAP>
 void go_siddon_bp_ql(const FanK& g, const float4* prj, float* img, int ns, int view0, int nviews, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_siddon_fan_bp_ql<BX, BY, NB, G, CAP, OCC>),
+    hipLaunchKernelGGL((k_siddon_fan_bp_ql<BX, BY, NB, G, CAP>),
                        dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)), dim3(BX, BY), 0, st, g, prj, img,
                        view0, nviews, ns);
 }
@@ -940,17 +965,16 @@ const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<64, kXcds, 1, 2>,  go_siddon_fp_q<128
                             go_siddon_fp_q<256, kXcds, 1>,    go_siddon_fp_q<128, 0, 1>,
                             go_siddon_fp_q<64, 0, 1>,         go_siddon_fp_q<128, kXcds, 2>,
                             go_siddon_fp_q<128, 0, 2>};
-const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 21>,  go_siddon_bp_ql<8, 8, 2, 8, 16>,
-                            go_siddon_bp_ql<8, 32, 2, 8, 21>, go_siddon_bp_ql<8, 8, 3, 8, 21>,
-                            go_siddon_bp_ql<8, 8, 2, 8, 32>,  go_siddon_bp_ql<8, 8, 2, 4, 21>,
-                            go_siddon_bp_ql<8, 8, 2, 2, 32>,  go_siddon_bp_q<16, 16, 2, 8>,
-                            go_siddon_bp_q<8, 8, 2, 8>,       go_siddon_bp_q<16, 16, 2, 4>,
-                            go_siddon_bp_q<8, 8, 2, 4>,       go_siddon_bp_q<8, 8, 2, 2>,
-                            go_siddon_bp_q<16, 16, 2, 2>,     go_siddon_bp_q<8, 8, 2, 1>,
-                            go_siddon_bp_q<16, 16, 2, 1>,     go_siddon_bp_q<8, 8, 3, 4>,
-                            go_siddon_bp_q<8, 8, 3, 2>};
+const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 16>,  go_siddon_bp_ql<8, 32, 2, 8, 16>,
+                            go_siddon_bp_ql<8, 8, 2, 4, 21>,  go_siddon_bp_ql<8, 8, 2, 2, 32>,
+                            go_siddon_bp_q<16, 16, 2, 8>,     go_siddon_bp_q<8, 8, 2, 8>,
+                            go_siddon_bp_q<16, 16, 2, 4>,     go_siddon_bp_q<8, 8, 2, 4>,
+                            go_siddon_bp_q<8, 8, 2, 2>,       go_siddon_bp_q<16, 16, 2, 2>,
+                            go_siddon_bp_q<8, 8, 2, 1>,       go_siddon_bp_q<16, 16, 2, 1>,
+                            go_siddon_bp_q<8, 8, 3, 4>,       go_siddon_bp_q<8, 8, 3, 2>};
 constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
 constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);
+constexpr int kNSidBpStaged = 4;   // the first four are k_siddon_fan_bp_ql
 
 } // namespace
 
@@ -1011,9 +1035,11 @@ void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, i
 {
     if (nslices <= 0 || nviews <= 0) return;
     require_offsets32(g, nviews, sizeof(float4));
-    const int c = tune_choose({TK_SIDDON_BP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidBpQ, [&](int i) {
-        kSidBpQ[i](g, prj_q, img_xy, nslices, view0, nviews, st);
-    }, st, true);
+    // the LDS-staged candidates address 8 groups' rows with 32-bit offsets
+    const int first = 9L * nviews * g.nu * 16L < 0x7fffffffL ? 0 : kNSidBpStaged;
+    const int c = first + tune_choose({TK_SIDDON_BP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidBpQ - first,
+                                      [&](int i) { kSidBpQ[first + i](g, prj_q, img_xy, nslices, view0, nviews, st); },
+                                      st, true);
     kSidBpQ[c](g, prj_q, img_xy, nslices, view0, nviews, st);
     check_launch("siddon_fan_bp_q");
 }

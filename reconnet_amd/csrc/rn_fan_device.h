@@ -121,33 +121,48 @@ __device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned
 }
 
 // seg_avg_rows for G rows staged in LDS: row q of `st` holds bins wlo ..
-// wlo + CAP - 1 of group q's row; the arithmetic is that of seg_avg_rows
-// (the caller's window covers every bin it reads).
-template <int G, int CAP>
+// wlo + CAP - 1 of group q's row (every slot written: the caller skips views
+// whose window is empty, and the window covers every bin the interval
+// reads).  The same sums in the same order, written without branches around
+// the sums: every lane takes NB bins from its first bin on (bins past its
+// last one re-read its last bin, or slot 0 if it reads none, with weight 0;
+// the degenerate zero-width interval takes its bin with weight 1 and no
+// scaling), the rest in a loop.
+template <int G, int CAP, int NB>
 __device__ __forceinline__ void seg_avg_staged(const float4* st, int wlo, int n, float lo, float hi, float4 (&val)[G])
 {
     const float w = hi - lo;
     const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
+    const bool normal = h > l;
+    const bool point = !normal && !(w > 0.0f) && lo >= 0.0f && lo < (float)n;
+    // (an interval that reads nothing: m0 = wlo, m1 = wlo - 1, so that no
+    // float outside the int range is converted)
+    const int m0 = normal ? (int)l : (point ? (int)lo : wlo);
+    const int m1 = normal ? min((int)h, n - 1) : (point ? m0 : m0 - 1);
+    const int t0 = m0 - wlo, tl = max(m1 - wlo, 0);
     #pragma unroll
-    for (int q = 0; q < G; ++q) val[q] = vzero<float4>();
-    if (!(h > l)) {
-        if (w > 0.0f || !(lo >= 0.0f && lo < (float)n)) return;
-        const int t = (int)lo - wlo;
+    for (int j = 0; j < NB; ++j) {
+        const int m = m0 + j;
+        const bool on = m <= m1;
+        const int t = on ? t0 + j : tl;
+        const float ov = normal ? fminf(h, (float)(m + 1)) - fmaxf(l, (float)m) : 1.0f;
+        const float wt = on ? ov : 0.0f;
         #pragma unroll
-        for (int q = 0; q < G; ++q) val[q] = st[q * CAP + t];
-        return;
+        for (int q = 0; q < G; ++q) {
+            if (j == 0) val[q] = vscale(st[q * CAP + t], wt);
+            else vmadd(val[q], st[q * CAP + t], wt);
+        }
     }
-    const int m0 = (int)l;
-    const int m1 = min((int)h, n - 1);
-    for (int m = m0; m <= m1; ++m) {
+    for (int m = m0 + NB; m <= m1; ++m) {
         const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
-        const int t = m - wlo;
         #pragma unroll
-        for (int q = 0; q < G; ++q) vmadd(val[q], st[q * CAP + t], ph - pl);
+        for (int q = 0; q < G; ++q) vmadd(val[q], st[q * CAP + m - wlo], ph - pl);
     }
-    const float inv = 1.0f / w;
-    #pragma unroll
-    for (int q = 0; q < G; ++q) val[q] = vscale(val[q], inv);
+    if (normal) {
+        const float inv = 1.0f / w;
+        #pragma unroll
+        for (int q = 0; q < G; ++q) val[q] = vscale(val[q], inv);
+    }
 }
 
 // The bins seg_avg_rows reads for the interval [lo, hi] of a row of n bins:
@@ -345,17 +360,29 @@ __device__ __forceinline__ void fp_narrow_lines(const T* const (&Pq)[S], int n, 
 
 // Minimum and maximum of v over the 64 lanes of a wave (every lane active):
 // pair, quad, half-row and row exchanges by DPP, then the four rows' values
-// read into scalar registers, so the result is wave-uniform.
+// read into scalar registers, so the result is wave-uniform.  The exchanges'
+// "old" operand is the operation's identity, so that each one compiles to a
+// single min / max with a DPP source.
 __device__ __forceinline__ int wave_min(int v)
 {
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));    // quad_perm [1, 0, 3, 2]
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));    // quad_perm [2, 3, 0, 1]
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));   // row_half_mirror
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));   // row_mirror
+    constexpr int id = 0x7fffffff;
+    v = min(v, __builtin_amdgcn_update_dpp(id, v, 0xB1, 0xf, 0xf, false));    // quad_perm [1, 0, 3, 2]
+    v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x4E, 0xf, 0xf, false));    // quad_perm [2, 3, 0, 1]
+    v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));   // row_half_mirror
+    v = min(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));   // row_mirror
     return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
                min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
-__device__ __forceinline__ int wave_max(int v) { return -wave_min(-v); }
+__device__ __forceinline__ int wave_max(int v)
+{
+    constexpr int id = (int)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(id, v, 0xB1, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x4E, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xf, 0xf, false));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
 
 // One ray (bin iu of `view`) through S slices of one geometry: slice q reads
 // planes Pyx + q * plane / Pxy + q * plane (slices q >= nsl read slice nsl - 1;
@@ -652,26 +679,29 @@ __device__ __forceinline__ void siddon_gather_bins(const float4* __restrict__ sr
 }
 
 // siddon_gather_bins for G slice groups from a wave's staged window in LDS:
-// row 0 of `st` holds the view's ray table and row 1 + q group q's values,
-// CAP bins per row from the window's first bin; t0 .. t1 are the pixel's bins
-// relative to that first bin.
+// row q of `st` holds group q's values and row G the view's ray table, CAP
+// bins per row from the window's first bin (every slot holds written values:
+// the caller clears the rows before its first window); t0 .. t1 are the pixel's bins
+// relative to that first bin.  The first NB bins are taken by every lane with
+// no branch around the accumulators: a lane past t1 reads slot 0 (a bin of
+// this view's window) with weight 0, which keeps the kernel in registers.
+// The rest in a loop.
 template <int NB, int G, int CAP, bool WIDE>
 __device__ __forceinline__ void siddon_gather_staged(const float4* st, int t0, int t1, float lr, float cell,
                                                      float4 (&acc)[G])
 {
     #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        if (t0 + j <= t1) {
-            const int t = t0 + j;
-            const float wt = siddon_cell_weight<WIDE>(st[t], lr, cell);
-            #pragma unroll
-            for (int q = 0; q < G; ++q) vmadd(acc[q], st[(1 + q) * CAP + t], wt);
-        }
+        const bool on = t0 + j <= t1;
+        const int t = on ? t0 + j : 0;
+        const float wt = on ? siddon_cell_weight<WIDE>(st[G * CAP + t], lr, cell) : 0.0f;
+        #pragma unroll
+        for (int q = 0; q < G; ++q) vmadd(acc[q], st[q * CAP + t], wt);
     }
     for (int t = t0 + NB; t <= t1; ++t) {
-        const float wt = siddon_cell_weight<WIDE>(st[t], lr, cell);
+        const float wt = siddon_cell_weight<WIDE>(st[G * CAP + t], lr, cell);
         #pragma unroll
-        for (int q = 0; q < G; ++q) vmadd(acc[q], st[(1 + q) * CAP + t], wt);
+        for (int q = 0; q < G; ++q) vmadd(acc[q], st[q * CAP + t], wt);
     }
 }
 

@@ -177,3 +177,78 @@ def test_fp_per_class_launches_match_one_launch(rect, monkeypatch):
         eng.close()
     err = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
     assert err <= 1e-6, err
+
+
+def _outside_fan_geometry():
+    # a 128 x 128 image of 1 mm pixels under a 64-bin fan that covers a disc
+    # of about 21 mm: most 8 x 8 pixel waves meet no bin in any view
+    from reconnet_amd.geometry import FanGeometry
+
+    return FanGeometry(nx=128, ny=128, dx=1.0, dy=1.0, nu=64, nview=48, da=1.2 / 1085.6, dsd=1085.6, dso=595.0)
+
+
+def test_staged_siddon_bp_reads_nothing_outside_its_window(monkeypatch):
+    # The LDS-staged BP (candidate 0 of the interleaved BP) after a launch that
+    # left non-finite values in LDS: waves with an empty bin window (and lanes
+    # past their last bin) must read nothing of it.  Checked against the plain
+    # one-slice layout, whose gathers do not use LDS.
+    import torch
+    from reconnet_amd._lib import autotune_pin
+    from reconnet_amd.sqs import OsSqsTV
+
+    geo = _outside_fan_geometry()
+    ns = 8
+    rng = np.random.default_rng(21)
+    b = torch.from_numpy(rng.uniform(0, 2, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
+    w = torch.from_numpy(rng.uniform(0.5, 1, (ns, geo.nu, geo.nview)).astype(np.float32)).cuda()
+    x0 = torch.from_numpy(rng.uniform(0, 0.05, (ns, geo.nx, geo.ny)).astype(np.float32)).cuda()
+    outs = []
+    try:
+        autotune_pin("siddon_bp_q", 0)
+        for flag in ("0", "1"):
+            monkeypatch.setenv("RECONNET_SQS_INTERLEAVE", flag)
+            if flag == "1":
+                # fill the LDS of every CU with the staged windows of a NaN residual
+                bad = OsSqsTV(geo, 2, ns)
+                bad.set_data(torch.full_like(b, float("nan")), w)
+                bad.set_norm(torch.ones_like(x0))
+                bad.set_image(x0)
+                bad.run_pass(beta=0.0, eps=1e-4, lb=0.0)
+                torch.cuda.synchronize()
+                bad.close()
+            eng = OsSqsTV(geo, 2, ns)
+            eng.set_data(b, w)
+            eng.set_norm(torch.ones_like(x0) * 50.0)
+            eng.set_image(x0)
+            eng.run_pass(beta=0.0, eps=1e-4, lb=0.0)
+            outs.append(eng.image())
+            eng.close()
+    finally:
+        autotune_pin("siddon_bp_q", None)
+    assert bool(torch.isfinite(outs[1]).all())
+    err = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
+    assert err <= 1e-6, err
+
+
+def test_staged_dd_fbp_reads_nothing_outside_its_window():
+    # the LDS-staged DD FBP backprojection (candidate 0) on 8 slices after a
+    # launch on a NaN sinogram, against the oracle
+    import oracle as orc
+    from reconnet_amd._lib import autotune_pin
+    from reconnet_amd.geometry import fan_angles
+    from tests.helpers import recon_for, rel_l2
+
+    geo = _outside_fan_geometry()
+    deg = fan_angles(geo.nview)
+    sino = np.random.default_rng(22).uniform(0, 1, (8, geo.nu, geo.nview, 1, 1)).astype(np.float32)
+    r = recon_for(geo)
+    try:
+        autotune_pin("dd_fbp_q", 0)
+        r.cDDFanBackprojection4d(np.full_like(sino, np.nan), type_projector=1)
+        out = r.cDDFanBackprojection4d(sino, type_projector=1)
+    finally:
+        autotune_pin("dd_fbp_q", None)
+    ref = orc.dd_fan_bp(sino, deg, geo.oracle_geo(), (geo.nx, geo.ny, 1), type_projector=1)
+    assert np.isfinite(out).all()
+    for s in range(8):
+        assert rel_l2(out[s], ref[s]) <= 1e-5, s
