@@ -195,19 +195,21 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp_q(FanK g, const float
 
 // k_siddon_fan_bp_q with each wave's bin window staged in LDS.  Per view the
 // wave's 64 pixels need the bins [wlo, whi] (the lane minimum and maximum of
-// their ranges); the G groups' values and the view's ray table over the CAP
-// bins from wlo are loaded once per wave with coalesced dwordx4 loads, issued
-// one view ahead into registers, and written to the wave's own LDS rows, from
-// which every pixel gathers its bins: a fraction of the per-pixel gathers'
-// vector-memory traffic.  A wave touches only its own rows and the LDS
-// instructions of one wave execute in order, so no barrier is needed; the
-// fences keep the compiler from moving LDS accesses across the handover.
-// Views whose window is wider than CAP bins take the direct gathers.  A wave
-// clears its rows once at the start, so that a view with an empty window
-// (every lane then takes weight 0 from slot 0) reads zeros or an earlier
-// view's staged bins, never unwritten memory.  The group rows are addressed
-// as 32-bit byte offsets from the groups' base (the launcher checks that G
-// groups fit).
+// their ranges); the G groups' values and the view's ray table over CAP bins
+// at a time are loaded once per wave with coalesced dwordx4 loads and written
+// to the wave's own LDS rows, from which every pixel gathers its bins: a
+// fraction of the per-pixel gathers' vector-memory traffic.  The first CAP
+// bins of a view are loaded one view ahead into registers; a wider window
+// takes further chunks of CAP bins, loaded in turn (the next view's loads are
+// then issued after them).  Every view runs the one gather code (no branch
+// around the accumulators, which would cost copies of all of them).  A wave
+// touches only its own rows and the LDS instructions of one wave execute in
+// order, so no barrier is needed; the fences keep the compiler from moving
+// LDS accesses across the handovers.  A wave clears its rows at the start, so
+// that a view with an empty window (every lane then takes weight 0 from slot
+// 0) reads zeros or an earlier view's bins, never unwritten memory.  The
+// group rows are addressed as 32-bit byte offsets from the groups' base (the
+// launcher checks that G groups fit).
 template <int BX, int BY, int NB, int G, int CAP>
 __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(FanK g, const float4* __restrict__ prj,
                                                           float* __restrict__ img, int view0, int nviews, int ns)
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(Fa
     const float inv_da = 1.0f / g.da;
     const float yr = (float)cy - 0.5f * g.ny, xr = (float)cx - 0.5f * g.nx;
     // group staging load j of this lane: element i = 64 j + lane of the group
-    // rows, that is group i / CAP, bin i % CAP of the window (LDS slot i), at
+    // rows, that is group i / CAP, bin i % CAP of the chunk (LDS slot i), at
     // byte offset roff[j] + (view row + bin) * 16 from P; the ray table row is
     // one more load by the first CAP lanes
     int bj[kLoads];
@@ -249,13 +251,9 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(Fa
         roff[j] = (unsigned)(min(r, nq - 1) * sstride) * 16u;
     }
     const float4* const sray_v = g.sray + (long)view0 * g.nu;
-    const float4* pq[G];
     float4 acc[G];
     #pragma unroll
-    for (int q = 0; q < G; ++q) {
-        pq[q] = P + (long)min(q, nq - 1) * sstride;
-        acc[q] = vzero<float4>();
-    }
+    for (int q = 0; q < G; ++q) acc[q] = vzero<float4>();
     float4 rg[kLoads], rr;
     // the pixel's bins of view vi (empty outside the image or the detector)
     auto range = [&](int vi, int& k0, int& k1) {
@@ -263,46 +261,49 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(Fa
         else bin_range<false>(g, xc, yc, rpix, inv_da, view0 + vi, k0, k1);
         if (!inside || k1 < k0) { k0 = g.nu; k1 = -1; }
     };
-    auto issue = [&](int vi, int wlo) {
+    // loads of the CAP bins from `base` of view vi, and their LDS handover
+    auto issue = [&](int vi, int base) {
         #pragma unroll
         for (int j = 0; j < kLoads; ++j)
-            if (has[j]) rg[j] = ld_at(P, roff[j] + (unsigned)(vi * g.nu + min(wlo + bj[j], g.nu - 1)) * 16u);
-        if (lane < CAP) rr = ld_at(sray_v, (unsigned)(vi * g.nu + min(wlo + lane, g.nu - 1)) * 16u);
+            if (has[j]) rg[j] = ld_at(P, roff[j] + (unsigned)(vi * g.nu + min(base + bj[j], g.nu - 1)) * 16u);
+        if (lane < CAP) rr = ld_at(sray_v, (unsigned)(vi * g.nu + min(base + lane, g.nu - 1)) * 16u);
+    };
+    auto handover = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        #pragma unroll
+        for (int j = 0; j < kLoads; ++j)
+            if (has[j]) st[j * 64 + lane] = rg[j];
+        if (lane < CAP) st[G * CAP + lane] = rr;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     };
     int k0, k1;
     range(0, k0, k1);
     int wlo = wave_min(k0), whi = wave_max(k1);
-    if (whi >= wlo && whi - wlo < CAP) issue(0, wlo);
+    if (whi >= wlo) issue(0, wlo);
     for (int vi = 0; vi < nviews; ++vi) {
-        const bool staged = whi - wlo < CAP;   // also when the window is empty
-        if (staged && whi >= wlo) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            #pragma unroll
-            for (int j = 0; j < kLoads; ++j)
-                if (has[j]) st[j * 64 + lane] = rg[j];
-            if (lane < CAP) st[G * CAP + lane] = rr;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        }
+        const bool any = whi >= wlo;
+        const int nch = any ? (whi - wlo) / CAP + 1 : 1;   // an empty window: one chunk of weight-0 reads
+        if (any) handover();
         int n0 = g.nu, n1 = -1, nlo = g.nu, nhi = -1;
         if (vi + 1 < nviews) {
             range(vi + 1, n0, n1);
             nlo = wave_min(n0);
             nhi = wave_max(n1);
-            if (nhi >= nlo && nhi - nlo < CAP) issue(vi + 1, nlo);
         }
+        const bool pre = nhi >= nlo;
+        if (pre && nch == 1) issue(vi + 1, nlo);
         const int view = view0 + vi;
         const bool xv = g.vcls[view] == 0;
         const float lr = xv ? yr : xr;
         const float cell = xv ? (float)cx : (float)cy;
-        if (staged && g.vwide[view]) {
-            siddon_gather_staged<NB, G, CAP, true>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
-        } else if (staged) {
-            siddon_gather_staged<NB, G, CAP, false>(st, k0 - wlo, k1 - wlo, lr, cell, acc);
-        } else {
-            // (one bin without branches: fewer registers for the rare wide window)
-            siddon_gather_bins<float4, 1, G>(g.sray, (unsigned)(view * g.nu) * 16u, pq, (unsigned)(vi * g.nu) * 16u,
-                                             g.nu, k0, k1, lr, cell, acc);
+        siddon_gather_staged<NB, G, CAP, false>(st, k0 - wlo, min(k1 - wlo, CAP - 1), lr, cell, acc);
+        for (int c = 1; c < nch; ++c) {
+            const int base = wlo + c * CAP;
+            issue(vi, base);
+            handover();
+            siddon_gather_staged<NB, G, CAP, false>(st, max(k0 - base, 0), min(k1 - base, CAP - 1), lr, cell, acc);
         }
+        if (pre && nch > 1) issue(vi + 1, nlo);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         k0 = n0; k1 = n1; wlo = nlo; whi = nhi;
     }
