@@ -46,32 +46,51 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
 
 // ---------------------------------------------------------------------------
 // layer 1: x (fp32 [B][H][W]) -> relu(conv(x) + b) (bf16 NHWC), w [9][32]
+//
+// One wave per 62 pixels of an image row: lane l holds column x0 + l - 1 of
+// the three input rows (one load per row, zero outside the image), the left
+// and right neighbours come from the adjacent lanes, and lanes 1 .. 62 write
+// their pixel's 32 channels (lanes 0 and 63 only supply neighbours).
 // ---------------------------------------------------------------------------
+constexpr int kConvInTile = 62;
+
 __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, const float* __restrict__ w,
                                                  const float* __restrict__ b, uint16_t* __restrict__ out,
-                                                 int B, int H, int W)
+                                                 int H, int W, int tiles, long nwaves)
 {
     // weights and biases are read at wave-uniform addresses (scalar loads)
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long npx = (long)B * H * W;
-    if (p >= npx) return;
-    const int xw = (int)(p % W), yh = (int)((p / W) % H);
-    const long base = p - (long)yh * W - xw;
-    float v[9];
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= nwaves) return;                  // whole waves
+    const int tile = (int)(wave % tiles);
+    const long row = wave / tiles;               // image * H + yh
+    const int yh = (int)(row % H);
+    const long img = (row - yh) * W;             // first pixel of the image
+    const int xc = tile * kConvInTile + lane - 1;
+    const bool col = xc >= 0 && xc < W;
+    float c[3];
     #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int yy = yh + t / 3 - 1, xx = xw + t % 3 - 1;
-        v[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[base + (long)yy * W + xx] : 0.0f;
+    for (int dy = 0; dy < 3; ++dy) {
+        const int yy = yh + dy - 1;
+        c[dy] = (col && yy >= 0 && yy < H) ? x[img + (long)yy * W + xc] : 0.0f;
     }
+    float v[9];                                  // tap t = 3 dy + dx
+    #pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+        v[3 * dy] = __shfl_up(c[dy], 1);
+        v[3 * dy + 1] = c[dy];
+        v[3 * dy + 2] = __shfl_down(c[dy], 1);
+    }
+    if (lane == 0 || lane == 63 || xc >= W) return;
     uint32_t packed[16];
     #pragma unroll
-    for (int c = 0; c < 32; c += 2) {
-        float z0 = b[c], z1 = b[c + 1];
+    for (int ch = 0; ch < 32; ch += 2) {
+        float z0 = b[ch], z1 = b[ch + 1];
         #pragma unroll
-        for (int t = 0; t < 9; ++t) { z0 = fmaf(w[t * 32 + c], v[t], z0); z1 = fmaf(w[t * 32 + c + 1], v[t], z1); }
-        packed[c / 2] = pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f));
+        for (int t = 0; t < 9; ++t) { z0 = fmaf(w[t * 32 + ch], v[t], z0); z1 = fmaf(w[t * 32 + ch + 1], v[t], z1); }
+        packed[ch / 2] = pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f));
     }
-    uint4* o = reinterpret_cast<uint4*>(out + p * 32);
+    uint4* o = reinterpret_cast<uint4*>(out + (img + (long)yh * W + xc) * 32);
     #pragma unroll
     for (int q = 0; q < 4; ++q) o[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
 }
@@ -512,8 +531,12 @@ int grid_for(long work, int per_block, int cap)
 
 void conv_in(const float* x, const float* w, const float* b, uint16_t* out, int B, int H, int W, hipStream_t st)
 {
-    const long npx = (long)B * H * W;
-    hipLaunchKernelGGL(k_conv_in, dim3(cdiv(npx, 256)), dim3(256), 0, st, x, w, b, out, B, H, W);
+    RN_REQUIRE(B > 0 && H > 0 && W > 0, "conv_in: bad batch or image size");
+    const int tiles = cdiv(W, kConvInTile);
+    const long nwaves = (long)B * H * tiles;
+    RN_REQUIRE(nwaves / 4 < 0x7fffffffL, "conv_in: batch too large for one launch");
+    hipLaunchKernelGGL(k_conv_in, dim3((unsigned)cdiv(nwaves * 64, 256)), dim3(256), 0, st, x, w, b, out, H, W, tiles,
+                       nwaves);
     check_launch("conv_in");
 }
 

@@ -966,8 +966,10 @@ const SidFpQFn kSidFpQ[] = {go_siddon_fp_q<64, kXcds, 1, 2>,  go_siddon_fp_q<128
                             go_siddon_fp_q<256, kXcds, 1>,    go_siddon_fp_q<128, 0, 1>,
                             go_siddon_fp_q<64, 0, 1>,         go_siddon_fp_q<128, kXcds, 2>,
                             go_siddon_fp_q<128, 0, 2>};
-const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 16>,  go_siddon_bp_ql<8, 32, 2, 8, 16>,
-                            go_siddon_bp_ql<8, 8, 2, 4, 21>,  go_siddon_bp_ql<8, 8, 2, 2, 32>,
+const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 24>,  go_siddon_bp_ql<8, 32, 2, 8, 24>,
+                            go_siddon_bp_ql<8, 8, 2, 8, 32>,  go_siddon_bp_ql<8, 8, 2, 8, 21>,
+                            go_siddon_bp_ql<8, 8, 2, 8, 16>,  go_siddon_bp_ql<8, 8, 2, 4, 24>,
+                            go_siddon_bp_ql<8, 8, 2, 2, 32>,
                             go_siddon_bp_q<16, 16, 2, 8>,     go_siddon_bp_q<8, 8, 2, 8>,
                             go_siddon_bp_q<16, 16, 2, 4>,     go_siddon_bp_q<8, 8, 2, 4>,
                             go_siddon_bp_q<8, 8, 2, 2>,       go_siddon_bp_q<16, 16, 2, 2>,
@@ -975,7 +977,7 @@ const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 16>,  go_siddon_bp_ql<8,
                             go_siddon_bp_q<8, 8, 3, 4>,       go_siddon_bp_q<8, 8, 3, 2>};
 constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
 constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);
-constexpr int kNSidBpStaged = 4;   // the first four are k_siddon_fan_bp_ql
+constexpr int kNSidBpStaged = 7;   // the first seven are k_siddon_fan_bp_ql
 
 } // namespace
 
