@@ -49,8 +49,10 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
 //
 // One wave per 62 pixels of an image row: lane l holds column x0 + l - 1 of
 // the three input rows (one load per row, zero outside the image), the left
-// and right neighbours come from the adjacent lanes, and lanes 1 .. 62 write
-// their pixel's 32 channels (lanes 0 and 63 only supply neighbours).
+// and right neighbours come from the adjacent lanes, and lanes 1 .. 62
+// compute their pixel's 32 channels (lanes 0 and 63 only supply neighbours).
+// The wave's 62 x 64 output bytes are gathered in LDS and written with
+// contiguous 16-byte stores, 1 KB per instruction.
 // ---------------------------------------------------------------------------
 constexpr int kConvInTile = 62;
 
@@ -81,18 +83,32 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
         v[3 * dy + 1] = c[dy];
         v[3 * dy + 2] = __shfl_down(c[dy], 1);
     }
-    if (lane == 0 || lane == 63 || xc >= W) return;
-    uint32_t packed[16];
-    #pragma unroll
-    for (int ch = 0; ch < 32; ch += 2) {
-        float z0 = b[ch], z1 = b[ch + 1];
+    __shared__ uint4 sb[4][kConvInTile * 4];   // per wave: 62 pixels x 4 x 16 bytes
+    uint4* ws = sb[(threadIdx.x >> 6) & 3];
+    if (lane >= 1 && lane <= kConvInTile) {
+        uint32_t packed[16];
         #pragma unroll
-        for (int t = 0; t < 9; ++t) { z0 = fmaf(w[t * 32 + ch], v[t], z0); z1 = fmaf(w[t * 32 + ch + 1], v[t], z1); }
-        packed[ch / 2] = pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f));
+        for (int ch = 0; ch < 32; ch += 2) {
+            float z0 = b[ch], z1 = b[ch + 1];
+            #pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                z0 = fmaf(w[t * 32 + ch], v[t], z0);
+                z1 = fmaf(w[t * 32 + ch + 1], v[t], z1);
+            }
+            packed[ch / 2] = pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f));
+        }
+        #pragma unroll
+        for (int q = 0; q < 4; ++q)
+            ws[(lane - 1) * 4 + q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
     }
-    uint4* o = reinterpret_cast<uint4*>(out + (img + (long)yh * W + xc) * 32);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint16_t* o = out + (img + (long)yh * W + tile * kConvInTile) * 32;
+    const int npix = min(kConvInTile, W - tile * kConvInTile);
     #pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+    for (int q = 0; q < 4; ++q) {
+        const int e = q * 64 + lane;               // 16-byte unit: pixel e / 4, channels 8 (e % 4) ..
+        if (e / 4 < npix) reinterpret_cast<uint4*>(o)[e] = ws[e];
+    }
 }
 
 // The three column-shifted B fragments (pixels xpix - 1, xpix, xpix + 1 of
@@ -183,12 +199,50 @@ __device__ __forceinline__ void conv_mid_store(const f32x4 (&acc)[2], const floa
     }
 }
 
+// conv_mid_store through the wave's 1 KB LDS row buffer `sb`: the 16 pixels x
+// 32 channels of the wave's output row are gathered in LDS and written with one
+// contiguous 16-byte store per lane (pixel lane / 4, channels 8 (lane % 4) ..),
+// 1 KB per instruction instead of 8-byte pieces of every pixel record.
+template <int MODE>
+__device__ __forceinline__ void conv_mid_store_lds(const f32x4 (&acc)[2], const float (&bv)[2][4],
+                                                   const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
+                                                   long rowbase, int xseg, int W, int r16, int g4, int lane, int relu,
+                                                   uint16_t* sb)
+{
+    constexpr int mode = MODE;
+    #pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int c0 = 16 * mt + 4 * g4;
+        float z[4];
+        if (mode == 0) {
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) { z[i] = acc[mt][i] + bv[mt][i]; if (relu) z[i] = fmaxf(z[i], 0.0f); }
+        } else {
+            uint2 m = make_uint2(0, 0);
+            if (xseg + r16 < W) m = *reinterpret_cast<const uint2*>(mask + (rowbase + xseg + r16) * 32 + c0);
+            const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16),
+                                    (uint16_t)(m.y & 0xffff), (uint16_t)(m.y >> 16)};
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
+        }
+        uint2 ov;
+        ov.x = pack_bf16(z[0], z[1]);
+        ov.y = pack_bf16(z[2], z[3]);
+        *reinterpret_cast<uint2*>(sb + r16 * 32 + c0) = ov;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint4 v = *reinterpret_cast<const uint4*>(sb + lane * 8);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (xseg + lane / 4 < W) *reinterpret_cast<uint4*>(out + (rowbase + xseg) * 32 + lane * 8) = v;
+}
+
 template <int L, int MODE>
-__global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+__global__ __launch_bounds__(256, 3) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
                                                   const float* __restrict__ bias, const uint16_t* __restrict__ mask,
                                                   uint16_t* __restrict__ out, int H, int W, int relu)
 {
     constexpr int mode = MODE;
+    __shared__ uint16_t sbuf[4][16 * 32];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int r16 = lane & 15, g4 = lane >> 4;
@@ -232,8 +286,9 @@ __global__ __launch_bounds__(256) void k_conv_mid(const uint16_t* __restrict__ i
             }
         }
         const int yo = yy - 1;                           // the row a0 completes
-        if (ir >= 2 && yo < H && store)
-            conv_mid_store<MODE>(a0, bv, mask, out, (img + (long)yo * W + xpix) * 32, g4, relu);
+        if (ir >= 2 && yo < H)
+            conv_mid_store_lds<MODE>(a0, bv, mask, out, img + (long)yo * W, xpix - r16, W, r16, g4, lane, relu,
+                                     sbuf[wv]);
         a0[0] = a1[0]; a0[1] = a1[1];
         a1[0] = a2[0]; a1[1] = a2[1];
     }
