@@ -172,35 +172,9 @@ __device__ __forceinline__ void row_fragments(const RowLoad& r, bf16x8 (&f)[3])
 // The weight fragments of all 9 taps and both 16-channel output tiles stay in
 // registers for the L rows.
 // ---------------------------------------------------------------------------
-template <int MODE>
-__device__ __forceinline__ void conv_mid_store(const f32x4 (&acc)[2], const float (&bv)[2][4],
-                                               const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
-                                               long pbase, int g4, int relu)
-{
-    constexpr int mode = MODE;
-    #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const int c0 = 16 * mt + 4 * g4;
-        float z[4];
-        if (mode == 0) {
-            #pragma unroll
-            for (int i = 0; i < 4; ++i) { z[i] = acc[mt][i] + bv[mt][i]; if (relu) z[i] = fmaxf(z[i], 0.0f); }
-        } else {
-            const uint2 m = *reinterpret_cast<const uint2*>(mask + pbase + c0);
-            const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16),
-                                    (uint16_t)(m.y & 0xffff), (uint16_t)(m.y >> 16)};
-            #pragma unroll
-            for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
-        }
-        uint2 ov;
-        ov.x = pack_bf16(z[0], z[1]);
-        ov.y = pack_bf16(z[2], z[3]);
-        *reinterpret_cast<uint2*>(out + pbase + c0) = ov;
-    }
-}
-
-// conv_mid_store through the wave's 1 KB LDS row buffer `sb`: the 16 pixels x
-// 32 channels of the wave's output row are gathered in LDS and written with one
+// One completed output row of a wave: out = relu?(acc + bias) (mode 0) or
+// acc * (mask > 0) (mode 1), as bf16.  The 16 pixels x 32 channels are
+// gathered in the wave's 1 KB LDS row buffer `sb` and written with one
 // contiguous 16-byte store per lane (pixel lane / 4, channels 8 (lane % 4) ..),
 // 1 KB per instruction instead of 8-byte pieces of every pixel record.
 template <int MODE>
