@@ -89,6 +89,23 @@ def test_filter_config0(c0, ftype):
     assert rel_l2(out, orc.ramp_filter(sino, geo.oracle_geo(), filter_type=ftype)) <= TOL
 
 
+@pytest.mark.parametrize("nu,ftype", [(1, 0), (37, 2), (130, 0), (736, 2), (1107, 1)])
+def test_filter_row_widths(nu, ftype):
+    # row widths not a multiple of 4 (padded tables, scalar stores), shorter than
+    # the near band, the OS-SQS bench width, and wider than one 256-lane pass
+    # (1107 bins: 277 lanes of 4 outputs); smooth rows plus noise, 23 or 24 views
+    # (an odd row count leaves the last row pair of a workgroup step half empty)
+    nview = 23 if nu % 2 else 24
+    geo = FanGeometry(nx=64, ny=64, dx=1.0, dy=1.0, nu=nu, nview=nview, da=1.0 / 1085.6, dsd=1085.6, dso=595.0)
+    rng = np.random.default_rng(nu)
+    u = (np.arange(nu) - (nu - 1) / 2) / max(nu / 2.2, 1.0)
+    base = np.sqrt(np.clip(1 - u * u, 0, None)) * 40
+    sino = (base[:, None] * rng.uniform(0.8, 1.2, (1, geo.nview)) + rng.normal(0, 0.3, (nu, geo.nview)))
+    sino = sino.astype(np.float32)[None, :, :, None, None]
+    out = recon_for(geo).cFilter4d(sino, type_filter=ftype)
+    assert rel_l2(out, orc.ramp_filter(sino, geo.oracle_geo(), filter_type=ftype)) <= TOL
+
+
 def test_parallel_filter(c0):
     geo, img, _, deg = c0
     sino = orc.dd_fan_fp(img, deg, geo.oracle_geo(), geo.nu).astype(np.float32)
