@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_ramp_filter(const float* __restrict__ i
     }
 }
 
-constexpr int kRampRows = 2;
+constexpr int kRampRows = 4;
 
 } // namespace
 
