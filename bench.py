@@ -313,11 +313,15 @@ def main():
     # update (x, g, D read, x written) + interleaved copies (x read, two 4-slice layouts written)
     bytes_upd = 4.0 * 7 * geo.nx * geo.ny * ns
     steps_k = max(kt["steps"], 1)
-    kernels = {
-        "siddon_fan_bp": (kt["bp"], bytes_proj),
-        "siddon_fan_fp_residual": (kt["fp"], bytes_proj),
-        "sqs_update": (kt["update"], bytes_upd),
-    }
+    kernels = {"siddon_fan_fp_residual": (kt["fp"], bytes_proj)}
+    if kt["update"] < 0.1 * kt["bp"]:
+        # the update ran in the backprojection's epilogue: residual, x and D read,
+        # the new image and its two interleaved copies written
+        bytes_bpu = 4.0 * (geo.nu * npers + 5 * geo.nx * geo.ny) * ns
+        kernels["siddon_fan_bp_sqs_update"] = (kt["bp"] + kt["update"], bytes_bpu)
+    else:
+        kernels["siddon_fan_bp"] = (kt["bp"], bytes_proj)
+        kernels["sqs_update"] = (kt["update"], bytes_upd)
     name = max(kernels, key=lambda k: kernels[k][0])
     tot_ms, nbytes = kernels[name]
     avg_ms = tot_ms / steps_k

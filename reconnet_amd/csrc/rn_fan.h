@@ -117,6 +117,24 @@ void launch_permute_rows(const float* in, float* out, const int* perm, int rows,
 // Siddon BP gather of a quad subset sinogram [q][view][nu] into plain images [s][nx][ny]
 void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, int nslices, int view0,
                             int nviews, hipStream_t st);
+// The OS-SQS update applied to a backprojected subset gradient: Xn = update(X, g, D)
+// per slice (plain [s][nx][ny]), and the new image's interleaved copies Qxy, Qyx
+// (padded quads, slices past the batch 0)
+struct SqsUpd {
+    const float* X;
+    float* Xn;
+    const float* D;
+    float4* Qxy;
+    float4* Qyx;
+    float step, beta2, eps, lb, ub;
+    int crop;
+};
+// launch_siddon_fan_bp_q with the update fused into the backprojection's epilogue
+// when the chosen launch shape is one of the LDS-staged ones (returns true; img_xy
+// is then not written), else the plain backprojection into img_xy (returns false:
+// the caller applies the update)
+bool launch_siddon_fan_bp_q_update(const FanK& g, const float4* prj_q, float* img_xy, int nslices, int view0,
+                                   int nviews, const SqsUpd& u, hipStream_t st);
 // reference-parity DD BP of a quad pre-weighted sinogram [q][view][nu] into plain images
 void launch_dd_fan_bp_q(const FanK& g, const float4* wprj_q, float* img_xy, int nslices, bool fbp,
                         hipStream_t st);

@@ -210,9 +210,16 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp_q(FanK g, const float
 // 0) reads zeros or an earlier view's bins, never unwritten memory.  The
 // group rows are addressed as 32-bit byte offsets from the groups' base (the
 // launcher checks that G groups fit).
-template <int BX, int BY, int NB, int G, int CAP>
+//
+// UPD: the epilogue applies the OS-SQS update to the backprojected gradient
+// instead of storing it (sqs_pixel_update per slice, the arithmetic of
+// k_sqs_update_q): the new image to u.Xn and, as float4 per group, to the
+// interleaved copies u.Qxy, u.Qyx.  Safe in place: this launch reads neither
+// u.Xn nor the interleaved copies, and u.X is only read.
+template <int BX, int BY, int NB, int G, int CAP, bool UPD = false>
 __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(FanK g, const float4* __restrict__ prj,
-                                                          float* __restrict__ img, int view0, int nviews, int ns)
+                                                          float* __restrict__ img, int view0, int nviews, int ns,
+                                                          SqsUpd u)
 {
     static_assert((BX * BY) % 64 == 0 && CAP <= 64, "whole waves, one load for the ray table row");
     // LDS rows 0 .. G - 1: the groups; row G: the ray table
@@ -313,10 +320,28 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(Fa
     for (int k = 0; k < G; ++k) {
         if (k >= nq) break;
         const int s = 4 * (q0 + k);
-        const float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
-        #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (s + c < ns) img[(long)(s + c) * plane + pix] = v[c];
+        float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+        if (UPD) {
+            #pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const long i = (long)(s + c) * plane + pix;
+                float xn = 0.0f;
+                if (s + c < ns) {
+                    xn = sqs_pixel_update(u.X + (long)(s + c) * plane, pix, ix, iy, g.nx, g.ny, v[c], u.D[i], u.step,
+                                          u.beta2, u.eps, u.lb, u.ub, u.crop);
+                    u.Xn[i] = xn;
+                }
+                v[c] = xn;
+            }
+            const float4 f = make_float4(v[0], v[1], v[2], v[3]);
+            const int pxy = qpitch(g.ny), pyx = qpitch(g.nx);
+            u.Qxy[(long)(q0 + k) * g.nx * pxy + (long)ix * pxy + kQPadL + iy] = f;
+            u.Qyx[(long)(q0 + k) * g.ny * pyx + (long)iy * pyx + kQPadL + ix] = f;
+        } else {
+            #pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (s + c < ns) img[(long)(s + c) * plane + pix] = v[c];
+        }
     }
 }
 
@@ -950,7 +975,16 @@ void go_siddon_bp_ql(const FanK& g, const float4* prj, float* img, int ns, int v
 {
     hipLaunchKernelGGL((k_siddon_fan_bp_ql<BX, BY, NB, G, CAP>),
                        dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)), dim3(BX, BY), 0, st, g, prj, img,
-                       view0, nviews, ns);
+                       view0, nviews, ns, SqsUpd{});
+}
+
+template <int BX, int BY, int NB, int G, int CAP>
+void go_siddon_bp_qlu(const FanK& g, const float4* prj, int ns, int view0, int nviews, const SqsUpd& u,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL((k_siddon_fan_bp_ql<BX, BY, NB, G, CAP, true>),
+                       dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(cdiv(ns, 4), G)), dim3(BX, BY), 0, st, g, prj,
+                       nullptr, view0, nviews, ns, u);
 }
 
 using SidFpQFn = void (*)(const FanK&, const float4*, const float4*, float4*, int, int, int, const float4*,
@@ -978,6 +1012,12 @@ const SidBpQFn kSidBpQ[] = {go_siddon_bp_ql<8, 8, 2, 8, 24>,  go_siddon_bp_ql<8,
 constexpr int kNSidFpQ = sizeof(kSidFpQ) / sizeof(kSidFpQ[0]);
 constexpr int kNSidBpQ = sizeof(kSidBpQ) / sizeof(kSidBpQ[0]);
 constexpr int kNSidBpStaged = 7;   // the first seven are k_siddon_fan_bp_ql
+using SidBpQUFn = void (*)(const FanK&, const float4*, int, int, int, const SqsUpd&, hipStream_t);
+// the staged shapes of kSidBpQ, in the same order, with the OS-SQS update in the epilogue
+const SidBpQUFn kSidBpQU[kNSidBpStaged] = {
+    go_siddon_bp_qlu<8, 8, 2, 8, 24>, go_siddon_bp_qlu<8, 32, 2, 8, 24>, go_siddon_bp_qlu<8, 8, 2, 8, 32>,
+    go_siddon_bp_qlu<8, 8, 2, 8, 21>, go_siddon_bp_qlu<8, 8, 2, 8, 16>, go_siddon_bp_qlu<8, 8, 2, 4, 24>,
+    go_siddon_bp_qlu<8, 8, 2, 2, 32>};
 
 } // namespace
 
@@ -1045,6 +1085,26 @@ void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, i
                                       st, true);
     kSidBpQ[c](g, prj_q, img_xy, nslices, view0, nviews, st);
     check_launch("siddon_fan_bp_q");
+}
+
+bool launch_siddon_fan_bp_q_update(const FanK& g, const float4* prj_q, float* img_xy, int nslices, int view0,
+                                   int nviews, const SqsUpd& u, hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return false;
+    require_offsets32(g, nviews, sizeof(float4));
+    const int first = 9L * nviews * g.nu * 16L < 0x7fffffffL ? 0 : kNSidBpStaged;
+    // the launch shape is tuned on the plain backprojection (img_xy as its output)
+    const int c = first + tune_choose({TK_SIDDON_BP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNSidBpQ - first,
+                                      [&](int i) { kSidBpQ[first + i](g, prj_q, img_xy, nslices, view0, nviews, st); },
+                                      st, true);
+    if (c < kNSidBpStaged) {
+        kSidBpQU[c](g, prj_q, nslices, view0, nviews, u, st);
+        check_launch("siddon_fan_bp_q_update");
+        return true;
+    }
+    kSidBpQ[c](g, prj_q, img_xy, nslices, view0, nviews, st);
+    check_launch("siddon_fan_bp_q");
+    return false;
 }
 
 void launch_siddon_fan_fp_f64(const FanK& g, const double* Pyx, const double* Pxy, double* prj_vu,

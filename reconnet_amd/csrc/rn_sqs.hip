@@ -8,10 +8,14 @@
 //   3. a fused update: TV gradient / curvature stencil, the SQS step
 //      x - (lambda M g + 2 beta gTV) / (D + 2 beta nTV + 1e-6), the crop to
 //      [lb, ub], and the transposed copy that the next FP's X views read.
+// With the interleaved layout and an LDS-staged backprojection shape, 2 and 3
+// are one launch: the backprojection's epilogue applies the update to the
+// gradient it holds in registers (the gradient image is not written).
 // The normaliser D = A'(w A 1) is accumulated in FP64; so are the TV cost and
 // the convergence sums.
 
 #include "rn_sqs.h"
+#include "rn_fan_device.h"
 
 #include <algorithm>
 #include <cmath>
@@ -39,23 +43,6 @@ __device__ __forceinline__ void tv_terms(const float* __restrict__ X, int ix, in
     gtv = (dx0 / t0 + dx1 / t1 + dy0 / t2 + dy1 / t3) / 2;
     ntv = 1 / t0 + 1 / t1 + 1 / t2 + 1 / t3;
     var = (t0 + t1 + t2 + t3) / 2;
-}
-
-// tv_terms with reciprocal square roots in place of the square roots and
-// divisions (about 1 ulp each), for the SQS-TV iteration's update
-__device__ __forceinline__ void tv_terms_rsq(const float* __restrict__ X, int ix, int iy, int nx, int ny, float eps,
-                                             float& gtv, float& ntv)
-{
-    const long ind = (long)ix * ny + iy;
-    const float x = X[ind];
-    const float dx0 = ix == 0 ? x : x - X[ind - ny];
-    const float dx1 = ix == nx - 1 ? x : x - X[ind + ny];
-    const float dy0 = iy == 0 ? x : x - X[ind - 1];
-    const float dy1 = iy == ny - 1 ? x : x - X[ind + 1];
-    const float r0 = __builtin_amdgcn_rsqf(fmaf(dx0, dx0, eps)), r1 = __builtin_amdgcn_rsqf(fmaf(dx1, dx1, eps));
-    const float r2 = __builtin_amdgcn_rsqf(fmaf(dy0, dy0, eps)), r3 = __builtin_amdgcn_rsqf(fmaf(dy1, dy1, eps));
-    gtv = 0.5f * (dx0 * r0 + dx1 * r1 + dy0 * r2 + dy1 * r3);
-    ntv = r0 + r1 + r2 + r3;
 }
 
 // block-wide FP64 sum of up to two values; thread 0 writes the partials
@@ -102,19 +89,8 @@ __global__ __launch_bounds__(256) void k_sqs_update(const float* __restrict__ X,
     const int s = blockIdx.z;
     const long plane = (long)nx * ny;
     if (ix >= nx || iy >= ny) return;
-    const long i = s * plane + (long)ix * ny + iy;
-    const float x = X[i];
-    float num = step * G[i];
-    float den = D[i];
-    if (beta2 != 0.0f) {
-        float g, n;
-        tv_terms_rsq(X + s * plane, ix, iy, nx, ny, eps, g, n);
-        num = fmaf(beta2, g, num);
-        den = fmaf(beta2, n, den);
-    }
-    float xn = x - num / (den + 1e-6f);
-    if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
-    Xn[i] = xn;
+    const long pix = (long)ix * ny + iy, i = s * plane + pix;
+    Xn[i] = sqs_pixel_update(X + s * plane, pix, ix, iy, nx, ny, G[i], D[i], step, beta2, eps, lb, ub, crop);
 }
 
 // k_sqs_update for the interleaved layout, fused with the layout copies: a
@@ -144,16 +120,8 @@ __global__ __launch_bounds__(256) void k_sqs_update_q(const float* __restrict__ 
             v[c] = 0.0f;
             if (s >= ns) continue;
             const long i = s * plane + pix;
-            float num = step * G[i];
-            float den = D[i];
-            if (beta2 != 0.0f) {
-                float gt, nt;
-                tv_terms_rsq(X + s * plane, ix, iy, nx, ny, eps, gt, nt);
-                num = fmaf(beta2, gt, num);
-                den = fmaf(beta2, nt, den);
-            }
-            float xn = X[i] - num / (den + 1e-6f);
-            if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
+            const float xn = sqs_pixel_update(X + s * plane, pix, ix, iy, nx, ny, G[i], D[i], step, beta2, eps, lb,
+                                              ub, crop);
             Xn[i] = xn;
             v[c] = xn;
         }
@@ -436,10 +404,20 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
                                           B_.as<float>(), W_.as<float>(), st);
         }
         if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 1], st));
-        if (quads_) launch_siddon_fan_bp_q(t_.k, R4_.as<float4>(), G_.as<float>(), ns_, start, cur, st);
-        else launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
-        if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 2], st));
+        // interleaved layout: the update runs in the backprojection's epilogue
+        // when the tuned shape allows it (then G is not written)
+        bool fused = false;
         if (quads_) {
+            const SqsUpd u{X, Xn, D_.as<float>(), Qxy_.as<float4>(), Qyx_.as<float4>(), o.lambda * nsub_,
+                           2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0};
+            fused = launch_siddon_fan_bp_q_update(t_.k, R4_.as<float4>(), G_.as<float>(), ns_, start, cur, u, st);
+        } else {
+            launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
+        }
+        if (prof_) RN_HIP_CHECK(hipEventRecord(ev_[4 * k + 2], st));
+        if (fused) {
+            // done in the backprojection
+        } else if (quads_) {
             const dim3 gq(cdiv(g.ny, 32), cdiv(g.nx, 32), cdiv(ns_, 4));
             hipLaunchKernelGGL(k_sqs_update_q, gq, dim3(256), 0, st, X, Xn, G_.as<float>(), D_.as<float>(),
                                Qxy_.as<float4>(), Qyx_.as<float4>(), g.nx, g.ny, ns_, o.lambda * nsub_,
