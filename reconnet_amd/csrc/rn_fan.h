@@ -117,13 +117,15 @@ void launch_permute_rows(const float* in, float* out, const int* perm, int rows,
 // Siddon BP gather of a quad subset sinogram [q][view][nu] into plain images [s][nx][ny]
 void launch_siddon_fan_bp_q(const FanK& g, const float4* prj_q, float* img_xy, int nslices, int view0,
                             int nviews, hipStream_t st);
-// The OS-SQS update applied to a backprojected subset gradient: Xn = update(X, g, D)
-// per slice (plain [s][nx][ny]), and the new image's interleaved copies Qxy, Qyx
-// (padded quads, slices past the batch 0)
+// The OS-SQS update applied to a backprojected subset gradient: the current
+// image is read from its interleaved copy Qold (padded quads [q][nx][qpitch(ny)])
+// and the normaliser from D4 (quads [q][nx][ny]); the new image goes to Xn
+// (plain [s][nx][ny]) and to its interleaved copies Qxy, Qyx (padded quads,
+// slices past the batch 0), which must not alias Qold
 struct SqsUpd {
-    const float* X;
+    const float4* Qold;
+    const float4* D4;
     float* Xn;
-    const float* D;
     float4* Qxy;
     float4* Qyx;
     float step, beta2, eps, lb, ub;

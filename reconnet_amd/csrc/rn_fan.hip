@@ -212,10 +212,11 @@ __global__ __launch_bounds__(BX * BY) void k_siddon_fan_bp_q(FanK g, const float
 // launcher checks that G groups fit).
 //
 // UPD: the epilogue applies the OS-SQS update to the backprojected gradient
-// instead of storing it (sqs_pixel_update per slice, the arithmetic of
-// k_sqs_update_q): the new image to u.Xn and, as float4 per group, to the
-// interleaved copies u.Qxy, u.Qyx.  Safe in place: this launch reads neither
-// u.Xn nor the interleaved copies, and u.X is only read.
+// instead of storing it (sqs_update_core per slice, the arithmetic of
+// k_sqs_update_q): the current image and its 4-neighbours are read as float4
+// groups from the interleaved copy u.Qold, the normaliser from u.D4, and the
+// new image goes to u.Xn and, as float4 per group, to the other interleaved
+// copies u.Qxy, u.Qyx (this launch reads none of the buffers it writes).
 template <int BX, int BY, int NB, int G, int CAP, bool UPD = false>
 __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(FanK g, const float4* __restrict__ prj,
                                                           float* __restrict__ img, int view0, int nviews, int ns,
@@ -322,19 +323,38 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(Fa
         const int s = 4 * (q0 + k);
         float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
         if (UPD) {
+            const int pxy = qpitch(g.ny), pyx = qpitch(g.nx);
+            const float4* qo = u.Qold + (long)(q0 + k) * g.nx * pxy;
+            const long c0 = (long)ix * pxy + kQPadL + iy;
+            const float4 xc = qo[c0];
+            // neighbours for the TV terms (padded lines: iy - 1 and iy + 1 stay inside
+            // the line; the border cases below use the pixel value itself, as the
+            // plain update)
+            float4 xl = xc, xr = xc, yl = xc, yr = xc;
+            if (u.beta2 != 0.0f) {
+                if (ix > 0) xl = qo[c0 - pxy];
+                if (ix < g.nx - 1) xr = qo[c0 + pxy];
+                yl = qo[c0 - 1];
+                yr = qo[c0 + 1];
+            }
+            const float4 d4 = u.D4[(long)(q0 + k) * plane + pix];
+            const float xs[4] = {xc.x, xc.y, xc.z, xc.w}, xls[4] = {xl.x, xl.y, xl.z, xl.w};
+            const float xrs[4] = {xr.x, xr.y, xr.z, xr.w}, yls[4] = {yl.x, yl.y, yl.z, yl.w};
+            const float yrs[4] = {yr.x, yr.y, yr.z, yr.w}, ds[4] = {d4.x, d4.y, d4.z, d4.w};
             #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const long i = (long)(s + c) * plane + pix;
+                const float x = xs[c];
+                const float dx0 = ix == 0 ? x : x - xls[c], dx1 = ix == g.nx - 1 ? x : x - xrs[c];
+                const float dy0 = iy == 0 ? x : x - yls[c], dy1 = iy == g.ny - 1 ? x : x - yrs[c];
                 float xn = 0.0f;
                 if (s + c < ns) {
-                    xn = sqs_pixel_update(u.X + (long)(s + c) * plane, pix, ix, iy, g.nx, g.ny, v[c], u.D[i], u.step,
-                                          u.beta2, u.eps, u.lb, u.ub, u.crop);
-                    u.Xn[i] = xn;
+                    xn = sqs_update_core(x, dx0, dx1, dy0, dy1, v[c], ds[c], u.step, u.beta2, u.eps, u.lb, u.ub,
+                                         u.crop);
+                    u.Xn[(long)(s + c) * plane + pix] = xn;
                 }
                 v[c] = xn;
             }
             const float4 f = make_float4(v[0], v[1], v[2], v[3]);
-            const int pxy = qpitch(g.ny), pyx = qpitch(g.nx);
             u.Qxy[(long)(q0 + k) * g.nx * pxy + (long)ix * pxy + kQPadL + iy] = f;
             u.Qyx[(long)(q0 + k) * g.ny * pyx + (long)iy * pyx + kQPadL + ix] = f;
         } else {

@@ -813,44 +813,46 @@ __device__ __forceinline__ float dd_adj_pixel(const FanK& g, const float* __rest
 
 namespace rn {
 
-// TV surrogate terms of pixel (ix, iy) of one slice X [nx][ny] (TotalVariation.cu:9-68):
-// the four differences to the 4-neighbours (the pixel value itself at the image
-// border), gTV = sum d / psi / 2 and nTV = sum 1 / psi with psi = sqrt(d^2 + eps),
-// here with reciprocal square roots (about 1 ulp each)
-__device__ __forceinline__ void tv_terms_rsq(const float* __restrict__ X, int ix, int iy, int nx, int ny, float eps,
-                                             float& gtv, float& ntv)
-{
-    const long ind = (long)ix * ny + iy;
-    const float x = X[ind];
-    const float dx0 = ix == 0 ? x : x - X[ind - ny];
-    const float dx1 = ix == nx - 1 ? x : x - X[ind + ny];
-    const float dy0 = iy == 0 ? x : x - X[ind - 1];
-    const float dy1 = iy == ny - 1 ? x : x - X[ind + 1];
-    const float r0 = __builtin_amdgcn_rsqf(fmaf(dx0, dx0, eps)), r1 = __builtin_amdgcn_rsqf(fmaf(dx1, dx1, eps));
-    const float r2 = __builtin_amdgcn_rsqf(fmaf(dy0, dy0, eps)), r3 = __builtin_amdgcn_rsqf(fmaf(dy1, dy1, eps));
-    gtv = 0.5f * (dx0 * r0 + dx1 * r1 + dy0 * r2 + dy1 * r3);
-    ntv = r0 + r1 + r2 + r3;
-}
-
 // The OS-SQS(-TV) update of one pixel of one slice (ossqs.cu:124-135 with the
-// TV terms of TotalVariation.cu): x - (step g + beta2 gTV) / (D + beta2 nTV + 1e-6),
-// cropped to [lb, ub] when crop is set.  Xs: the slice's current image, pix =
-// ix * ny + iy, g: the backprojected weighted residual, d: the normaliser.
-__device__ __forceinline__ float sqs_pixel_update(const float* __restrict__ Xs, long pix, int ix, int iy, int nx,
-                                                  int ny, float g, float d, float step, float beta2, float eps,
-                                                  float lb, float ub, int crop)
+// TV terms of TotalVariation.cu:9-68): x - (step g + beta2 gTV) / (D + beta2 nTV + 1e-6),
+// cropped to [lb, ub] when crop is set.  dx0, dx1, dy0, dy1 are the differences
+// to the 4-neighbours (the pixel value itself at the image border); gTV =
+// sum d / psi / 2 and nTV = sum 1 / psi with psi = sqrt(d^2 + eps), here with
+// reciprocal square roots (about 1 ulp each).  g: the backprojected weighted
+// residual, d: the normaliser.
+__device__ __forceinline__ float sqs_update_core(float x, float dx0, float dx1, float dy0, float dy1, float g,
+                                                 float d, float step, float beta2, float eps, float lb, float ub,
+                                                 int crop)
 {
     float num = step * g;
     float den = d;
     if (beta2 != 0.0f) {
-        float gt, nt;
-        tv_terms_rsq(Xs, ix, iy, nx, ny, eps, gt, nt);
+        const float r0 = __builtin_amdgcn_rsqf(fmaf(dx0, dx0, eps)), r1 = __builtin_amdgcn_rsqf(fmaf(dx1, dx1, eps));
+        const float r2 = __builtin_amdgcn_rsqf(fmaf(dy0, dy0, eps)), r3 = __builtin_amdgcn_rsqf(fmaf(dy1, dy1, eps));
+        const float gt = 0.5f * (dx0 * r0 + dx1 * r1 + dy0 * r2 + dy1 * r3);
+        const float nt = r0 + r1 + r2 + r3;
         num = fmaf(beta2, gt, num);
         den = fmaf(beta2, nt, den);
     }
-    float xn = Xs[pix] - num / (den + 1e-6f);
+    float xn = x - num / (den + 1e-6f);
     if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
     return xn;
+}
+
+// sqs_update_core for pixel (ix, iy) of one slice Xs [nx][ny], pix = ix * ny + iy
+__device__ __forceinline__ float sqs_pixel_update(const float* __restrict__ Xs, long pix, int ix, int iy, int nx,
+                                                  int ny, float g, float d, float step, float beta2, float eps,
+                                                  float lb, float ub, int crop)
+{
+    const float x = Xs[pix];
+    float dx0 = 0.0f, dx1 = 0.0f, dy0 = 0.0f, dy1 = 0.0f;
+    if (beta2 != 0.0f) {
+        dx0 = ix == 0 ? x : x - Xs[pix - ny];
+        dx1 = ix == nx - 1 ? x : x - Xs[pix + ny];
+        dy0 = iy == 0 ? x : x - Xs[pix - 1];
+        dy1 = iy == ny - 1 ? x : x - Xs[pix + 1];
+    }
+    return sqs_update_core(x, dx0, dx1, dy0, dy1, g, d, step, beta2, eps, lb, ub, crop);
 }
 
 } // namespace rn

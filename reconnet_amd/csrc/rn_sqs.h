@@ -58,7 +58,10 @@ private:
     // one FP launch over the whole subset (default), or one per view class
     // (RECONNET_SQS_FP_SPLIT=1)
     bool fp_split_ = false;
-    DevBuf Qxy_, Qyx_, B4_, W4_, R4_;
+    // Qxy_[c], Qyx_[c]: the interleaved copies of image X_[c] (the update
+    // writes the new image's copies while the current ones are read); D4_: the
+    // normaliser as quads
+    DevBuf Qxy_[2], Qyx_[2], B4_, W4_, R4_, D4_;
     // Within each subset the engine orders the views X class first (a view
     // with |cos| > |sin| walks image rows, the others columns): in XCD order
     // the interleaved forward projection's workgroups of one XCD then walk one

@@ -263,10 +263,13 @@ void OsSqs::setup(const FanGeo& g, const float* deg_host, int nslices, int nview
         const size_t nq = (size_t)(ns_ + 3) / 4;
         // padded lines; the pad cells stay zero (only line interiors are written)
         const size_t nxy = nq * g.nx * (size_t)qpitch(g.ny), nyx = nq * g.ny * (size_t)qpitch(g.nx);
-        Qxy_.ensure(nxy * sizeof(float4));
-        Qyx_.ensure(nyx * sizeof(float4));
-        launch_fill(Qxy_.as<float>(), 0.0f, 4 * nxy, st);
-        launch_fill(Qyx_.as<float>(), 0.0f, 4 * nyx, st);
+        for (int c = 0; c < 2; c++) {
+            Qxy_[c].ensure(nxy * sizeof(float4));
+            Qyx_[c].ensure(nyx * sizeof(float4));
+            launch_fill(Qxy_[c].as<float>(), 0.0f, 4 * nxy, st);
+            launch_fill(Qyx_[c].as<float>(), 0.0f, 4 * nyx, st);
+        }
+        D4_.ensure(nq * plane * sizeof(float4));
         B4_.ensure(nq * sino * sizeof(float4));
         W4_.ensure(nq * sino * sizeof(float4));
         R4_.ensure(nq * (size_t)g.nu * npers_ * sizeof(float4));
@@ -314,6 +317,7 @@ void OsSqs::compute_norm(hipStream_t st)
     hipLaunchKernelGGL(k_f64_to_f32, dim3(cdiv((long)(ns_ * plane), 256)), dim3(256), 0, st,
                        d64.as<double>(), D_.as<float>(), ns_ * plane);
     check_launch("f64_to_f32");
+    if (quads_) launch_pack_quads(D_.as<float>(), D4_.as<float4>(), nullptr, g.nx, g.ny, ns_, false, st);
     RN_HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -321,12 +325,14 @@ void OsSqs::set_norm(const float* norm_xy, hipStream_t st)
 {
     const size_t n = (size_t)ns_ * t_.geo.nx * t_.geo.ny;
     RN_HIP_CHECK(hipMemcpyAsync(D_.get(), norm_xy, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    if (quads_)
+        launch_pack_quads(D_.as<float>(), D4_.as<float4>(), nullptr, t_.geo.nx, t_.geo.ny, ns_, false, st);
 }
 
 void OsSqs::sync_transposed(hipStream_t st)
 {
-    if (quads_) launch_pack_quads(X_[cur_].as<float>(), Qxy_.as<float4>(), Qyx_.as<float4>(), t_.geo.nx, t_.geo.ny,
-                                  ns_, true, st);
+    if (quads_) launch_pack_quads(X_[cur_].as<float>(), Qxy_[cur_].as<float4>(), Qyx_[cur_].as<float4>(), t_.geo.nx,
+                                  t_.geo.ny, ns_, true, st);
     else launch_transpose(X_[cur_].as<float>(), Xyx_.as<float>(), t_.geo.nx, t_.geo.ny, ns_, st);
 }
 
@@ -392,13 +398,13 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
             // its view's class; the views are ordered X class first), or one
             // per class
             const int nxv = fp_split_ ? sub_nx_[k] : cur;
+            const float4 *qyx = Qyx_[cur_].as<float4>(), *qxy = Qxy_[cur_].as<float4>();
             if (nxv > 0)
-                launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
-                                                start, nxv, B4_.as<float4>(), W4_.as<float4>(), cur, 0, st);
+                launch_siddon_fan_fp_residual_q(t_.k, qyx, qxy, R4_.as<float4>(), ns_, start, nxv, B4_.as<float4>(),
+                                                W4_.as<float4>(), cur, 0, st);
             if (nxv < cur)
-                launch_siddon_fan_fp_residual_q(t_.k, Qyx_.as<float4>(), Qxy_.as<float4>(), R4_.as<float4>(), ns_,
-                                                start + nxv, cur - nxv, B4_.as<float4>(), W4_.as<float4>(), cur, nxv,
-                                                st);
+                launch_siddon_fan_fp_residual_q(t_.k, qyx, qxy, R4_.as<float4>(), ns_, start + nxv, cur - nxv,
+                                                B4_.as<float4>(), W4_.as<float4>(), cur, nxv, st);
         } else {
             launch_siddon_fan_fp_residual(t_.k, Xyx_.as<float>(), X, R_.as<float>(), ns_, start, cur,
                                           B_.as<float>(), W_.as<float>(), st);
@@ -408,8 +414,9 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         // when the tuned shape allows it (then G is not written)
         bool fused = false;
         if (quads_) {
-            const SqsUpd u{X, Xn, D_.as<float>(), Qxy_.as<float4>(), Qyx_.as<float4>(), o.lambda * nsub_,
-                           2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0};
+            const SqsUpd u{Qxy_[cur_].as<float4>(), D4_.as<float4>(), Xn, Qxy_[1 - cur_].as<float4>(),
+                           Qyx_[1 - cur_].as<float4>(), o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub,
+                           crop ? 1 : 0};
             fused = launch_siddon_fan_bp_q_update(t_.k, R4_.as<float4>(), G_.as<float>(), ns_, start, cur, u, st);
         } else {
             launch_siddon_fan_bp(t_.k, R_.as<float>(), G_.as<float>(), ns_, start, cur, false, st);
@@ -420,8 +427,8 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         } else if (quads_) {
             const dim3 gq(cdiv(g.ny, 32), cdiv(g.nx, 32), cdiv(ns_, 4));
             hipLaunchKernelGGL(k_sqs_update_q, gq, dim3(256), 0, st, X, Xn, G_.as<float>(), D_.as<float>(),
-                               Qxy_.as<float4>(), Qyx_.as<float4>(), g.nx, g.ny, ns_, o.lambda * nsub_,
-                               2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
+                               Qxy_[1 - cur_].as<float4>(), Qyx_[1 - cur_].as<float4>(), g.nx, g.ny, ns_,
+                               o.lambda * nsub_, 2.0f * o.beta, o.eps, o.lb, o.ub, crop ? 1 : 0);
             check_launch("sqs_update_q");
         } else {
             hipLaunchKernelGGL(k_sqs_update, grd, blk, 0, st, X, Xn, G_.as<float>(), D_.as<float>(), g.nx, g.ny,
