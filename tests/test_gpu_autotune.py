@@ -144,6 +144,32 @@ def test_every_interleaved_launch_shape_agrees():
         eng.close()
 
 
+@pytest.mark.parametrize("beta", [0.0, 1e-3])
+def test_fused_update_matches_separate_update(beta):
+    # 32 slices (8 groups of 4 per lane): two OS-SQS passes with the update in
+    # the staged backprojection's epilogue (shape 0; image and TV neighbours read
+    # from the ping-pong interleaved copies) against two passes with a plain
+    # gather shape (index 7: no staging) and the separate update kernel; the
+    # second pass reads the copies the first one wrote
+    from reconnet_amd._lib import autotune_pin
+
+    eng, x0 = _engine(32)
+    try:
+        outs = []
+        for c in (0, 7):
+            autotune_pin("siddon_bp_q", c)
+            eng.set_image(x0)
+            for _ in range(2):
+                eng.run_pass(beta=beta, eps=1e-4, lb=0.0)
+            outs.append(eng.image().clone())
+        err = float((outs[0] - outs[1]).abs().max() / outs[1].abs().max())
+        assert err <= 1e-6, err
+        assert float(outs[1].abs().max()) > 0
+    finally:
+        autotune_pin("siddon_bp_q", None)
+        eng.close()
+
+
 @pytest.mark.parametrize("ns,rect", [(4, False), (7, False), (6, True)])
 def test_interleaved_engine_matches_plain_layout(ns, rect, monkeypatch):
     # the same passes with the engine's plain one-slice layout
