@@ -290,11 +290,23 @@ extern "C" void cOSSQS(float* res, const float* img, const float* prj,
     (void)typeProjector;
     guarded("cOSSQS", [&] {
         check_bc(nBatches, nChannels);
-        RN_REQUIRE(nz == 1 && nv == 1 && nChannels == 1,
-                   "cOSSQS supports 2D fan slices (nz = nv = nChannels = 1)");
         RN_REQUIRE(degs != nullptr && wprj != nullptr && normImg != nullptr, "null input pointer");
         FanGeo g = make_fan(nx, ny, nz, dx, dy, dz, nu, nview, nv, da, dv, off_a, off_v, dsd, dso);
         ThreadCtx& c = ctx();
+        if (nz != 1 || nv != 1 || nChannels != 1) {
+            // several z planes, detector rows or channels: the pass on the
+            // reference layouts with the z/v mixing of the Siddon fan pair
+            const size_t nimg = (size_t)nBatches * nx * ny * nz * nChannels;
+            const size_t nprj = (size_t)nBatches * nu * nview * nv * nChannels;
+            h2d(c.a, prj, nprj, c.st());
+            h2d(c.b, wprj, nprj, c.st());
+            h2d(c.c, normImg, nimg, c.st());
+            h2d(c.d, img, nimg, c.st());
+            os_sqs_ref_layout(g, degs, nBatches, nChannels, nviewPerSubset, nSubsets, c.d.as<float>(),
+                              c.a.as<float>(), c.b.as<float>(), c.c.as<float>(), lambda, lb, ub, c.st());
+            d2h(res, c.d, nimg, c.st());
+            return;
+        }
         const size_t nimg = (size_t)nBatches * nx * ny, nprj = (size_t)nBatches * nu * nview;
         OsSqs eng;
         eng.setup(g, degs, nBatches, nviewPerSubset, nSubsets, c.st());

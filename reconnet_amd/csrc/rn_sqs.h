@@ -13,6 +13,16 @@ namespace rn {
 void tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
             std::vector<double>* tv, DevBuf& scratch, hipStream_t st);
 
+// One cOSSQS pass (ossqs.cu:12-161) on reference layouts for any nB, nC, nz,
+// nv (device pointers): per subset of views [start, start + cur), the Siddon
+// FP of x, r = w (Ax - b) against that block of the full sinogram, the Siddon
+// BP of r and x <- x - lambda nSubsets A'r / (D + 1e-6), cropped to [lb, ub]
+// when lb < ub.  x is updated in place; a short last subset has
+// nview - start views.
+void os_sqs_ref_layout(const FanGeo& g, const float* deg_host, int nB, int nC, int nviewPerSubset, int nSubsets,
+                       float* x, const float* prj, const float* w, const float* D, float lambda, float lb, float ub,
+                       hipStream_t st);
+
 struct SqsStep {
     float lambda = 1.0f;    // step scale of the data term (ossqs.cu:139)
     float beta = 0.0f;      // TV weight (0 = data term only, as cOSSQS)

@@ -16,6 +16,7 @@
 
 #include "rn_sqs.h"
 #include "rn_fan_device.h"
+#include "rn_fan_ops.h"
 
 #include <algorithm>
 #include <cmath>
@@ -197,7 +198,68 @@ __global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b
     if (i < n) b[i] = (float)a[i];
 }
 
+// fp[(bu, j, q)] = w (fp - meas) with (bu, start + j, q) the element of the
+// full reference-layout sinogram [bu = b nu + iu][view][q = iv nC + c]
+__global__ __launch_bounds__(256) void k_os_residual_ref(float* __restrict__ fp, const float* __restrict__ meas,
+                                                         const float* __restrict__ w, long n, int cur, int nview,
+                                                         int inner, int start)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long q = i % inner;
+    const long t = i / inner;
+    const long j = t % cur;
+    const long bu = t / cur;
+    const long src = (bu * nview + start + j) * inner + q;
+    fp[i] = (fp[i] - meas[src]) * w[src];
+}
+
+// ossqs.cu:133-145: u = g / (D + 1e-6); x = x - step u; crop to [lb, ub] if lb < ub
+__global__ __launch_bounds__(256) void k_os_update_ref(float* __restrict__ x, const float* __restrict__ g,
+                                                       const float* __restrict__ D, long n, float step, float lb,
+                                                       float ub, int crop)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float u = g[i] / (D[i] + 1e-6f);
+    float v = x[i] - step * u;
+    if (crop) v = fminf(fmaxf(v, lb), ub);
+    x[i] = v;
+}
+
 } // namespace
+
+void os_sqs_ref_layout(const FanGeo& g, const float* deg_host, int nB, int nC, int nviewPerSubset, int nSubsets,
+                       float* x, const float* prj, const float* w, const float* D, float lambda, float lb, float ub,
+                       hipStream_t st)
+{
+    RN_REQUIRE(nviewPerSubset > 0 && nSubsets > 0, "nviewPerSubset and nSubsets must be positive");
+    const long nimg = (long)nB * g.nx * g.ny * g.nz * nC;
+    const int inner = g.nv * nC;
+    DevBuf fp((size_t)nB * g.nu * nviewPerSubset * inner * sizeof(float));
+    DevBuf grad((size_t)nimg * sizeof(float));
+    FanWork work;
+    // one table set per subset: a table is not rebuilt while launches that
+    // read it may still be queued
+    std::vector<FanTables> tabs(nSubsets);
+    const float step = lambda * (float)nSubsets;
+    for (int s = 0, start = 0; s < nSubsets && start < g.nview; s++) {
+        const int cur = std::min(nviewPerSubset, g.nview - start);
+        FanGeo gs = g;
+        gs.nview = cur;
+        FanTables& tab = tabs[s];
+        tab.build(gs, deg_host + start, st);
+        fan_forward(tab, FanFP::Siddon, nB, nC, x, fp.as<float>(), work, st);
+        const long nres = (long)nB * g.nu * cur * inner;
+        k_os_residual_ref<<<cdiv(nres, 256), 256, 0, st>>>(fp.as<float>(), prj, w, nres, cur, g.nview, inner, start);
+        RN_HIP_CHECK(hipGetLastError());
+        fan_backward(tab, FanBP::Siddon, nB, nC, fp.as<float>(), grad.as<float>(), work, st);
+        k_os_update_ref<<<cdiv(nimg, 256), 256, 0, st>>>(x, grad.as<float>(), D, nimg, step, lb, ub, lb < ub ? 1 : 0);
+        RN_HIP_CHECK(hipGetLastError());
+        start += cur;
+    }
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+}
 
 void tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
             std::vector<double>* tv, DevBuf& scratch, hipStream_t st)

@@ -6,9 +6,18 @@ from reconnet_amd.geometry import FanGeometry, fan_angles
 
 
 def rel_l2(a, b):
+    """Relative L2 error of a against the reference b.
+
+    An all-zero (or non-finite) reference makes the ratio meaningless, so it
+    raises instead of reporting 0 vs 0 as a pass."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    if a.shape != b.shape:
+        raise ValueError("rel_l2: shapes differ: %s vs %s" % (a.shape, b.shape))
+    nb = np.linalg.norm(b)
+    if not np.isfinite(nb) or nb == 0.0:
+        raise ValueError("rel_l2: reference norm is %r; a parity check against it proves nothing" % nb)
+    return float(np.linalg.norm(a - b) / nb)
 
 
 def small_fan(nx=64, nu=128, nview=90, dx=1.0, du=1.2, off_a=0.0):

@@ -262,6 +262,35 @@ def test_ossqs_parity_small():
     assert rel_l2(out, ref) <= TOL
 
 
+@pytest.mark.parametrize("nz,dz,nv,dv,nC,nB", [(3, 1.5, 3, 1.0, 2, 1), (2, 3.0, 3, 1.0, 1, 2), (3, 1.0, 2, 1.5, 2, 2)])
+def test_ossqs_multislice_multichannel(nz, dz, nv, dv, nC, nB):
+    # cOSSQS with several image planes, detector rows and channels
+    # (ossqs.cu:79-81 passes them to SiddonFan): rows map to planes through
+    # the fan rays' heights (none on a plane boundary), one plane may collect
+    # several rows or none, and every
+    # (batch, channel) pair is an independent reconstruction
+    geo = FanGeometry(nx=40, ny=36, dx=1.0, dy=1.1, nu=72, nview=48, da=1.2 / 1085.6, dsd=1085.6, dso=595.0,
+                      nz=nz, dz=dz, nv=nv, dv=dv, off_v=0.2)
+    deg = fan_angles(geo.nview)
+    g = geo.oracle_geo()
+    rng = np.random.default_rng(12)
+    truth = rng.uniform(0, 0.05, (nB, geo.nx, geo.ny, nz, nC)).astype(np.float32)
+    b = orc.siddon_fan_fp(truth, deg, g, geo.nu, nv).astype(np.float32)
+    w = rng.uniform(0.5, 1.0, b.shape).astype(np.float32)
+    D = rng.uniform(20, 40, truth.shape).astype(np.float32)
+    x0 = rng.uniform(0, 0.02, truth.shape).astype(np.float32)
+    out = recon_for(geo).cSiddonFanOSSQS4d(6, x0, b, D, w, lb=0.0, ub=1.0)
+    ref = orc.os_sqs(x0, b, D, w, deg, g, 6, lb=0.0, ub=1.0)
+    assert np.abs(ref - x0).max() > 1e-4   # the pass moved the image
+    for bb in range(nB):
+        for c in range(nC):
+            assert rel_l2(out[bb, ..., c], ref[bb, ..., c]) <= TOL, (bb, c)
+    # a short last subset and no crop (lb = ub)
+    out = recon_for(geo).cSiddonFanOSSQS4d(5, x0, b, D, w, lb=0.0, ub=0.0)
+    ref = orc.os_sqs(x0, b, D, w, deg, g, 5, lb=0.0, ub=0.0)
+    assert rel_l2(out, ref) <= TOL
+
+
 def _rect_fan(nx, ny, dx, dy, nu, nview):
     return FanGeometry(nx=nx, ny=ny, dx=dx, dy=dy, nu=nu, nview=nview, da=1.2 / 1085.6, dsd=1085.6, dso=595.0)
 
@@ -380,19 +409,32 @@ def test_siddon_cone_parity():
                   orc.siddon_cone_bp(y, C, U, V, S, g, (24, 24, 12))) <= TOL
 
 
-def test_siddon_parallel_parity():
-    th = fan_angles(18).astype(np.float64)
+@pytest.mark.parametrize("tilted", [False, True])
+def test_siddon_parallel_parity(tilted):
+    # invRayDir points from the detector back through the volume towards the
+    # source (SiddonParallel.cu:28-30); the tilted case has rays climbing in z
+    # and a grid centre off the origin
+    nview = 18
+    th = fan_angles(nview).astype(np.float64)
     C = np.stack([-200 * np.sin(th), 200 * np.cos(th), np.zeros_like(th)], 1).astype(np.float32)
     U = np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], 1).astype(np.float32)
-    V = np.tile([0.0, 0.0, 1.0], (18, 1)).astype(np.float32)
-    D = np.stack([-np.sin(th), np.cos(th), np.zeros_like(th)], 1).astype(np.float32)   # detector -> source
-    r = _arb_recon(20, 20, 6, 36, 8, 1.0, 1.0, 18)
+    V = np.tile([0.0, 0.0, 1.0], (nview, 1)).astype(np.float32)
+    D = np.stack([np.sin(th), -np.cos(th), np.full_like(th, 0.02 if tilted else 0.0)], 1)
+    D = (D / np.linalg.norm(D, axis=1, keepdims=True)).astype(np.float32)
+    r = _arb_recon(20, 20, 6, 36, 8, 1.0, 1.0, nview)
+    g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.0, dv=1.0, off_u=0.3, off_v=-0.2)
+    if tilted:
+        r.cx, r.cy, r.cz = 1.5, -2.0, 0.5
+        g.update(cx=1.5, cy=-2.0, cz=0.5)
     rng = np.random.default_rng(6)
     x = rng.uniform(0, 1, (1, 20, 20, 6, 1)).astype(np.float32)
-    y = rng.uniform(0, 1, (1, 36, 18, 8, 1)).astype(np.float32)
-    g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.0, dv=1.0, off_u=0.3, off_v=-0.2)
-    assert rel_l2(r.cSiddonParallelProjection4d(x, C, U, V, D), orc.siddon_par_fp(x, C, U, V, D, g, 36, 8)) <= TOL
-    assert rel_l2(r.cSiddonParallelBackprojection4d(y, C, U, V, D), orc.siddon_par_bp(y, C, U, V, D, g, (20, 20, 6))) <= TOL
+    y = rng.uniform(0, 1, (1, 36, nview, 8, 1)).astype(np.float32)
+    ref_fp = orc.siddon_par_fp(x, C, U, V, D, g, 36, 8)
+    ref_bp = orc.siddon_par_bp(y, C, U, V, D, g, (20, 20, 6))
+    # most rays cross the volume: the comparison is not of two zero arrays
+    assert (ref_fp > 0).mean() > 0.3 and (ref_bp > 0).mean() > 0.9
+    assert rel_l2(r.cSiddonParallelProjection4d(x, C, U, V, D), ref_fp) <= TOL
+    assert rel_l2(r.cSiddonParallelBackprojection4d(y, C, U, V, D), ref_bp) <= TOL
 
 
 def test_dd_tomo_parity():
