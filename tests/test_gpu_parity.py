@@ -277,7 +277,9 @@ def test_ossqs_multislice_multichannel(nz, dz, nv, dv, nC, nB):
     truth = rng.uniform(0, 0.05, (nB, geo.nx, geo.ny, nz, nC)).astype(np.float32)
     b = orc.siddon_fan_fp(truth, deg, g, geo.nu, nv).astype(np.float32)
     w = rng.uniform(0.5, 1.0, b.shape).astype(np.float32)
-    D = rng.uniform(20, 40, truth.shape).astype(np.float32)
+    # the SQS normaliser A'(w A 1) of every (batch, channel) pair
+    fp1 = orc.siddon_fan_fp(np.ones_like(truth), deg, g, geo.nu, nv)
+    D = orc.siddon_fan_bp((fp1 * w).astype(np.float32), deg, g, (geo.nx, geo.ny, nz)).astype(np.float32)
     x0 = rng.uniform(0, 0.02, truth.shape).astype(np.float32)
     out = recon_for(geo).cSiddonFanOSSQS4d(6, x0, b, D, w, lb=0.0, ub=1.0)
     ref = orc.os_sqs(x0, b, D, w, deg, g, 6, lb=0.0, ub=1.0)
@@ -324,13 +326,20 @@ def test_sqs_tv_pass_matches_oracle_composition(ns, rect):
     from reconnet_amd.sqs import OsSqsTV
 
     geo = _rect_fan(56, 37, 1.2, 0.8, 96, 40) if rect else small_fan(nx=48, nu=96, nview=40)
-    nsub, beta = 4, 1e-3
+    nsub, beta = 4, 0.2
     deg = fan_angles(geo.nview)
     rng = np.random.default_rng(5)
-    b = rng.uniform(0, 2, (ns, geo.nu, geo.nview)).astype(np.float32)
+    g5 = geo.oracle_geo()
+    # measurements of piecewise-constant slices with noise, the normaliser
+    # A'(w A 1), and a start image that the pass moves by a visible amount
+    truth = np.stack([phantom(40 + s, geo.nx, geo.ny, geo.dx, geo.dy) for s in range(ns)])
+    b = orc.siddon_fan_fp(truth[..., None, None], deg, g5, geo.nu)[:, :, :, 0, 0]
+    b = (b * rng.uniform(0.97, 1.03, b.shape)).astype(np.float32)
     w = rng.uniform(0.5, 1, (ns, geo.nu, geo.nview)).astype(np.float32)
-    x = rng.uniform(0, 0.05, (ns, geo.nx, geo.ny)).astype(np.float32)
-    D = rng.uniform(5, 10, (ns, geo.nx, geo.ny)).astype(np.float32)
+    x = (truth + rng.uniform(0, 0.01, truth.shape)).astype(np.float32)
+    fp1 = orc.siddon_fan_fp(np.ones((1, geo.nx, geo.ny, 1, 1), np.float32), deg, g5, geo.nu)[:, :, :, 0, 0]
+    D = orc.siddon_fan_bp((fp1 * w)[..., None, None].astype(np.float32), deg, g5,
+                          (geo.nx, geo.ny, 1))[:, :, :, 0, 0].astype(np.float32)
     eng = OsSqsTV(geo, nsub, ns, order=None)
     eng.set_data(torch.from_numpy(b).cuda(), torch.from_numpy(w).cuda())
     eng.set_norm(torch.from_numpy(D).cuda())
@@ -340,7 +349,6 @@ def test_sqs_tv_pass_matches_oracle_composition(ns, rect):
     eng.close()
     npers = geo.nview // nsub
     ref = x.astype(np.float64)
-    g5 = geo.oracle_geo()
     for k in range(nsub):
         v = slice(k * npers, (k + 1) * npers)
         xi = ref.astype(np.float32)[:, :, :, None, None]
@@ -432,7 +440,7 @@ def test_siddon_parallel_parity(tilted):
     ref_fp = orc.siddon_par_fp(x, C, U, V, D, g, 36, 8)
     ref_bp = orc.siddon_par_bp(y, C, U, V, D, g, (20, 20, 6))
     # most rays cross the volume: the comparison is not of two zero arrays
-    assert (ref_fp > 0).mean() > 0.3 and (ref_bp > 0).mean() > 0.9
+    assert (ref_fp > 0).mean() > 0.2 and (ref_bp > 0).mean() > 0.5
     assert rel_l2(r.cSiddonParallelProjection4d(x, C, U, V, D), ref_fp) <= TOL
     assert rel_l2(r.cSiddonParallelBackprojection4d(y, C, U, V, D), ref_bp) <= TOL
 
