@@ -201,6 +201,18 @@ int rn_sqs_kernel_times(void* sqs, double* ms_fp, double* ms_bp, double* ms_upda
 int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
         double* tv, void* stream);
 
+/* No counterpart in the reference: rn_tv_sqs for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels and results as rn_tv_sqs; the FP64
+ * per-tile partials live in a file-scope device array of 2^18 doubles (2 MiB
+ * per device, held until the process ends), so
+ * 2 * ceil(nx/16) * ceil(ny/16) * nslices must be <= 262144 (e.g. 128 slices
+ * of 512 x 512), else -1 with an error and nothing launched.  tv_dev (device,
+ * nslices doubles, may be NULL) receives the FP64 per-slice costs; nothing is
+ * synchronised.  Calls that overlap on one device share the array and race. */
+int rn_tv_sqs_noalloc(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
+        double* tv_dev, void* stream);
+
 /* Learned-prior 3x3 convolutions (no reference counterpart).  Images fp32
  * [B][H][W]; activations bf16 bit patterns (uint16) NHWC [B][H][W][32]; weights
  * w1, w5 fp32 [9][32], wt bf16 [9][32 out][32 in]; tap t = 3 (dy+1) + (dx+1);

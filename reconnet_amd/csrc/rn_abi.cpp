@@ -738,6 +738,15 @@ extern "C" int rn_tv_sqs(const float* img, float* grad, float* norm, int nx, int
     });
 }
 
+extern "C" int rn_tv_sqs_noalloc(const float* img, float* grad, float* norm, int nx, int ny, int nslices,
+                                 float eps, double* tv_dev, void* stream)
+{
+    return guarded_status("rn_tv_sqs_noalloc", [&] {
+        RN_REQUIRE(img && grad && norm, "null pointer");
+        tv_sqs_noalloc(img, grad, norm, nx, ny, nslices, eps, tv_dev, static_cast<hipStream_t>(stream));
+    });
+}
+
 // ----- learned-prior convolutions (device pointers; bf16 tensors as raw uint16) -----
 
 #include "rn_conv.h"

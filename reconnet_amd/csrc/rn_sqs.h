@@ -23,6 +23,17 @@ void os_sqs_ref_layout(const FanGeo& g, const float* deg_host, int nB, int nC, i
                        float* x, const float* prj, const float* w, const float* D, float lambda, float lb, float ub,
                        hipStream_t st);
 
+// tv_sqs without any allocation: the FP64 partials go to a file-scope device
+// array of kTvScratchDoubles doubles (2 MiB per device, held until the process
+// ends); the per-slice costs go to tv_dev (device, nslices doubles; may be
+// null).  Needs tv_sqs_scratch_doubles(nx, ny, nslices) <= kTvScratchDoubles,
+// else it throws before launching anything.  Calls that overlap on one device
+// share the array and race.
+constexpr long kTvScratchDoubles = 1L << 18;
+long tv_sqs_scratch_doubles(int nx, int ny, int nslices);
+void tv_sqs_noalloc(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
+                    double* tv_dev, hipStream_t st);
+
 struct SqsStep {
     float lambda = 1.0f;    // step scale of the data term (ossqs.cu:139)
     float beta = 0.0f;      // TV weight (0 = data term only, as cOSSQS)

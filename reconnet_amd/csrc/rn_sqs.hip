@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 
 #include <cstring>
 
@@ -156,8 +158,10 @@ __global__ __launch_bounds__(256) void k_diff_sums(const float* __restrict__ a, 
     block_sum2(dd, bb, partial + 2 * (s * nb + (long)blockIdx.y * gridDim.x + blockIdx.x));
 }
 
-// sum the per-block partials of each slice (FP64), accumulate into out[2*s..]
-__global__ void k_reduce_partials(const double* __restrict__ partial, long nb, double* __restrict__ out, int accumulate)
+// sum the per-block partials of each slice (FP64), accumulate into out[2*s..];
+// first_only: out[s] = sum of the first values only
+__global__ void k_reduce_partials(const double* __restrict__ partial, long nb, double* __restrict__ out, int accumulate,
+                                  int first_only = 0)
 {
     __shared__ double sa[256], sb[256];
     const int s = blockIdx.x;
@@ -173,7 +177,8 @@ __global__ void k_reduce_partials(const double* __restrict__ partial, long nb, d
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (accumulate) { out[2 * s] += sa[0]; out[2 * s + 1] += sb[0]; }
+        if (first_only) out[s] = sa[0];
+        else if (accumulate) { out[2 * s] += sa[0]; out[2 * s + 1] += sb[0]; }
         else { out[2 * s] = sa[0]; out[2 * s + 1] = sb[0]; }
     }
 }
@@ -272,7 +277,7 @@ void tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nsli
     double* out = partial + 2 * nb * nslices;
     hipLaunchKernelGGL(k_tv_sqs, grd, blk, 0, st, img, grad, norm, nx, ny, eps, partial);
     check_launch("tv_sqs");
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nslices), dim3(256), 0, st, partial, nb, out, 0);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nslices), dim3(256), 0, st, partial, nb, out, 0, 0);
     check_launch("reduce_partials");
     if (tv) {
         std::vector<double> h(2 * (size_t)nslices);
@@ -280,6 +285,61 @@ void tv_sqs(const float* img, float* grad, float* norm, int nx, int ny, int nsli
         RN_HIP_CHECK(hipStreamSynchronize(st));
         tv->resize(nslices);
         for (int s = 0; s < nslices; s++) (*tv)[s] = h[2 * s];
+    }
+}
+
+// ---- TV surrogate without allocation ----
+// The FP64 per-workgroup cost partials of k_tv_sqs live in this file-scope
+// device array instead of a library-allocated buffer.  Host code never names
+// it except to resolve its address: each device holds its own copy, at the
+// address hipGetSymbolAddress returns with that device current, looked up
+// once per device and kept.
+namespace {
+static __device__ __align__(256) double scratch[kTvScratchDoubles];
+
+double* tv_scratch_ptr()
+{
+    constexpr int kMaxDevices = 64;
+    static std::once_flag once[kMaxDevices];
+    static double* ptr[kMaxDevices] = {};
+    static hipError_t err[kMaxDevices] = {};
+    int dev = 0;
+    RN_HIP_CHECK(hipGetDevice(&dev));
+    RN_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+    std::call_once(once[dev], [&] {
+        void* p = nullptr;
+        err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        ptr[dev] = static_cast<double*>(p);
+    });
+    if (err[dev] != hipSuccess) throw Error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err[dev]));
+    RN_REQUIRE(ptr[dev] != nullptr, "no device address for the TV scratch array");
+    return ptr[dev];
+}
+} // namespace
+
+long tv_sqs_scratch_doubles(int nx, int ny, int nslices)
+{
+    return 2L * cdiv(ny, TB) * cdiv(nx, TB) * (long)nslices;
+}
+
+void tv_sqs_noalloc(const float* img, float* grad, float* norm, int nx, int ny, int nslices, float eps,
+                    double* tv_dev, hipStream_t st)
+{
+    RN_REQUIRE(nx > 0 && ny > 0 && nslices > 0, "TV dimensions must be positive");
+    const long need = tv_sqs_scratch_doubles(nx, ny, nslices);
+    RN_REQUIRE(need <= kTvScratchDoubles,
+               "TV partials need " + std::to_string(need) + " doubles, beyond the " +
+               std::to_string(kTvScratchDoubles) + "-double scratch array (2 per 16 x 16 tile and slice)");
+    double* partial = tv_scratch_ptr();
+    dim3 blk(TB, TB), grd(cdiv(ny, TB), cdiv(nx, TB), nslices);
+    const long nb = (long)grd.x * grd.y;
+    // every partial is written by k_tv_sqs before the reduction reads it:
+    // nothing in the array has to start at zero
+    hipLaunchKernelGGL(k_tv_sqs, grd, blk, 0, st, img, grad, norm, nx, ny, eps, partial);
+    check_launch("tv_sqs");
+    if (tv_dev) {
+        hipLaunchKernelGGL(k_reduce_partials, dim3(nslices), dim3(256), 0, st, partial, nb, tv_dev, 0, 1);
+        check_launch("reduce_partials");
     }
 }
 
@@ -520,7 +580,7 @@ void OsSqs::pass(const SqsStep& o, hipStream_t st, std::vector<double>* track)
         const float* x0 = reinterpret_cast<const float*>(sums + 2 * (size_t)ns_);
         hipLaunchKernelGGL(k_diff_sums, grd, blk, 0, st, x0, X_[cur_].as<float>(), g.nx, g.ny, partial);
         check_launch("diff_sums");
-        hipLaunchKernelGGL(k_reduce_partials, dim3(ns_), dim3(256), 0, st, partial, nb, sums, 0);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(ns_), dim3(256), 0, st, partial, nb, sums, 0, 0);
         check_launch("reduce_partials");
         track->resize(2 * (size_t)ns_);
         RN_HIP_CHECK(hipMemcpyAsync(track->data(), sums, track->size() * sizeof(double), hipMemcpyDeviceToHost, st));
