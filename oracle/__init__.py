@@ -232,12 +232,12 @@ def tv_iso_sqs(img, dims, eps=1e-8):
     return tuple(out)
 
 
-def dd_cone_fp(img, det_center, det_u, det_v, src, geo, nu, nv):
-    return _arb_fp("dd_cone_fp", img, [det_center, det_u, det_v, src], geo, nu, nv, "f64")
+def dd_cone_fp(img, det_center, det_u, det_v, src, geo, nu, nv, precision="f64"):
+    return _arb_fp("dd_cone_fp", img, [det_center, det_u, det_v, src], geo, nu, nv, precision)
 
 
-def dd_cone_adj(prj, det_center, det_u, det_v, src, geo, img_shape):
-    return _arb_bp("dd_cone_adj", prj, [det_center, det_u, det_v, src], geo, img_shape, "f64")
+def dd_cone_adj(prj, det_center, det_u, det_v, src, geo, img_shape, precision="f64"):
+    return _arb_bp("dd_cone_adj", prj, [det_center, det_u, det_v, src], geo, img_shape, precision)
 
 
 def dd_tomo_fp(img, det_center, src, geo, nu, nv, precision="f64"):

@@ -1239,6 +1239,12 @@ void orc_dd_cone_fp_f64(double* prj, const float* img, const float* C, const flo
 void orc_dd_cone_adj_f64(double* img, const float* prj, const float* C, const float* U, const float* V,
                          const float* src, ARB_ARGS)
 { dd_cone_adj<double>(img, prj, C, U, V, src, ARB_MAKE); }
+void orc_dd_cone_fp_f32(float* prj, const float* img, const float* C, const float* U, const float* V,
+                        const float* src, ARB_ARGS)
+{ dd_cone_fp<float>(prj, img, C, U, V, src, ARB_MAKE); }
+void orc_dd_cone_adj_f32(float* img, const float* prj, const float* C, const float* U, const float* V,
+                         const float* src, ARB_ARGS)
+{ dd_cone_adj<float>(img, prj, C, U, V, src, ARB_MAKE); }
 
 void orc_tv_iso_sqs_f64(double* s1, double* s2, double* var, const float* img, int nb, int nx, int ny, int nz, int nc,
                         int dims, double eps)

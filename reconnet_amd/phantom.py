@@ -8,7 +8,8 @@ rasterisation on the image grid (pixel centres at (i - (n-1)/2) d).
 
 import numpy as np
 
-__all__ = ["ellipse_params", "rasterize", "phantom", "phantom_batch", "add_poisson_noise"]
+__all__ = ["ellipse_params", "rasterize", "phantom", "phantom_batch", "add_poisson_noise", "ellipsoid_params",
+           "phantom3d"]
 
 
 def ellipse_params(seed, n_ellipses=10, r_max=110.0, axis_range=(5.0, 80.0), mu_max=0.04):
@@ -66,6 +67,52 @@ def phantom_batch(seeds, nx, ny, dx, dy, n_ellipses=10, supersample=4, device=No
 
         return torch.stack([phantom(s, nx, ny, dx, dy, n_ellipses, supersample, device) for s in seeds])
     return np.stack([phantom(s, nx, ny, dx, dy, n_ellipses, supersample) for s in seeds])
+
+
+def ellipsoid_params(seed, n_ellipsoids=12, r_max=110.0, z_max=50.0, axis_range=(5.0, 80.0),
+                     axis_z_range=(5.0, 40.0), mu_max=0.04):
+    """Seeded random ellipsoids: per ellipsoid a centre at radius r_max sqrt(U),
+    angle 2 pi U, height U[-z_max, z_max]; semi-axes U[5, 80] mm in-plane and
+    U[5, 40] mm along z; rotation about z U[0, pi); attenuation U[0, 0.04] /mm.
+    Rows (cx, cy, cz, a, b, c, rot, mu)."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((n_ellipsoids, 8), np.float64)
+    for k in range(n_ellipsoids):
+        r = r_max * np.sqrt(rng.uniform())
+        phi = 2 * np.pi * rng.uniform()
+        cz = rng.uniform(-z_max, z_max)
+        a, b = rng.uniform(*axis_range), rng.uniform(*axis_range)
+        c = rng.uniform(*axis_z_range)
+        rot = rng.uniform(0, np.pi)
+        mu = rng.uniform(0, mu_max)
+        out[k] = [r * np.cos(phi), r * np.sin(phi), cz, a, b, c, rot, mu]
+    return out
+
+
+def phantom3d(seed, nx, ny, nz, dx, dy, dz, n_ellipsoids=12, supersample=2, device=None):
+    """Volume [nx, ny, nz] (float32, voxel centres at (i - (n-1)/2) d) of
+    additive seeded ellipsoids, supersample^3 points per voxel; torch on
+    `device` if given, else numpy."""
+    import torch
+
+    dev = device if device is not None else "cpu"
+    dt = torch.float32
+    off = (torch.arange(supersample, dtype=dt, device=dev) + 0.5) / supersample - 0.5
+
+    def axis(n, d):
+        return ((torch.arange(n, dtype=dt, device=dev) - (n - 1) / 2)[:, None] + off[None, :]).reshape(-1) * d
+
+    xs, ys, zs = axis(nx, dx), axis(ny, dy), axis(nz, dz)
+    vol = torch.zeros((nx * supersample, ny * supersample, nz * supersample), dtype=dt, device=dev)
+    X, Y = xs[:, None], ys[None, :]
+    for cx, cy, cz, a, b, c, rot, mu in ellipsoid_params(seed, n_ellipsoids):
+        co, si = np.cos(rot), np.sin(rot)
+        u = ((X - cx) * co + (Y - cy) * si) / a
+        v = (-(X - cx) * si + (Y - cy) * co) / b
+        r2 = (u * u + v * v)[:, :, None] + (((zs - cz) / c) ** 2)[None, None, :]
+        vol += float(mu) * (r2 <= 1.0).to(dt)
+    vol = vol.reshape(nx, supersample, ny, supersample, nz, supersample).mean(dim=(1, 3, 5))
+    return vol if device is not None else vol.numpy()
 
 
 def add_poisson_noise(sino, i0, seed):
