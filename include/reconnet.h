@@ -288,9 +288,9 @@ int rn_autotune_enable(int on);
 int rn_autotune_report(char* buf, int len);
 int rn_autotune_reset(void);
 /* Force candidate `candidate` of kernel `kernel` (1 Siddon FP, 2 Siddon BP,
- * 3 DD FP, 4 DD adjoint, 5 DD BP, 7 DD FBP BP; 8 / 9 Siddon FP / BP and
- * 10 / 11 DD BP / FBP BP on the interleaved 4-slice layout used for 4 or more
- * slices) for every shape, bypassing the
+ * 3 DD FP, 4 DD adjoint, 5 DD BP, 7 DD FBP BP; 8 / 9 Siddon FP / BP,
+ * 10 / 11 DD BP / FBP BP and 12 DD FP on the interleaved 4-slice layout used
+ * for 4 or more slices) for every shape, bypassing the
  * timing; candidate < 0 removes the pin.  A launch of a kernel with fewer
  * candidates fails with an error.  Lets tests run every launch shape. */
 int rn_autotune_pin(int kernel, int candidate);

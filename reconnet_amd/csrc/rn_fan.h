@@ -48,6 +48,10 @@ struct FanK {
     // (A, B, L, 0) with L the path length per image line (dy / |cos(a - t)| or
     // dx / |sin(a - t)|).  Computed in FP64.
     const float4* sray;
+    // [nview][nu] the two edge rays of bin k in the same line form,
+    // (A_lo, B_lo, A_hi, B_hi) for the edges a -+ da/2 (the DD footprint on a
+    // line lies between them).  Computed in FP64.
+    const float4* dray;
     const int* vwide;    // [nview] 1: every Siddon line interval of the view has a positive FP32 width
     int unit_angle;      // 1: |rx / (ry + dso)| < 1 for every pixel centre and view (image well inside the circle)
 };
@@ -63,7 +67,7 @@ public:
     std::vector<float> deg;
     std::vector<int> cls;
 private:
-    DevBuf buf_, ray_;
+    DevBuf buf_, ray_, dray_;
     std::vector<float> host_;
 };
 
@@ -72,6 +76,10 @@ private:
 // line-driven DD FP: RX = row-averaged planes (yx), RY = column-averaged planes (xy)
 void launch_dd_fan_fp(const FanK& g, const float* RX, const float* RY, float* prj_vu,
                       int nslices, int view0, int nviews, hipStream_t st);
+// DD FP on padded quads of the averaged planes: QX [q][ny][qpitch(nx)],
+// QY [q][nx][qpitch(ny)]; writes the plain view-major sinograms [s][nviews][nu]
+void launch_dd_fan_fp_q(const FanK& g, const float4* QX, const float4* QY, float* prj_vu, int nslices, int view0,
+                        int nviews, hipStream_t st);
 // line-driven Siddon FP: Pyx, Pxy = the image planes in both layouts
 void launch_siddon_fan_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj_vu,
                           int nslices, int view0, int nviews, hipStream_t st);
@@ -140,7 +148,7 @@ bool launch_siddon_fan_bp_q_update(const FanK& g, const float4* prj_q, float* im
 // reference-parity DD BP of a quad pre-weighted sinogram [q][view][nu] into plain images
 void launch_dd_fan_bp_q(const FanK& g, const float4* wprj_q, float* img_xy, int nslices, bool fbp,
                         hipStream_t st);
-// plain [s][n1][n2] -> quads [q][n1][n2] and (if q_tr) the transposed quads [q][n2][n1];
+// plain [s][n1][n2] -> quads [q][n1][n2] (if q_same) and (if q_tr) the transposed quads [q][n2][n1];
 // padded: lines of the padded layout (pitch qpitch(n2), resp. qpitch(n1); the
 // pad cells are left as they are)
 void launch_pack_quads(const float* in, float4* q_same, float4* q_tr, int n1, int n2, int nslices, bool padded,

@@ -29,17 +29,82 @@ using namespace fan;
 // forward projectors (line driven)
 // ---------------------------------------------------------------------------
 
-template <int BLK>
+// XCD > 0: 1-D grid whose workgroups are renumbered per XCD (as in
+// k_siddon_fan_fp); S slices per lane share the ray's lines and shares.
+template <int BLK, int XCD, int S>
 __global__ __launch_bounds__(BLK) void k_dd_fan_fp(FanK g, const float* __restrict__ RX,
                                                    const float* __restrict__ RY, float* __restrict__ prj,
-                                                   int view0, int nviews)
+                                                   int view0, int nviews, int ns)
 {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int vi = blockIdx.y;
-    const int s = blockIdx.z;
-    if (iu >= g.nu) return;
+    int tile, vi, sg;
+    if (XCD > 0) {
+        const int per = gridDim.x / XCD;
+        const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
+        const int tiles = (g.nu + BLK - 1) / BLK;
+        tile = l % tiles;
+        vi = (l / tiles) % nviews;
+        sg = l / (tiles * nviews);
+    } else {
+        tile = blockIdx.x;
+        vi = blockIdx.y;
+        sg = blockIdx.z;
+    }
+    const int iu = tile * BLK + threadIdx.x;
+    const bool active = iu < g.nu;
     const long plane = (long)g.nx * g.ny;
-    prj[((long)s * nviews + vi) * g.nu + iu] = dd_fp_ray(g, RX + s * plane, RY + s * plane, view0 + vi, iu);
+    const int s0 = sg * S;
+    const int nsl = min(S, ns - s0);
+    float acc[S];
+    dd_fp_ray_slices<float, S>(g, RX + s0 * plane, RY + s0 * plane, plane, nsl, view0 + vi,
+                               active ? iu : g.nu - 1, active, acc);
+    if (!active) return;
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        if (q >= nsl) break;
+        prj[((long)(s0 + q) * nviews + vi) * g.nu + iu] = acc[q];
+    }
+}
+
+// DD FP of G interleaved groups of 4 slices per lane (padded quad averaged
+// planes), H parts of a wave per ray; writes the plain sinograms of the real
+// slices [s][nviews][nu].
+template <int BLK, int XCD, int G, int H>
+__global__ __launch_bounds__(BLK) void k_dd_fan_fp_q(FanK g, const float4* __restrict__ QX,
+                                                     const float4* __restrict__ QY, float* __restrict__ prj,
+                                                     int view0, int nviews, int ns)
+{
+    int tile, vi, sg;
+    if (XCD > 0) {
+        const int per = gridDim.x / XCD;
+        const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
+        const int tiles = (g.nu + BLK / H - 1) / (BLK / H);
+        tile = l % tiles;
+        vi = (l / tiles) % nviews;
+        sg = l / (tiles * nviews);
+    } else {
+        tile = blockIdx.x;
+        vi = blockIdx.y;
+        sg = blockIdx.z;
+    }
+    const int iu = tile * (BLK / H) + (int)(threadIdx.x >> 6) * (64 / H) + (int)(threadIdx.x & (64 / H - 1));
+    const bool active = iu < g.nu;
+    const int nquads = (ns + 3) / 4;
+    const int q0 = sg * G;
+    const int nq = min(G, nquads - q0);
+    float4 acc[G];
+    dd_fp_ray_quads<G, H>(g, QX + (long)q0 * g.ny * qpitch(g.nx), QY + (long)q0 * g.nx * qpitch(g.ny), nq,
+                          view0 + vi, active ? iu : g.nu - 1, active, acc);
+    if (!active || (threadIdx.x & 63) >= 64 / H) return;
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= nq) break;
+        const float v[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+        #pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int sl = 4 * (q0 + k) + c;
+            if (sl < ns) prj[((long)sl * nviews + vi) * g.nu + iu] = v[c];
+        }
+    }
 }
 
 // With meas != nullptr the epilogue writes the weighted residual
@@ -658,7 +723,7 @@ __global__ __launch_bounds__(256) void k_pack_quads(const float* __restrict__ in
             }
             const float4 f = make_float4(v[0], v[1], v[2], v[3]);
             tile[r][tx] = f;
-            q_same[(long)q * n1 * p_same + (long)i * p_same + off + j] = f;
+            if (q_same != nullptr) q_same[(long)q * n1 * p_same + (long)i * p_same + off + j] = f;
         }
     }
     if (q_tr == nullptr) return;
@@ -847,6 +912,34 @@ void FanTables::build(const FanGeo& g, const float* deg_host, hipStream_t st)
                                 hipMemcpyHostToDevice, st));
     RN_HIP_CHECK(hipStreamSynchronize(st));
     k.sray = reinterpret_cast<const float4*>(ray_.get());
+    // DD bin-edge rays per (view, bin), FP64 then rounded: edge e has angle
+    // (-(e - cu) - off_a) da + da / 2, so bin k lies between edges k and k + 1
+    // (angles a_k + da/2 and a_k - da/2)
+    std::vector<double> eA((size_t)nu + 1), eB((size_t)nu + 1);
+    std::vector<float4> dray((size_t)nview * nu);
+    for (int v = 0; v < nview; v++) {
+        const double t = (double)deg_host[v];
+        for (int e = 0; e <= nu; e++) {
+            const double a = (-(e - cu) - off) * da + da / 2.0;
+            const double c = std::cos(a - t), sn = std::sin(a - t), sa = std::sin(a);
+            double A, B;
+            if (cls[v] == 0) {
+                A = 0.5 * g.nx + ((double)g.dso / g.dx) * sa / c;
+                B = ((double)g.dy / g.dx) * sn / c;
+            } else {
+                A = 0.5 * g.ny - ((double)g.dso / g.dy) * sa / sn;
+                B = ((double)g.dx / g.dy) * c / sn;
+            }
+            eA[e] = A;
+            eB[e] = B;
+        }
+        for (int kb = 0; kb < nu; kb++)
+            dray[(size_t)v * nu + kb] = make_float4((float)eA[kb], (float)eB[kb], (float)eA[kb + 1], (float)eB[kb + 1]);
+    }
+    dray_.alloc(dray.size() * sizeof(float4));
+    RN_HIP_CHECK(hipMemcpyAsync(dray_.get(), dray.data(), dray.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipStreamSynchronize(st));
+    k.dray = reinterpret_cast<const float4*>(dray_.get());
     // pixel centres lie within R of the centre: |rx| / (dso - |ry|) <= R / (dso - R) < 1 if R < 0.45 dso
     const double R = 0.5 * std::sqrt((double)g.nx * g.dx * g.nx * g.dx + (double)g.ny * g.dy * g.ny * g.dy);
     k.unit_angle = R < 0.45 * (double)g.dso ? 1 : 0;
@@ -864,16 +957,38 @@ bool FanTables::matches(const FanGeo& g, const float* deg_host) const
 
 namespace {
 
-template <int BLK>
+// MI355X: 8 XCDs, each with its own L2
+constexpr int kXcds = 8;
+
+template <int BLK, int XCD, int S>
 void go_dd_fp(const FanK& g, const float* RX, const float* RY, float* prj, int ns, int view0, int nviews,
               hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dd_fan_fp<BLK>), dim3(cdiv(g.nu, BLK), nviews, ns), dim3(BLK), 0, st, g, RX, RY, prj,
-                       view0, nviews);
+    const int ngroups = cdiv(ns, S);
+    const long nblk = (long)cdiv(g.nu, BLK) * nviews * ngroups;
+    if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
+        hipLaunchKernelGGL((k_dd_fan_fp<BLK, XCD, S>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, RX, RY, prj, view0,
+                           nviews, ns);
+    } else {
+        hipLaunchKernelGGL((k_dd_fan_fp<BLK, 0, S>), dim3(cdiv(g.nu, BLK), nviews, ngroups), dim3(BLK), 0, st, g, RX,
+                           RY, prj, view0, nviews, ns);
+    }
 }
 
-// MI355X: 8 XCDs, each with its own L2
-constexpr int kXcds = 8;
+template <int BLK, int XCD, int G, int H = 1>
+void go_dd_fp_q(const FanK& g, const float4* QX, const float4* QY, float* prj, int ns, int view0, int nviews,
+                hipStream_t st)
+{
+    const int ngroups = cdiv(cdiv(ns, 4), G);
+    const long nblk = (long)cdiv(g.nu, BLK / H) * nviews * ngroups;
+    if (XCD > 0 && nblk % XCD == 0 && nblk <= 0x7fffffffL) {
+        hipLaunchKernelGGL((k_dd_fan_fp_q<BLK, XCD, G, H>), dim3((unsigned)nblk), dim3(BLK), 0, st, g, QX, QY, prj,
+                           view0, nviews, ns);
+    } else {
+        hipLaunchKernelGGL((k_dd_fan_fp_q<BLK, 0, G, H>), dim3(cdiv(g.nu, BLK / H), nviews, ngroups), dim3(BLK), 0, st,
+                           g, QX, QY, prj, view0, nviews, ns);
+    }
+}
 
 template <int BLK, int XCD, int S>
 void go_siddon_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj, int ns, int view0, int nviews,
@@ -919,7 +1034,16 @@ using GatherFn = void (*)(const FanK&, const float*, float*, int, int, int, int,
 using DdBpFn = void (*)(const FanK&, const float*, float*, int, int, hipStream_t);
 
 // 1-D blocks along the detector; 2-D pixel tiles (x lanes along y, the contiguous image axis)
-const DdFpFn kDdFp[] = {go_dd_fp<128>, go_dd_fp<64>, go_dd_fp<256>};
+const DdFpFn kDdFp[] = {go_dd_fp<128, kXcds, 1>, go_dd_fp<64, kXcds, 1>, go_dd_fp<256, kXcds, 1>,
+                        go_dd_fp<128, 0, 1>,     go_dd_fp<64, 0, 1>,     go_dd_fp<128, kXcds, 2>,
+                        go_dd_fp<64, kXcds, 2>,  go_dd_fp<128, 0, 2>};
+constexpr int kNDdFp = sizeof(kDdFp) / sizeof(kDdFp[0]);
+using DdFpQFn = void (*)(const FanK&, const float4*, const float4*, float*, int, int, int, hipStream_t);
+const DdFpQFn kDdFpQ[] = {go_dd_fp_q<64, kXcds, 1, 2>,  go_dd_fp_q<128, kXcds, 1, 2>, go_dd_fp_q<256, kXcds, 1, 2>,
+                          go_dd_fp_q<64, kXcds, 1, 4>,  go_dd_fp_q<128, kXcds, 1>,    go_dd_fp_q<64, kXcds, 1>,
+                          go_dd_fp_q<256, kXcds, 1>,    go_dd_fp_q<128, 0, 1>,        go_dd_fp_q<64, 0, 1>,
+                          go_dd_fp_q<128, kXcds, 2>,    go_dd_fp_q<64, kXcds, 2, 2>,  go_dd_fp_q<128, 0, 2>};
+constexpr int kNDdFpQ = sizeof(kDdFpQ) / sizeof(kDdFpQ[0]);
 // S: slices per lane (1 = one slice per lane; larger S shares the geometry work)
 const SidFpFn kSidFp[] = {go_siddon_fp<128, 0, 1>,     go_siddon_fp<64, 0, 1>,     go_siddon_fp<256, 0, 1>,
                           go_siddon_fp<128, kXcds, 1>, go_siddon_fp<64, kXcds, 1>, go_siddon_fp<256, kXcds, 1>,
@@ -1045,10 +1169,24 @@ void launch_dd_fan_fp(const FanK& g, const float* RX, const float* RY, float* pr
                       int nslices, int view0, int nviews, hipStream_t st)
 {
     if (nslices <= 0 || nviews <= 0) return;
-    const int c = tune_choose({TK_DD_FP, g.nx, g.ny, g.nu, nviews, nslices}, 3,
+    require_offsets32(g, nviews, sizeof(float));
+    const int c = tune_choose({TK_DD_FP, g.nx, g.ny, g.nu, nviews, nslices}, kNDdFp,
                               [&](int i) { kDdFp[i](g, RX, RY, prj_vu, nslices, view0, nviews, st); }, st, true);
     kDdFp[c](g, RX, RY, prj_vu, nslices, view0, nviews, st);
     check_launch("dd_fan_fp");
+}
+
+void launch_dd_fan_fp_q(const FanK& g, const float4* QX, const float4* QY, float* prj_vu, int nslices, int view0,
+                        int nviews, hipStream_t st)
+{
+    if (nslices <= 0 || nviews <= 0) return;
+    require_offsets32(g, nviews, sizeof(float));
+    RN_REQUIRE((long)std::max(g.nx * qpitch(g.ny), g.ny * qpitch(g.nx)) * 16L < 0x7fffffffL,
+               "padded image plane of one slice group exceeds 2 GiB");
+    const int c = tune_choose({TK_DD_FP_Q, g.nx, g.ny, g.nu, nviews, nslices}, kNDdFpQ,
+                              [&](int i) { kDdFpQ[i](g, QX, QY, prj_vu, nslices, view0, nviews, st); }, st, true);
+    kDdFpQ[c](g, QX, QY, prj_vu, nslices, view0, nviews, st);
+    check_launch("dd_fan_fp_q");
 }
 
 void launch_siddon_fan_fp(const FanK& g, const float* Pyx, const float* Pxy, float* prj_vu,

@@ -202,42 +202,6 @@ __device__ __forceinline__ float half_width_bins(float r, float dist, float inv_
 // per-ray forward projections and per-pixel gathers
 // ---------------------------------------------------------------------------
 
-// DDFPFanKernelX/Y (DistanceDriven.cu:64-196) for bin iu of a view: line y = iy
-// (X views) averages rows iy-1 and iy; the footprint between the two bin-edge
-// rays is integrated exactly on every line; result times dy/|cos(t-a)| (X) or
-// dx/|sin(t-a)| (Y).  RX: row-averaged plane [ny][nx]; RY: column-averaged [nx][ny].
-__device__ __forceinline__ float dd_fp_ray(const FanK& g, const float* __restrict__ RX,
-                                           const float* __restrict__ RY, int view, int iu)
-{
-    const float ct = g.vcos[view], st = g.vsin[view];
-    float acc = 0.0f;
-    float A0, B0, A1, B1, c0, c1;
-    if (g.vcls[view] == 0) {
-        xline(g, ct, st, g.esin[iu], g.ecos[iu], A0, B0, c0);
-        xline(g, ct, st, g.esin[iu + 1], g.ecos[iu + 1], A1, B1, c1);
-        const float y0 = -0.5f * g.ny;
-        for (int iy = 0; iy < g.ny; ++iy) {
-            const float yr = y0 + (float)iy;
-            const float xa = fmaf(B0, yr, A0), xb = fmaf(B1, yr, A1);
-            acc += seg_avg(RX + (long)iy * g.nx, g.nx, fminf(xa, xb), fmaxf(xa, xb));
-        }
-        const float cuc = g.bcos[iu] * ct + g.bsin[iu] * st;
-        acc *= g.dy / fabsf(cuc);
-    } else {
-        yline(g, ct, st, g.esin[iu], g.ecos[iu], A0, B0, c0);
-        yline(g, ct, st, g.esin[iu + 1], g.ecos[iu + 1], A1, B1, c1);
-        const float x0 = -0.5f * g.nx;
-        for (int ix = 0; ix < g.nx; ++ix) {
-            const float xr = x0 + (float)ix;
-            const float ya = fmaf(B0, xr, A0), yb = fmaf(B1, xr, A1);
-            acc += seg_avg(RY + (long)ix * g.ny, g.ny, fminf(ya, yb), fmaxf(ya, yb));
-        }
-        const float suc = g.bsin[iu] * ct - g.bcos[iu] * st;
-        acc *= g.dx / fabsf(suc);
-    }
-    return acc;
-}
-
 // Sum over the cells of a line of n cells (zero outside [0, n)) of value times
 // overlap with [lo, hi] (no division: the caller scales by L / |B|).
 template <class T>
@@ -575,6 +539,224 @@ __device__ __forceinline__ void siddon_fp_ray_quads(const FanK& g, const float4*
     }
     #pragma unroll
     for (int k = 0; k < G; ++k) acc[k] = vscale(acc[k], r.z);
+}
+
+// ---- distance-driven forward projection on the line machinery ----
+//
+// DDFPFanKernelX/Y (DistanceDriven.cu:64-196): on line l (image row iy for X
+// views, column ix for Y views; line coordinate lr = l - nl/2) the footprint
+// of bin iu lies between its two edge rays, [e0, e1] with e_k = A_k + B_k lr
+// (the FP64 table g.dray), and the ray's value adds the average of the
+// line's row-averaged plane over that footprint; the sum is scaled by the
+// path length per line L = dy/|cos(a - t)| (X) or dx/|sin(a - t)| (Y), the
+// Siddon table's L.  The footprint average is the share arithmetic of the
+// Siddon line intervals (overlap / width, the width unclipped), so the same
+// branch-free two- or three-cell line code serves both projectors.
+
+// Lines whose footprint can meet [0, n]: the union over the two edges of the
+// lines where the edge lies in [-2, n + 2] (one line of margin on each side;
+// the other lines add exact zeros while the footprint is narrower than 2).
+__device__ __forceinline__ void dd_edge_lines(float A, float B, int n, int nl, float base, int& j0, int& j1)
+{
+    j0 = 0;
+    j1 = nl - 1;
+    if (B != 0.0f) {
+        const float ib = 1.0f / B;
+        const float ta = (-2.0f - A) * ib - base, tb = ((float)n + 2.0f - A) * ib - base;
+        const float tlo = fminf(fmaxf(fminf(ta, tb) - 1.0f, 0.0f), (float)nl);
+        const float thi = fminf(fmaxf(ta, tb) + 1.0f, (float)(nl - 1));
+        j0 = (int)tlo;
+        j1 = thi >= tlo ? (int)thi : -1;
+    } else if (!(A >= -2.0f && A <= (float)n + 2.0f)) {
+        j0 = nl;
+        j1 = -1;
+    }
+}
+
+// Line range of bin ray d = (A0, B0, A1, B1) and the largest footprint width
+// over it (0 if the range is empty).
+__device__ __forceinline__ void dd_ray_lines(float4 d, int n, int nl, float base, int& j0, int& j1, float& wmax)
+{
+    int a0, a1, b0, b1;
+    dd_edge_lines(d.x, d.y, n, nl, base, a0, a1);
+    dd_edge_lines(d.z, d.w, n, nl, base, b0, b1);
+    j0 = min(a0, b0);
+    j1 = max(a1, b1);
+    wmax = 0.0f;
+    if (j1 >= j0) {
+        const float dA = d.z - d.x, dB = d.w - d.y;
+        wmax = fmaxf(fabsf(fmaf(dB, base + (float)j0, dA)), fabsf(fmaf(dB, base + (float)j1, dA)));
+    }
+}
+
+// Plain planes: lines wj0 .. wj1 of S slices, footprints at most NC cells wide
+template <class T, int S, int NC>
+__device__ __forceinline__ void dd_narrow_lines(const T* const (&Pq)[S], int n, float4 d, float base, int wj0,
+                                                int wj1, T (&acc)[S])
+{
+    constexpr int kUnroll = S > 4 ? 1 : (S > 2 ? 2 : 4);
+    #pragma unroll kUnroll
+    for (int j = wj0; j <= wj1; ++j) {
+        const float lr = base + (float)j;
+        const float ea = fmaf(d.y, lr, d.x), eb = fmaf(d.w, lr, d.z);
+        const Line3 c = line3(n, fminf(ea, eb), fmaxf(ea, eb));
+        const unsigned lboff = (unsigned)(j * n) * (unsigned)sizeof(T);
+        #pragma unroll
+        for (int q = 0; q < S; ++q) line_madd_at<T, NC>(acc[q], Pq[q], lboff, c);
+    }
+}
+
+// One bin of one view through S consecutive slices of the row-averaged planes
+// RX [s][ny][nx] (X views) and RY [s][nx][ny] (Y views); slices q >= nsl read
+// slice nsl - 1.  Wave-uniform line loop as in siddon_fp_ray_slices.
+template <class T, int S>
+__device__ __forceinline__ void dd_fp_ray_slices(const FanK& g, const T* __restrict__ RX, const T* __restrict__ RY,
+                                                 long plane, int nsl, int view, int iu, bool active, T (&acc)[S])
+{
+    const float4 d = g.dray[(long)view * g.nu + iu];
+    const float L = g.sray[(long)view * g.nu + iu].z;
+    const bool xv = g.vcls[view] == 0;
+    const T* __restrict__ P = xv ? RX : RY;
+    const int nl = xv ? g.ny : g.nx;
+    const int n = xv ? g.nx : g.ny;
+    const float base = -0.5f * (float)nl;
+    int j0, j1;
+    float wmax;
+    dd_ray_lines(d, n, nl, base, j0, j1, wmax);
+    if (!active) { j0 = nl; j1 = -1; wmax = 0.0f; }
+    const int wj0 = wave_min(j0);
+    const int wj1 = wave_max(j1);
+    const T* Pq[S];
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        Pq[q] = P + (long)min(q, nsl - 1) * plane;
+        acc[q] = vzero<T>();
+    }
+    if (__all(wmax <= 0.999f)) {
+        dd_narrow_lines<T, S, 2>(Pq, n, d, base, wj0, wj1, acc);
+    } else if (__all(wmax <= 1.999f)) {
+        dd_narrow_lines<T, S, 3>(Pq, n, d, base, wj0, wj1, acc);
+    } else {
+        for (int j = wj0; j <= wj1; ++j) {
+            const float lr = base + (float)j;
+            const float ea = fmaf(d.y, lr, d.x), eb = fmaf(d.w, lr, d.z);
+            #pragma unroll
+            for (int q = 0; q < S; ++q)
+                vadd(acc[q], vscale(seg_sum<T>(Pq[q] + (long)j * n, n, fminf(ea, eb), fmaxf(ea, eb)),
+                                    1.0f / fabsf(eb - ea)));
+        }
+    }
+    #pragma unroll
+    for (int q = 0; q < S; ++q) acc[q] = vscale(acc[q], L);
+}
+
+// Padded quad planes (see fp_quad_lines_padded): G groups, footprints at most
+// NC cells wide
+template <int G, int NC>
+__device__ __forceinline__ void dd_quad_lines_padded(const float4* const (&Pq)[G], int n, int pitch, float4 d,
+                                                     float base, int wj0, int wj1, float4 (&acc)[G])
+{
+    #pragma unroll 2
+    for (int j = wj0; j <= wj1; ++j) {
+        const float lr = base + (float)j;
+        const float ea = fmaf(d.y, lr, d.x), eb = fmaf(d.w, lr, d.z);
+        const float lo = fminf(ea, eb), hi = fmaxf(ea, eb);
+        const float w = hi - lo;
+        const float f0 = floorf(lo);
+        const int m0 = (int)__builtin_amdgcn_fmed3f(f0, -(float)kQPadL, (float)(n + kQPadR - NC));
+        const bool pos = w > 0.0f;
+        const float inv = __builtin_amdgcn_rcpf(pos ? w : 1.0f);
+        const float o0 = fminf(hi, f0 + 1.0f) - lo;
+        float w0, w1, w2 = 0.0f;
+        if (NC == 2) {
+            w0 = (pos ? o0 : 1.0f) * inv;
+            w1 = (w - o0) * inv;
+        } else {
+            const float o1 = fmaxf(fminf(hi, f0 + 2.0f) - (f0 + 1.0f), 0.0f);
+            w0 = (pos ? o0 : 1.0f) * inv;
+            w1 = o1 * inv;
+            w2 = (w - o0 - o1) * inv;
+        }
+        const unsigned boff = (unsigned)(j * pitch + kQPadL + m0) * 16u;
+        #pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const float4 v0 = ld_at(Pq[k], boff), v1 = ld_at(Pq[k], boff + 16u);
+            float4& a = acc[k];
+            a.x = fmaf(v0.x, w0, a.x); a.y = fmaf(v0.y, w0, a.y); a.z = fmaf(v0.z, w0, a.z); a.w = fmaf(v0.w, w0, a.w);
+            a.x = fmaf(v1.x, w1, a.x); a.y = fmaf(v1.y, w1, a.y); a.z = fmaf(v1.z, w1, a.z); a.w = fmaf(v1.w, w1, a.w);
+            if (NC == 3) {
+                const float4 v2 = ld_at(Pq[k], boff + 32u);
+                a.x = fmaf(v2.x, w2, a.x); a.y = fmaf(v2.y, w2, a.y);
+                a.z = fmaf(v2.z, w2, a.z); a.w = fmaf(v2.w, w2, a.w);
+            }
+        }
+    }
+}
+
+// One bin ray through G interleaved slice groups of the padded quad averaged
+// planes QX [q][ny][qpitch(nx)] (X views) and QY [q][nx][qpitch(ny)] (Y views);
+// H parts of 64 / H lanes split the wave's line range as in
+// siddon_fp_ray_quads.
+template <int G, int H = 1>
+__device__ __forceinline__ void dd_fp_ray_quads(const FanK& g, const float4* __restrict__ QX,
+                                                const float4* __restrict__ QY, int nq, int view, int iu, bool active,
+                                                float4 (&acc)[G])
+{
+    const float4 d = g.dray[(long)view * g.nu + iu];
+    const float L = g.sray[(long)view * g.nu + iu].z;
+    const bool xv = g.vcls[view] == 0;
+    const float4* __restrict__ P = xv ? QX : QY;
+    const int nl = xv ? g.ny : g.nx;
+    const int n = xv ? g.nx : g.ny;
+    const int pitch = qpitch(n);
+    const long qstride = (long)nl * pitch;
+    const float base = -0.5f * (float)nl;
+    int j0, j1;
+    float wmax;
+    dd_ray_lines(d, n, nl, base, j0, j1, wmax);
+    if (!active) { j0 = nl; j1 = -1; wmax = 0.0f; }
+    int wj0 = wave_min(j0);
+    int wj1 = wave_max(j1);
+    static_assert(H == 1 || H == 2 || H == 4, "parts of a wave");
+    const int len = wj1 - wj0 + 1;
+    if (H > 1 && len > 0) {
+        const int part = (int)(threadIdx.x & 63) / (64 / H);
+        const int a = wj0 + len * part / H, b = wj0 + len * (part + 1) / H - 1;
+        wj0 = a;
+        wj1 = b;
+    }
+    const float4* Pq[G];
+    #pragma unroll
+    for (int k = 0; k < G; ++k) {
+        Pq[k] = P + (long)min(k, nq - 1) * qstride;
+        acc[k] = vzero<float4>();
+    }
+    if (__all(wmax <= 0.999f)) {
+        dd_quad_lines_padded<G, 2>(Pq, n, pitch, d, base, wj0, wj1, acc);
+    } else if (__all(wmax <= 1.999f)) {
+        dd_quad_lines_padded<G, 3>(Pq, n, pitch, d, base, wj0, wj1, acc);
+    } else {
+        for (int j = wj0; j <= wj1; ++j) {
+            const float lr = base + (float)j;
+            const float ea = fmaf(d.y, lr, d.x), eb = fmaf(d.w, lr, d.z);
+            #pragma unroll
+            for (int k = 0; k < G; ++k)
+                vadd(acc[k], vscale(seg_sum<float4>(Pq[k] + (long)j * pitch + kQPadL, n, fminf(ea, eb),
+                                                    fmaxf(ea, eb)), 1.0f / fabsf(eb - ea)));
+        }
+    }
+    #pragma unroll
+    for (int o = 32; o >= 64 / H; o >>= 1) {
+        #pragma unroll
+        for (int k = 0; k < G; ++k) {
+            acc[k].x += __shfl_xor(acc[k].x, o);
+            acc[k].y += __shfl_xor(acc[k].y, o);
+            acc[k].z += __shfl_xor(acc[k].z, o);
+            acc[k].w += __shfl_xor(acc[k].w, o);
+        }
+    }
+    #pragma unroll
+    for (int k = 0; k < G; ++k) acc[k] = vscale(acc[k], L);
 }
 
 // One ray of one slice (siddon_fp_ray_slices with S = 1).

@@ -112,7 +112,21 @@ void fan_forward(const FanTables& t, FanFP kind, int nB, int nC, const float* im
         P = w.planes.as<float>();
     }
     float* out_vu = w.sino.as<float>();
-    if (kind == FanFP::DD) {
+    if (kind == FanFP::DD && ns >= 4) {
+        // groups of 4 slices interleaved, padded lines (zero pads):
+        // QY[q][x][pad + y] = (P[x-1][y] + P[x][y]) / 2,  QX[q][y][pad + x] = (P[x][y-1] + P[x][y]) / 2
+        const size_t qx = (size_t)g.ny * qpitch(g.nx), qy = (size_t)g.nx * qpitch(g.ny);
+        const int nq = cdiv(ns, 4);
+        w.qdd.ensure((qx + qy) * nq * sizeof(float4));
+        float4* QX = w.qdd.as<float4>();
+        float4* QY = QX + qx * nq;
+        launch_fill(w.qdd.as<float>(), 0.0f, (qx + qy) * nq * 4, st);
+        launch_avg_prev(P, w.ry.as<float>(), g.nx, g.ny, ns, 0, st);
+        launch_pack_quads(w.ry.as<float>(), QY, nullptr, g.nx, g.ny, ns, true, st);
+        launch_avg_prev(P, w.rx.as<float>(), g.nx, g.ny, ns, 1, st);
+        launch_pack_quads(w.rx.as<float>(), nullptr, QX, g.nx, g.ny, ns, true, st);
+        launch_dd_fan_fp_q(t.k, QX, QY, out_vu, ns, 0, g.nview, st);
+    } else if (kind == FanFP::DD) {
         // RY[x][y] = (P[x-1][y] + P[x][y]) / 2 ;  RX = transpose((P[x][y-1] + P[x][y]) / 2)
         launch_avg_prev(P, w.ry.as<float>(), g.nx, g.ny, ns, 0, st);
         launch_avg_prev(P, out_vu, g.nx, g.ny, ns, 1, st);

@@ -10,7 +10,7 @@ enum class FanBP { DD, DDFBP, DDAdjoint, Siddon, Pixel };
 
 // Scratch owned by a caller (plan or ABI call); grows on demand.
 struct FanWork {
-    DevBuf planes, rx, ry, sino, mix, quad;
+    DevBuf planes, rx, ry, sino, mix, quad, qdd;
 };
 
 // img_ref [nB][nx][ny][nz][nC] -> prj_ref [nB][nu][nview][nv][nC] (overwritten)

@@ -56,13 +56,16 @@ __global__ __launch_bounds__(256) void k_ragged_fp(const RagSlice* __restrict__ 
         const int local = item - d.item0;
         const int view = local / d.tiles, iu = (local % d.tiles) * 64 + lane;
         const bool active = iu < d.g.nu;
+        // every lane takes part (wave-uniform line loops)
+        float v;
         if (siddon) {
-            // every lane takes part (wave-uniform line loop)
-            const float v = siddon_fp_ray<float>(d.g, d.pa, d.pb, view, active ? iu : d.g.nu - 1, active);
-            if (active) d.out[(long)iu * d.g.nview + view] = v;
-        } else if (active) {
-            d.out[(long)iu * d.g.nview + view] = dd_fp_ray(d.g, d.pa, d.pb, view, iu);
+            v = siddon_fp_ray<float>(d.g, d.pa, d.pb, view, active ? iu : d.g.nu - 1, active);
+        } else {
+            float acc[1];
+            dd_fp_ray_slices<float, 1>(d.g, d.pa, d.pb, 1, 1, view, active ? iu : d.g.nu - 1, active, acc);
+            v = acc[0];
         }
+        if (active) d.out[(long)iu * d.g.nview + view] = v;
     }
 }
 

@@ -40,6 +40,7 @@ enum TuneKernel {
     TK_SIDDON_BP_Q = 9,
     TK_DD_BP_Q = 10,
     TK_DD_FBP_Q = 11,
+    TK_DD_FP_Q = 12,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

@@ -44,9 +44,10 @@ def test_tuned_shapes_give_identical_results(fp, bp):
 
 @pytest.mark.parametrize("ns", [1, 3, 11])
 def test_every_launch_shape_agrees(ns):
-    # every candidate of the Siddon FP / BP and DD BP / FBP launches (block
-    # shapes, XCD order, slices per lane S = 1, 2, 4, 8) on slice counts that
-    # leave partial groups of S, against candidate 0 (one slice per lane)
+    # every candidate of the Siddon FP / BP and DD FP / BP / FBP launches
+    # (block shapes, XCD order, slices per lane S = 1, 2, 4, 8, groups of 4
+    # interleaved slices and line-range parts) on slice counts that leave
+    # partial groups, against candidate 0
     from reconnet_amd._lib import autotune, autotune_pin
     from reconnet_amd.torch_ops import FanPlan
 
@@ -59,17 +60,21 @@ def test_every_launch_shape_agrees(ns):
     try:
         autotune(enable=True, reset=True)
         plan.forward(x, "siddon")
+        plan.forward(x, "dd")
         plan.backward(y, "siddon")
         plan.backward(y, "dd")
         plan.backward(y, "dd_fbp")
         for (kernel, *_), (_, ms) in autotune().items():
             counts[kernel] = len(ms)
         assert counts.get(1, 0) >= 13 and counts.get(2, 0) >= 13
-        # 4 or more slices take the DD BP on the interleaved 4-slice layout
-        dd_bp, dd_fbp = (("dd_bp", 5), ("dd_fbp", 7)) if ns < 4 else (("dd_bp_q", 10), ("dd_fbp_q", 11))
+        # 4 or more slices take the DD FP and BP on the interleaved 4-slice layout
+        dd_fp, dd_bp, dd_fbp = ((("dd_fp", 3), ("dd_bp", 5), ("dd_fbp", 7)) if ns < 4 else
+                                (("dd_fp_q", 12), ("dd_bp_q", 10), ("dd_fbp_q", 11)))
         assert counts.get(dd_bp[1], 0) >= 5 and counts.get(dd_fbp[1], 0) >= 5, counts
+        assert counts.get(dd_fp[1], 0) >= 8, counts
         refs = {}
         for kname, kid, run in (("siddon_fp", 1, lambda: plan.forward(x, "siddon")),
+                                (dd_fp[0], dd_fp[1], lambda: plan.forward(x, "dd")),
                                 ("siddon_bp", 2, lambda: plan.backward(y, "siddon")),
                                 (dd_bp[0], dd_bp[1], lambda: plan.backward(y, "dd")),
                                 (dd_fbp[0], dd_fbp[1], lambda: plan.backward(y, "dd_fbp"))):
@@ -88,7 +93,7 @@ def test_every_launch_shape_agrees(ns):
         with pytest.raises(RuntimeError, match="out of range"):
             plan.forward(x, "siddon")
     finally:
-        for k in ("siddon_fp", "siddon_bp", "dd_bp", "dd_fbp", "dd_bp_q", "dd_fbp_q"):
+        for k in ("siddon_fp", "siddon_bp", "dd_fp", "dd_fp_q", "dd_bp", "dd_fbp", "dd_bp_q", "dd_fbp_q"):
             autotune_pin(k, None)
         autotune(enable=True, reset=True)
 

@@ -42,6 +42,21 @@ def test_dd_fp_config0(c0):
     assert rel_l2(out, ref) <= TOL
 
 
+def test_dd_fp_batch_interleaved_config0(c0):
+    # 5 slices of the config-0 geometry take the DD FP on padded quads of the
+    # averaged planes (one full and one partial group of 4): slices 0 and 4
+    # against the oracle
+    import torch
+    from reconnet_amd.torch_ops import FanPlan
+
+    geo, _, _, deg = c0
+    imgs = np.stack([phantom(40 + s, geo.nx, geo.ny, geo.dx, geo.dy) for s in range(5)]).astype(np.float32)
+    out = FanPlan(geo).forward(torch.from_numpy(imgs).cuda(), "dd").cpu().numpy()
+    for s in (0, 4):
+        ref = orc.dd_fan_fp(_img5(imgs[s]), deg, geo.oracle_geo(), geo.nu)[0, :, :, 0, 0]
+        assert rel_l2(out[s], ref) <= TOL, s
+
+
 def test_siddon_fp_config0(c0):
     geo, img, _, deg = c0
     out = recon_for(geo).cSiddonFanProjection4d(img)
