@@ -430,17 +430,26 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_siddon_fan_bp_ql(Fa
     }
 }
 
-template <int BX, int BY>
+template <int BX, int BY, int S, bool UNIT>
 __global__ __launch_bounds__(BX * BY) void k_dd_fan_adj(FanK g, const float* __restrict__ prj,
-                                                    float* __restrict__ img, int view0, int nviews, int accumulate)
+                                                    float* __restrict__ img, int view0, int nviews, int ns,
+                                                    int accumulate)
 {
     const int iy = blockIdx.x * BX + threadIdx.x;
     const int ix = blockIdx.y * BY + threadIdx.y;
-    const int s = blockIdx.z;
+    const int s0 = blockIdx.z * S;
+    const int nsl = min(S, ns - s0);
     if (ix >= g.nx || iy >= g.ny) return;
-    const float acc = dd_adj_pixel(g, prj + (long)s * nviews * g.nu, view0, nviews, ix, iy);
-    float* o = img + (long)s * g.nx * g.ny + (long)ix * g.ny + iy;
-    *o = accumulate ? *o + acc : acc;
+    const long sstride = (long)nviews * g.nu;
+    float acc[S];
+    dd_adj_pixel_slices<UNIT, S>(g, prj + s0 * sstride, sstride, nsl, view0, nviews, ix, iy, acc);
+    const long plane = (long)g.nx * g.ny;
+    #pragma unroll
+    for (int q = 0; q < S; ++q) {
+        if (q >= nsl) break;
+        float* o = img + (long)(s0 + q) * plane + (long)ix * g.ny + iy;
+        *o = accumulate ? *o + acc[q] : acc[q];
+    }
 }
 
 // PreweightBPKernelX/Y: length factor for the iterative (non-FBP) DD BP.
@@ -1013,11 +1022,16 @@ void go_siddon_bp(const FanK& g, const float* prj, float* img, int ns, int view0
                        dim3(BX, BY), 0, st, g, prj, img, view0, nviews, ns, acc);
 }
 
-template <int BX, int BY>
+template <int BX, int BY, int S>
 void go_dd_adj(const FanK& g, const float* prj, float* img, int ns, int view0, int nviews, int acc, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_dd_fan_adj<BX, BY>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), ns), dim3(BX, BY), 0, st, g,
-                       prj, img, view0, nviews, acc);
+    const dim3 grd(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(ns, S));
+    if (g.unit_angle)
+        hipLaunchKernelGGL((k_dd_fan_adj<BX, BY, S, true>), grd, dim3(BX, BY), 0, st, g, prj, img, view0, nviews, ns,
+                           acc);
+    else
+        hipLaunchKernelGGL((k_dd_fan_adj<BX, BY, S, false>), grd, dim3(BX, BY), 0, st, g, prj, img, view0, nviews,
+                           ns, acc);
 }
 
 template <int BX, int BY, int S>
@@ -1057,7 +1071,9 @@ const GatherFn kSidBp[] = {go_siddon_bp<16, 16, 3, 1>, go_siddon_bp<8, 8, 3, 1>,
                            go_siddon_bp<8, 8, 3, 8>,   go_siddon_bp<8, 8, 2, 16>,  go_siddon_bp<16, 16, 2, 16>};
 constexpr int kNSidFp = sizeof(kSidFp) / sizeof(kSidFp[0]);
 constexpr int kNSidBp = sizeof(kSidBp) / sizeof(kSidBp[0]);
-const GatherFn kDdAdj[] = {go_dd_adj<16, 16>, go_dd_adj<8, 32>, go_dd_adj<32, 8>, go_dd_adj<8, 8>};
+const GatherFn kDdAdj[] = {go_dd_adj<16, 16, 1>, go_dd_adj<8, 32, 1>, go_dd_adj<32, 8, 1>, go_dd_adj<8, 8, 1>,
+                           go_dd_adj<16, 16, 2>, go_dd_adj<8, 8, 2>,  go_dd_adj<16, 16, 4>, go_dd_adj<8, 8, 4>};
+constexpr int kNDdAdj = sizeof(kDdAdj) / sizeof(kDdAdj[0]);
 const DdBpFn kDdBp[] = {go_dd_bp<16, 16, 1>, go_dd_bp<8, 32, 1>, go_dd_bp<32, 8, 1>, go_dd_bp<8, 8, 1>,
                        go_dd_bp<8, 8, 4>,   go_dd_bp<16, 16, 4>, go_dd_bp<8, 8, 8>, go_dd_bp<16, 16, 8>,
                        go_dd_bp<8, 8, 16>};
@@ -1303,7 +1319,8 @@ void launch_dd_fan_adj(const FanK& g, const float* prj_vu, float* img_xy, int ns
                        int view0, int nviews, bool accumulate, hipStream_t st)
 {
     if (nslices <= 0) return;
-    const int c = tune_choose({TK_DD_ADJ, g.nx, g.ny, g.nu, nviews, nslices}, 4, [&](int i) {
+    require_offsets32(g, nviews, sizeof(float));
+    const int c = tune_choose({TK_DD_ADJ, g.nx, g.ny, g.nu, nviews, nslices}, kNDdAdj, [&](int i) {
         kDdAdj[i](g, prj_vu, img_xy, nslices, view0, nviews, 0, st);
     }, st, !accumulate);
     kDdAdj[c](g, prj_vu, img_xy, nslices, view0, nviews, accumulate ? 1 : 0, st);

@@ -940,54 +940,65 @@ __device__ __forceinline__ T siddon_bp_pixel(const FanK& g, const T* __restrict_
     return acc[0];
 }
 
-// Exact transpose of dd_fp_ray (no reference counterpart: the reference DD BP
-// is not the adjoint of its FP).  The pixel enters line iy (or ix) and line
-// iy+1 (or ix+1) with weight 1/2; on each it gathers the bins whose footprint
-// meets the pixel's segment.
-__device__ __forceinline__ float dd_adj_pixel(const FanK& g, const float* __restrict__ prj, int view0, int nviews,
-                                              int ix, int iy)
+// Exact transpose of the DD FP (dd_fp_ray_slices; no reference counterpart:
+// the reference DD BP is not the adjoint of its FP).  The pixel enters line
+// iy (or ix), its lower edge, and line iy + 1 (or ix + 1), its upper edge,
+// with weight 1/2 each (each line averages two rows); on each it takes, from
+// every bin whose footprint meets its segment, the FP's weight
+// L_k overlap / width from the same edge-ray table.  One pass over the
+// candidate bins serves both lines: the bins whose centre lies within half a
+// bin of the angles the pixel's disc spans (bin_range plus half a bin; the
+// others have zero overlap).  S slices share the weights (slice q's
+// view-major sinogram `sstride` elements after prj; q >= nsl re-reads the last).
+template <bool UNIT, int S>
+__device__ __forceinline__ void dd_adj_pixel_slices(const FanK& g, const float* __restrict__ prj, long sstride,
+                                                    int nsl, int view0, int nviews, int ix, int iy,
+                                                    float (&acc)[S])
 {
     const float inv_da = 1.0f / g.da;
-    float acc = 0.0f;
+    const float ainv = fabsf(inv_da);
+    #pragma unroll
+    for (int q = 0; q < S; ++q) acc[q] = 0.0f;
+    const float* pq[S];
+    #pragma unroll
+    for (int q = 0; q < S; ++q) pq[q] = prj + (long)min(q, nsl - 1) * sstride;
+    const float xc = ((float)ix - 0.5f * (g.nx - 1)) * g.dx, yc = ((float)iy - 0.5f * (g.ny - 1)) * g.dy;
+    const float rpix = 0.5f * sqrtf(g.dx * g.dx + g.dy * g.dy);
     for (int vi = 0; vi < nviews; ++vi) {
         const int view = view0 + vi;
         const float ct = g.vcos[view], st = g.vsin[view];
-        const float* p = prj + (long)vi * g.nu;
+        const float4* __restrict__ dr = g.dray + (long)view * g.nu;
+        const float4* __restrict__ sr = g.sray + (long)view * g.nu;
+        const long row = (long)vi * g.nu;
         const bool isx = g.vcls[view] == 0;
         const int l0 = isx ? iy : ix;
         const int nl = isx ? g.ny : g.nx;
-        const int cell = isx ? ix : iy;
-        for (int l = l0; l <= l0 + 1 && l < nl; ++l) {
-            // centre of the pixel's segment on this line, physical coordinates
-            float px, py, r;
-            if (isx) { px = (ix + 0.5f - 0.5f * g.nx) * g.dx; py = (l - 0.5f * g.ny) * g.dy; r = 0.5f * g.dx; }
-            else     { px = (l - 0.5f * g.nx) * g.dx; py = (iy + 0.5f - 0.5f * g.ny) * g.dy; r = 0.5f * g.dy; }
-            const float rx = px * ct + py * st, ry = -px * st + py * ct;
-            const float d0 = ry + g.dso;
-            const float uc = g.cu - g.off_a - atanf(rx / d0) * inv_da;
-            // bins whose footprint (centre +- half a bin) meets the segment
-            const float hw = half_width_bins(r, sqrtf(rx * rx + d0 * d0), inv_da) + 0.5f;
-            const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
-            if (k1 < k0) continue;
-            const float lr = isx ? (float)l - 0.5f * g.ny : (float)l - 0.5f * g.nx;
-            float A, B, c;
-            if (isx) xline(g, ct, st, g.esin[k0], g.ecos[k0], A, B, c);
-            else     yline(g, ct, st, g.esin[k0], g.ecos[k0], A, B, c);
-            float e_prev = fmaf(B, lr, A);
-            for (int k = k0; k <= k1; ++k) {
-                if (isx) xline(g, ct, st, g.esin[k + 1], g.ecos[k + 1], A, B, c);
-                else     yline(g, ct, st, g.esin[k + 1], g.ecos[k + 1], A, B, c);
-                const float e_next = fmaf(B, lr, A);
-                const float wgt = seg_weight(cell, fminf(e_prev, e_next), fmaxf(e_prev, e_next));
-                float L;
-                if (isx) L = g.dy / fabsf(g.bcos[k] * ct + g.bsin[k] * st);
-                else     L = g.dx / fabsf(g.bsin[k] * ct - g.bcos[k] * st);
-                acc = fmaf(0.5f * L * p[k], wgt, acc);
-                e_prev = e_next;
+        const float fcell = (float)(isx ? ix : iy);
+        const float up = l0 + 1 < nl ? 0.5f : 0.0f;     // the upper edge's line exists
+        const float rx = xc * ct + yc * st, ry = -xc * st + yc * ct;
+        const float d0 = ry + g.dso;
+        const float uc = g.cu - g.off_a - atan_fast<UNIT>(rx * __builtin_amdgcn_rcpf(d0)) * inv_da;
+        const float sn = fminf(rpix * __builtin_amdgcn_rsqf(fmaf(rx, rx, d0 * d0)), 1.0f);
+        const float hw = sn * fmaf(2.0f, sn, 1.0f) * ainv + 0.52f + RN_ATAN_FAST_ERR * ainv;
+        const int k0 = max((int)ceilf(uc - hw), 0), k1 = min((int)floorf(uc + hw), g.nu - 1);
+        const float lr = (float)l0 - 0.5f * (float)nl;
+        for (int k = k0; k <= k1; ++k) {
+            const float4 d = dr[k];
+            float sh[2];
+            #pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float le = lr + (float)e;
+                const float ea = fmaf(d.y, le, d.x), eb = fmaf(d.w, le, d.z);
+                const float lo = fminf(ea, eb), hi = fmaxf(ea, eb), w = hi - lo;
+                const float ov = fmaxf(fminf(hi, fcell + 1.0f) - fmaxf(lo, fcell), 0.0f);
+                const bool pos = w > 0.0f;
+                sh[e] = pos ? ov * __builtin_amdgcn_rcpf(w) : ((lo >= fcell && lo < fcell + 1.0f) ? 1.0f : 0.0f);
             }
+            const float wk = fmaf(up, sh[1], 0.5f * sh[0]) * sr[k].z;
+            #pragma unroll
+            for (int q = 0; q < S; ++q) acc[q] = fmaf(pq[q][row + k], wk, acc[q]);
         }
     }
-    return acc;
 }
 
 } // namespace fan

@@ -80,8 +80,15 @@ __global__ __launch_bounds__(256) void k_ragged_bp(const RagSlice* __restrict__ 
         const int local = item - d.item0;
         const int ix = (local / d.tiles) * 8 + (lane >> 3), iy = (local % d.tiles) * 8 + (lane & 7);
         if (ix < d.g.nx && iy < d.g.ny) {
-            const float v = siddon ? siddon_bp_pixel<float>(d.g, d.pa, 0, d.g.nview, ix, iy)
-                                   : dd_adj_pixel(d.g, d.pa, 0, d.g.nview, ix, iy);
+            float v;
+            if (siddon) {
+                v = siddon_bp_pixel<float>(d.g, d.pa, 0, d.g.nview, ix, iy);
+            } else {
+                float acc[1];
+                if (d.g.unit_angle) dd_adj_pixel_slices<true, 1>(d.g, d.pa, 0, 1, 0, d.g.nview, ix, iy, acc);
+                else dd_adj_pixel_slices<false, 1>(d.g, d.pa, 0, 1, 0, d.g.nview, ix, iy, acc);
+                v = acc[0];
+            }
             d.out[(long)ix * d.g.ny + iy] = v;
         }
     }
