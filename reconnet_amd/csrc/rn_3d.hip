@@ -372,80 +372,104 @@ __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const flo
 }
 
 // Exact transpose of k_dd_cone_fp: voxel (ix, iy, iz) gathers, per view, the
-// cells whose footprint on its plane overlaps it.  Candidate columns come
-// from inverting the plane crossing of the u-edge rays, u = (A - q Cc) /
-// (q D - B) with q the in-plane slope of the voxel edge; candidate rows from
-// the (linear) v mapping of the column's centre ray.  The weights are
-// recomputed with cone_ray, as in the forward pass.  Reference layouts.
+// cells whose footprint on its plane overlaps it, with the forward pass's
+// weight overlap / area * L.  A wave holds one (ix, iy) column of 64 voxels
+// along z, so the view's plane, the candidate detector columns and each
+// column's in-plane footprint [e1, e2] (the rays through its u edges, which
+// do not depend on v because the detector rows run along z) are
+// wave-uniform.  Candidate columns come from inverting the plane crossing of
+// the u-edge rays, u = (A - q Cc) / (q D - B) with q the in-plane slope of the
+// voxel edge.  Along a column the rows' z edges on plane p are affine in v,
+// z(v) = alpha + beta v (the forward pass's z-edge crossings), which gives
+// each lane its candidate rows and every row its footprint with a few FMAs.
+// Reference layouts.
 __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const float* __restrict__ prj,
                                                      float* __restrict__ img, const float* Cp, const float* Up,
                                                      const float* Vp, const float* Sp, const int* __restrict__ cls,
                                                      int nC)
 {
     const int iz = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int iy = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (int)(threadIdx.x >> 6));
     const int ix = blockIdx.z % g.nx, bc = blockIdx.z / g.nx;
     if (iz >= g.nz || iy >= g.ny) return;
     const int cc = bc % nC, b = bc / nC;
     const float* q = prj + (long)b * a.nu * a.nview * a.nv * nC + cc;
     const float cu = (a.nu - 1) / 2.0f + a.off_u, cv = (a.nv - 1) / 2.0f + a.off_v;
     const float margin = 0.05f;
+    const float fz = (float)iz;
+    const float idu = 1.0f / a.du, idv = 1.0f / a.dv, idz = 1.0f / g.dz;
     float acc = 0.f;
     for (int view = 0; view < a.nview; ++view) {
         const bool xv = cls[view] == 0;
         const F3 c = ld3(Cp, view), U = ld3(Up, view), V = ld3(Vp, view), s = ld3(Sp, view);
         const int p = xv ? ix : iy, j = xv ? iy : ix;
-        const int nm = xv ? g.nx : g.ny;
+        const int nm = xv ? g.nx : g.ny, nn = xv ? g.ny : g.nx;
         const float dm = xv ? g.dx : g.dy, dn = xv ? g.dy : g.dx;
+        const float idn = 1.0f / dn;
         const float sm = xv ? s.x : s.y, sn = xv ? s.y : s.x;
-        const float xp = ((float)p + 0.5f - 0.5f * nm) * dm + (xv ? g.cx : g.cy) - sm;   // plane, from the source
+        const float cn = xv ? g.cy : g.cx;
+        const float cm = xv ? c.x : c.y, cnd = xv ? c.y : c.x, Um = xv ? U.x : U.y, Un = xv ? U.y : U.x;
+        // plane p, relative to the source along the main axis
+        const float mp = ((float)p + 0.5f - 0.5f * nm) * dm + (xv ? g.cx : g.cy) - sm;
+        const float imp = __builtin_amdgcn_rcpf(mp);
         // u of the rays through the voxel's two in-plane edges
-        const float cn0 = (xv ? g.cy : g.cx) - 0.5f * (xv ? g.ny : g.nx) * dn;
-        const float A = (xv ? c.y : c.x) - sn, Bu = xv ? U.y : U.x, Cc = (xv ? c.x : c.y) - sm, D = xv ? U.x : U.y;
+        const float cn0 = cn - 0.5f * nn * dn;
+        const float A = cnd - sn, Cc = cm - sm;
         float ulo = 3.0e38f, uhi = -3.0e38f;
         #pragma unroll
         for (int e = 0; e < 2; ++e) {
-            const float qe = (cn0 + (float)(j + e) * dn - sn) / xp;
-            const float u = (A - qe * Cc) / (qe * D - Bu);
+            const float qe = (cn0 + (float)(j + e) * dn - sn) * imp;
+            const float u = (A - qe * Cc) * __builtin_amdgcn_rcpf(qe * Um - Un);
             ulo = fminf(ulo, u);
             uhi = fmaxf(uhi, u);
         }
-        const float flo = fmaxf(ulo / a.du + cu - 0.5f - margin, -1.0f);
-        const float fhi = fminf(uhi / a.du + cu + 0.5f + margin, (float)a.nu);
+        const float flo = fmaxf(ulo * idu + cu - 0.5f - margin, -1.0f);
+        const float fhi = fminf(uhi * idu + cu + 0.5f + margin, (float)a.nu);
         if (!(fhi >= flo)) continue;                // also rejects NaN
         const int u0 = max((int)ceilf(flo), 0), u1 = min((int)floorf(fhi), a.nu - 1);
         for (int iu = u0; iu <= u1; ++iu) {
-            // rows: z of the column's centre ray is linear in v
-            const float uc = (iu - cu) * a.du;
-            const float fx = fmaf(uc, xv ? U.x : U.y, xv ? c.x : c.y) - sm;
-            const float zc0 = fmaf(uc, U.z, c.z) - s.z;
-            const float zs = xp / fx;                    // plane distance / column distance
-            float vlo = 3.0e38f, vhi = -3.0e38f;
+            const float uc = ((float)iu - cu) * a.du;
+            // in-plane footprint of the column on plane p (index coordinates)
+            bool ok = true;
+            float ee[2];
             #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                const float zw = ((float)(iz + e) - 0.5f * g.nz) * g.dz + g.cz - s.z;
-                const float v = (zw / zs - zc0) / V.z;
-                vlo = fminf(vlo, v);
-                vhi = fmaxf(vhi, v);
+                const float ue = fmaf(e ? 0.5f : -0.5f, a.du, uc);
+                const float em = fmaf(ue, Um, cm) - sm, eo = fmaf(ue, Un, cnd) - sn;
+                ok = ok && fabsf(em) > 1e-6f;
+                ee[e] = (fmaf(mp, eo * __builtin_amdgcn_rcpf(em), sn) - cn) * idn + 0.5f * nn;
             }
-            const float glo = fmaxf(vlo / a.dv + cv - 0.5f - margin, -1.0f);
-            const float ghi = fminf(vhi / a.dv + cv + 0.5f + margin, (float)a.nv);
+            const float e1 = fminf(ee[0], ee[1]), e2 = fmaxf(ee[0], ee[1]);
+            const float oe = fminf(e2, (float)(j + 1)) - fmaxf(e1, (float)j);
+            const float dpc = fmaf(uc, Um, cm) - sm;     // centre ray's main-axis run to the detector
+            if (!ok || !(fabsf(dpc) > 1e-6f) || !(oe > 0.f) || !(e2 > e1)) continue;
+            const float idpc = __builtin_amdgcn_rcpf(dpc);
+            const float lx = fmaf(uc, U.x, c.x) - s.x, ly = fmaf(uc, U.y, c.y) - s.y;
+            const float lxy2 = fmaf(lx, lx, ly * ly);
+            const float zc0 = fmaf(uc, U.z, c.z) - s.z;
+            // z edges of a row at detector height v: alpha + beta v (index
+            // coordinates); every row's footprint is |beta| dv high
+            const float alpha = (fmaf(mp, zc0 * idpc, s.z) - g.cz) * idz + 0.5f * g.nz;
+            const float beta = mp * V.z * idpc * idz;
+            if (!(fabsf(beta) > 0.f)) continue;
+            const float ib = __builtin_amdgcn_rcpf(beta);
+            const float va = (fz - alpha) * ib, vb = (fz + 1.0f - alpha) * ib;
+            const float glo = fmaxf(fminf(va, vb) * idv + cv - 0.5f - margin, -1.0f);
+            const float ghi = fminf(fmaxf(va, vb) * idv + cv + 0.5f + margin, (float)a.nv);
             if (!(ghi >= glo)) continue;
             const int v0 = max((int)ceilf(glo), 0), v1 = min((int)floorf(ghi), a.nv - 1);
+            // oe / (e2 - e1) / (|beta| dv) * dm / |dpc|: every factor of the weight but oz and the length
+            const float wcol = oe * __builtin_amdgcn_rcpf((e2 - e1) * fabsf(beta) * a.dv) * dm * fabsf(idpc);
+            const float hb = 0.5f * a.dv;
             for (int iv = v0; iv <= v1; ++iv) {
-                const ConeRay r = cone_ray(a, g, c, U, V, s, xv, iu, iv);
-                if (!r.ok) continue;
-                const float fp = (float)p;
-                float e1 = fmaf(r.b[0], fp, r.a[0]), e2 = fmaf(r.b[1], fp, r.a[1]);
-                float z1 = fmaf(r.b[2], fp, r.a[2]), z2 = fmaf(r.b[3], fp, r.a[3]);
-                if (e1 > e2) { const float t = e1; e1 = e2; e2 = t; }
-                if (z1 > z2) { const float t = z1; z1 = z2; z2 = t; }
-                const float area = (e2 - e1) * (z2 - z1);
-                if (!(area > 0.f)) continue;
-                const float oe = fminf(e2, (float)(j + 1)) - fmaxf(e1, (float)j);
-                const float oz = fminf(z2, (float)(iz + 1)) - fmaxf(z1, (float)iz);
-                if (oe <= 0.f || oz <= 0.f) continue;
-                acc = fmaf(q[(((long)iu * a.nview + view) * a.nv + iv) * nC], oe * oz / area * r.L, acc);
+                const float vc = ((float)iv - cv) * a.dv;
+                const float za = fmaf(beta, vc - hb, alpha), zb = fmaf(beta, vc + hb, alpha);
+                const float z1 = fminf(za, zb), z2 = fmaxf(za, zb);
+                const float oz = fminf(z2, fz + 1.0f) - fmaxf(z1, fz);
+                if (!(oz > 0.f)) continue;
+                const float lz = fmaf(vc, V.z, zc0);
+                acc = fmaf(q[(((long)iu * a.nview + view) * a.nv + iv) * nC], wcol * oz * sqrtf(fmaf(lz, lz, lxy2)),
+                           acc);
             }
         }
     }
