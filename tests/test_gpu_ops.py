@@ -85,3 +85,43 @@ def test_cone_ops_opcheck_and_adjoint():
         assert abs(lhs - rhs) / abs(lhs) <= 1e-5
     finally:
         ops.release_plan(pid)
+
+
+@pytest.mark.parametrize("kind", ["dd", "siddon"])
+def test_fan_ops_values_and_gradient_match_oracle(fan, kind):
+    # the TF op contract (zOpsReconNet.cc:270-321, ReconNet.py:773-830): the
+    # forward op's values, and the gradient of 0.5 |A x - y|^2 taken through the
+    # op's registered backward, A^T (A x - y), against the FP64 oracle (for DD
+    # the oracle's A^T is the explicit transpose of its DD FP on this grid)
+    import oracle as orc
+    from reconnet_amd import ops
+    from tests.helpers import rel_l2
+
+    geo, pid = fan
+    deg = geo.angles()
+    g5 = geo.oracle_geo()
+    rng = np.random.default_rng(17)
+    x0 = rng.uniform(0, 1, (2, geo.nx, geo.ny)).astype(np.float32)
+    meas = rng.uniform(0, 20, (2, geo.nu, geo.nview)).astype(np.float32)
+    x = torch.from_numpy(x0).cuda().requires_grad_(True)
+    y = ops.fan_project(x, pid, kind)
+    r = y - torch.from_numpy(meas).cuda()
+    (0.5 * (r * r).sum()).backward()
+    fp = orc.dd_fan_fp if kind == "dd" else orc.siddon_fan_fp
+    ref_y = fp(x0[..., None, None], deg, g5, geo.nu)[..., 0, 0]
+    assert rel_l2(y.detach().cpu().numpy(), ref_y) <= 1e-5
+    res = (ref_y - meas).astype(np.float64)
+    if kind == "siddon":
+        ref_g = orc.siddon_fan_bp(res.astype(np.float32)[..., None, None], deg, g5, (geo.nx, geo.ny, 1))[..., 0, 0]
+        # the residual fed to the oracle is rounded to float32 once
+        tol = 1e-5
+    else:
+        cols = []
+        for k in range(geo.nx * geo.ny):
+            e = np.zeros(geo.nx * geo.ny, np.float32)
+            e[k] = 1
+            cols.append(orc.dd_fan_fp(e.reshape(1, geo.nx, geo.ny, 1, 1), deg, g5, geo.nu).ravel())
+        A = np.stack(cols, 1)
+        ref_g = np.stack([(A.T @ res[b].ravel()).reshape(geo.nx, geo.ny) for b in range(2)])
+        tol = 1e-5
+    assert rel_l2(x.grad.cpu().numpy(), ref_g) <= tol
