@@ -277,6 +277,23 @@ void rn_cone_plan_destroy(void* plan);
 int rn_cone_forward(void* plan, int kind, const float* img, float* prj, int nBatches, int nChannels, void* stream);
 int rn_cone_backward(void* plan, int kind, const float* prj, float* img, int nBatches, int nChannels, void* stream);
 
+/* Helical fan -> parallel-cone rebinning (no reference counterpart: the
+ * reference's helical BP, Recon3D/HelicalBP.cu:10-83, consumes rebinned data).
+ * fan: [B][nu_f][nview_f][nv_f][C], flat-detector views of a helical scan,
+ * source angle beta0 + m dbeta at (dso cos b, dso sin b, z), detector at
+ * distance dsd with columns along (sin b, -cos b, 0) and rows along z, cell
+ * (iu, iv) at ((iu - off_u_f - (nu_f-1)/2) du_f, (iv - off_v_f - (nv_f-1)/2) dv_f)
+ * (cSiddonConeProjectionAbitrary's detector).  par: [B][nu_p][nview_p][nv_p][C],
+ * parallel view theta0 + k dtheta, u = (l - (nu_p-1)/2 - off_u_p) du_p,
+ * v = (iv - (nv_p-1)/2 - off_v_p) dv_p on the cylinder of radius dsd around the
+ * source: the input of cHelicalBPFromParallelConebeamBackprojection.
+ * Trilinear interpolation, zero outside the raw data.  Device pointers. */
+int rn_helical_rebin(const float* fan, float* par, int nBatches, int nChannels,
+        int nu_f, int nview_f, int nv_f, float du_f, float dv_f, float off_u_f, float off_v_f,
+        float beta0, float dbeta,
+        int nu_p, int nview_p, int nv_p, float du_p, float dv_p, float off_u_p, float off_v_p,
+        float theta0, float dtheta, float dsd, float dso, void* stream);
+
 /* Launch-shape autotuner (no reference counterpart).  On the first launch of
  * a shape key the projector launchers time their candidate block / tile shapes
  * on the caller's stream and keep the fastest for the process; the choice

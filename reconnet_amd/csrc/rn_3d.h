@@ -66,4 +66,30 @@ struct HelicalGeo {
 // bpHelicalParallelConebeam (Recon3D/HelicalBP.cu:10-83)
 void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st);
 
+// Helical fan -> parallel-cone rebinning (no reference counterpart: the
+// helical BP consumes rebinned data).  Raw views: source angle
+// b_m = beta0 + m dbeta at (dso cos b, dso sin b, z(b)), a flat detector at
+// distance dsd, column s = (iu - off_u - (nu-1)/2) du along (sin b, -cos b, 0),
+// row w = (iv - off_v - (nv-1)/2) dv along z (the arguments of
+// cSiddonConeProjectionAbitrary).  Parallel views: angle
+// t_k = theta0 + k dtheta, rays of direction -(cos t, sin t) at signed
+// distance u = (l - (nu-1)/2 - off_u) du from the axis along (sin t, -cos t),
+// rows v = (iv - (nv-1)/2 - off_v) dv on the cylinder of radius dsd around
+// the source (the coordinates of bpHelicalParallelConebeam).  The ray (t, u)
+// comes from the source at b = t - asin(u / dso) through the flat-detector
+// point s = dsd tan(asin(u / dso)), w = v / cos(asin(u / dso)).
+struct RebinGeo {
+    int nu_f, nview_f, nv_f;
+    float du_f, dv_f, off_u_f, off_v_f;
+    int nu_p, nview_p, nv_p;
+    float du_p, dv_p, off_u_p, off_v_p;
+    float dso, dsd;
+    // fractional raw view of parallel view k with gamma = 0: k * kscale + moff;
+    // minus gamma * idbeta (kscale = dtheta / dbeta, moff = (theta0 - beta0) / dbeta)
+    double kscale, moff, idbeta;
+};
+// fan [B][nu_f][nview_f][nv_f][C] -> par [B][nu_p][nview_p][nv_p][C] (device):
+// trilinear interpolation in (column, view, row), zero outside the raw data
+void helical_rebin(const RebinGeo& r, int nB, int nC, const float* fan, float* par, hipStream_t st);
+
 } // namespace rn

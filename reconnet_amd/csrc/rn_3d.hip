@@ -723,6 +723,85 @@ void os_sqs_cone(const ArbGeo& g, int nB, int nC, float* x, const float* prj, co
     RN_HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// one lane per (row, channel) of a parallel ray: the rows of one (u, view)
+// are contiguous in both layouts
+// views per thread: the column's geometry (FP64, so that the interpolation
+// weights carry no index rounding of the 2^12-view magnitudes) is set up once
+constexpr int kRebinViews = 8;
+
+__global__ __launch_bounds__(256) void k_helical_rebin(RebinGeo r, const float* __restrict__ fan,
+                                                       float* __restrict__ par, int nC)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;      // iv * nC + c
+    const int k0 = blockIdx.y * kRebinViews;
+    const int lb = blockIdx.z;                                  // b * nu_p + l
+    const int nq = r.nv_p * nC;
+    if (q >= nq) return;
+    const int l = lb % r.nu_p, b = lb / r.nu_p;
+    const int iv = q / nC, c = q - iv * nC;
+    const int kend = min(k0 + kRebinViews, r.nview_p);
+    float* out = par + ((long)lb * r.nview_p) * nq + q;
+    const double u = ((double)l - 0.5 * (r.nu_p - 1) - (double)r.off_u_p) * (double)r.du_p;
+    const double sg = u / (double)r.dso;
+    if (!(fabs(sg) < 1.0)) {
+        for (int k = k0; k < kend; ++k) out[(long)k * nq] = 0.0f;
+        return;
+    }
+    const double cg = sqrt(fma(-sg, sg, 1.0));
+    // column s = dsd tan(g) and row height w = v / cos(g) on the flat detector
+    const double fc = (double)r.dsd * sg / (cg * (double)r.du_f) + (double)r.off_u_f + 0.5 * (r.nu_f - 1);
+    const double v = ((double)iv - 0.5 * (r.nv_p - 1) - (double)r.off_v_p) * (double)r.dv_p;
+    const double fr = v / (cg * (double)r.dv_f) + (double)r.off_v_f + 0.5 * (r.nv_f - 1);
+    const double gm = asin(sg) * r.idbeta;
+    const double c0 = floor(fc), r0 = floor(fr);
+    const float wc = (float)(fc - c0), wr = (float)(fr - r0);
+    const bool in_cr = c0 > -2.0 && c0 < (double)r.nu_f && r0 > -2.0 && r0 < (double)r.nv_f;
+    const int ic = in_cr ? (int)c0 : 0, ir = in_cr ? (int)r0 : 0;
+    const float* base = fan + (long)b * r.nu_f * r.nview_f * r.nv_f * nC + c;
+    for (int k = k0; k < kend; ++k) {
+        const double fm = fma((double)k, r.kscale, r.moff) - gm;
+        const double m0 = floor(fm);
+        const float wm = (float)(fm - m0);
+        float val = 0.0f;
+        // the 8 corner samples; beyond the raw data's index range a sample is 0
+        if (in_cr && m0 > -2.0 && m0 < (double)r.nview_f) {
+            const int im = (int)m0;
+            #pragma unroll
+            for (int dc = 0; dc < 2; ++dc) {
+                const int cc = ic + dc;
+                if (cc < 0 || cc >= r.nu_f) continue;
+                #pragma unroll
+                for (int dm = 0; dm < 2; ++dm) {
+                    const int mm = im + dm;
+                    if (mm < 0 || mm >= r.nview_f) continue;
+                    const float* row = base + ((long)cc * r.nview_f + mm) * r.nv_f * nC;
+                    const float w2 = (dc ? wc : 1.0f - wc) * (dm ? wm : 1.0f - wm);
+                    #pragma unroll
+                    for (int dr = 0; dr < 2; ++dr) {
+                        const int rr = ir + dr;
+                        if (rr < 0 || rr >= r.nv_f) continue;
+                        val = fmaf(row[(long)rr * nC], w2 * (dr ? wr : 1.0f - wr), val);
+                    }
+                }
+            }
+        }
+        out[(long)k * nq] = val;
+    }
+}
+
+void helical_rebin(const RebinGeo& r, int nB, int nC, const float* fan, float* par, hipStream_t st)
+{
+    RN_REQUIRE(nB > 0 && nC > 0, "nBatches and nChannels must be positive");
+    RN_REQUIRE(r.nu_f > 0 && r.nview_f > 0 && r.nv_f > 0 && r.nu_p > 0 && r.nview_p > 0 && r.nv_p > 0,
+               "rebinning dimensions must be positive");
+    RN_REQUIRE(r.dso > 0 && r.dsd > 0 && r.du_f != 0 && r.dv_f != 0 && r.idbeta != 0 && std::isfinite(r.idbeta),
+               "rebinning distances and spacings must be non-zero");
+    RN_REQUIRE((long)nB * r.nu_p < 0x7fffffffL && r.nview_p < 65536, "rebinning grid too large");
+    const dim3 grd(cdiv((long)r.nv_p * nC, 256), cdiv(r.nview_p, kRebinViews), nB * r.nu_p);
+    hipLaunchKernelGGL(k_helical_rebin, grd, dim3(256), 0, st, r, fan, par, nC);
+    check_launch("helical_rebin");
+}
+
 void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st)
 {
     const size_t per = (size_t)h.nu * h.nview * h.nv;

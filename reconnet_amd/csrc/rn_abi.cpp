@@ -495,6 +495,21 @@ extern "C" void cHelicalBPFromParallelConebeamBackprojection(float* pImg, const 
 // Additive API
 // =====================================================================
 
+extern "C" int rn_helical_rebin(const float* fan, float* par, int nBatches, int nChannels,
+        int nu_f, int nview_f, int nv_f, float du_f, float dv_f, float off_u_f, float off_v_f,
+        float beta0, float dbeta,
+        int nu_p, int nview_p, int nv_p, float du_p, float dv_p, float off_u_p, float off_v_p,
+        float theta0, float dtheta, float dsd, float dso, void* stream)
+{
+    return guarded_status("rn_helical_rebin", [&] {
+        RN_REQUIRE(fan && par, "null pointer");
+        RN_REQUIRE(dbeta != 0.0f && dtheta != 0.0f, "angular steps must be non-zero");
+        RebinGeo r{nu_f, nview_f, nv_f, du_f, dv_f, off_u_f, off_v_f, nu_p, nview_p, nv_p, du_p, dv_p, off_u_p,
+                   off_v_p, dso, dsd, (double)dtheta / dbeta, ((double)theta0 - beta0) / dbeta, 1.0 / dbeta};
+        helical_rebin(r, nBatches, nChannels, fan, par, static_cast<hipStream_t>(stream));
+    });
+}
+
 extern "C" const char* rn_last_error(void) { return last_error(); }
 
 extern "C" int rn_autotune_enable(int on)
