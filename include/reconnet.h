@@ -226,6 +226,12 @@ int rn_cnn_conv_mid(const unsigned short* in, const unsigned short* wt, const fl
 /* y = xres + b5 + conv(a4; w5) (xres may be NULL) */
 int rn_cnn_conv_out(const unsigned short* a4, const float* w5, float b5, const float* xres, float* y,
         int B, int H, int W, void* stream);
+/* The whole forward network in one launch, no activations kept: the same
+ * values as rn_cnn_conv_in, three rn_cnn_conv_mid (relu) and rn_cnn_conv_out
+ * with xres = x, bit for bit.  w2..w4 as wt of rn_cnn_conv_mid. */
+int rn_cnn_forward(const float* x, const float* w1, const float* b1, const unsigned short* w2, const float* b2,
+        const unsigned short* w3, const float* b3, const unsigned short* w4, const float* b4,
+        const float* w5, float b5, float* y, int B, int H, int W, void* stream);
 /* dz4 = (a4 > 0) * conv^T(dy; w5) */
 int rn_cnn_out_dgrad(const float* dy, const float* w5, const unsigned short* a4, unsigned short* dz4,
         int B, int H, int W, void* stream);
@@ -307,7 +313,8 @@ int rn_autotune_reset(void);
 /* Force candidate `candidate` of kernel `kernel` (1 Siddon FP, 2 Siddon BP,
  * 3 DD FP, 4 DD adjoint, 5 DD BP, 7 DD FBP BP; 8 / 9 Siddon FP / BP,
  * 10 / 11 DD BP / FBP BP and 12 DD FP on the interleaved 4-slice layout used
- * for 4 or more slices) for every shape, bypassing the
+ * for 4 or more slices; 13 the CNN's 32 -> 32 convolution, 14 the one-launch CNN forward) for every
+ * shape, bypassing the
  * timing; candidate < 0 removes the pin.  A launch of a kernel with fewer
  * candidates fails with an error.  Lets tests run every launch shape. */
 int rn_autotune_pin(int kernel, int candidate);

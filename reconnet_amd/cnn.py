@@ -29,6 +29,7 @@ def _sigs():
         ("rn_cnn_conv_in", [_P, _P, _P, _P, ci, ci, ci, _P]),
         ("rn_cnn_conv_mid", [_P, _P, _P, _P, _P, ci, ci, ci, ci, _P]),
         ("rn_cnn_conv_out", [_P, _P, cf, _P, _P, ci, ci, ci, _P]),
+        ("rn_cnn_forward", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, cf, _P, ci, ci, ci, _P]),
         ("rn_cnn_out_dgrad", [_P, _P, _P, _P, ci, ci, ci, _P]),
         ("rn_cnn_wgrad_narrow", [_P, _P, _P, ci, ci, ci, ci, _P]),
         ("rn_cnn_bias_grad", [_P, _P, cl, _P]),
@@ -107,10 +108,21 @@ class PriorCNN:
     def refresh(self):
         self._prep()
 
-    def forward(self, x, keep=False):
+    def forward(self, x, keep=False, fused=True):
+        """y = x + prior(x) for x [B, H, W].  Without `keep` the whole network
+        runs as one launch (rn_cnn_forward, no activations in memory); with
+        `keep` (or fused=False) layer by layer, keeping the activations for
+        backward.  Both give the same bits."""
         h = lib()
         x = x.contiguous()
         B, H, W = x.shape
+        if fused and not keep:
+            y = torch.empty_like(x)
+            check(h.rn_cnn_forward(_p(x), _p(self.w1e), _p(self.p["b1"]), _p(self.wt[2]), _p(self.p["b2"]),
+                                   _p(self.wt[3]), _p(self.p["b3"]), _p(self.wt[4]), _p(self.p["b4"]),
+                                   _p(self.w5e), self.b5, _p(y), B, H, W, _st()), "rn_cnn_forward")
+            self._cache = None
+            return y
         acts = []
         a = torch.empty((B, H, W, 32), dtype=torch.bfloat16, device=x.device)
         check(h.rn_cnn_conv_in(_p(x), _p(self.w1e), _p(self.p["b1"]), _p(a), B, H, W, _st()), "rn_cnn_conv_in")

@@ -801,6 +801,17 @@ extern "C" int rn_cnn_conv_out(const uint16_t* a4, const float* w5, float b5, co
     });
 }
 
+extern "C" int rn_cnn_forward(const float* x, const float* w1, const float* b1, const uint16_t* w2, const float* b2,
+                              const uint16_t* w3, const float* b3, const uint16_t* w4, const float* b4,
+                              const float* w5, float b5, float* y, int B, int H, int W, void* stream)
+{
+    return guarded_status("rn_cnn_forward", [&] {
+        RN_REQUIRE(x && w1 && b1 && w2 && b2 && w3 && b3 && w4 && b4 && w5 && y && B > 0 && H > 0 && W > 0,
+                   "invalid argument");
+        cnn_forward(x, w1, b1, w2, b2, w3, b3, w4, b4, w5, b5, y, B, H, W, static_cast<hipStream_t>(stream));
+    });
+}
+
 extern "C" int rn_cnn_out_dgrad(const float* dy, const float* w5, const uint16_t* a4, uint16_t* dz4, int B, int H,
                                 int W, void* stream)
 {

@@ -20,6 +20,11 @@ void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const u
 // y = xres + b5 + conv(a4; w5)   (xres may be null)
 void conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, float* y, int B, int H, int W,
               hipStream_t st);
+// The whole network in one launch: y = x + b5 + conv(a4; w5), a4 = relu(conv(a3; w4) + b4), ...,
+// a1 = relu(conv(x; w1) + b1) -- the same bits as conv_in, 3 x conv_mid (relu) and conv_out
+void cnn_forward(const float* x, const float* w1, const float* b1, const uint16_t* w2, const float* b2,
+                 const uint16_t* w3, const float* b3, const uint16_t* w4, const float* b4, const float* w5, float b5,
+                 float* y, int B, int H, int W, hipStream_t st);
 // dz4 = (a4 > 0) * conv^T(dy; w5)
 void conv_out_dgrad(const float* dy, const float* w5, const uint16_t* a4, uint16_t* dz4, int B, int H, int W,
                     hipStream_t st);

@@ -13,6 +13,9 @@
 // transposes.  The 1 -> 32 and 32 -> 1 layers (K = 9, N = 1) are VALU kernels.
 
 #include "rn_conv.h"
+#include "rn_tune.h"
+
+#include <type_traits>
 
 namespace rn {
 
@@ -269,6 +272,124 @@ __global__ __launch_bounds__(256, 3) void k_conv_mid(const uint16_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// 32 -> 32 layer with the input rows staged in LDS.  Same walk, fragments and
+// MFMA sequence as k_conv_mid (so the same bits), but the NW waves of a
+// workgroup (16 NW adjacent pixel columns) share each input row: the row's
+// 16 NW + 2 pixels (the columns and a 1-pixel halo, 64 bytes each) are loaded
+// once per workgroup with coalesced 16-byte loads, two rows ahead of their
+// use, into one of two LDS row buffers, and every wave reads its three
+// column-shifted fragments from there (one ds_read_b128 each).  The vector
+// memory path then moves each input byte about once per workgroup row instead
+// of once per wave plus two edge loads.
+//
+// LDS row layout: [channel group g][pixel], 16 bytes per entry, pitch NPP
+// with NPP = 4 (mod 16) entries: the 16 lanes of a fragment read (same g,
+// consecutive pixels) and of a staging write (4 pixels x 4 groups) hit
+// distinct banks.
+// ---------------------------------------------------------------------------
+template <int NW>
+struct ConvRowStage {
+    static constexpr int NP = 16 * NW + 2;                        // pixels of a staged row
+    static constexpr int NPP = NP + (((4 - NP) % 16) + 16) % 16;  // pitch = 4 (mod 16)
+    static constexpr int NCH = NP * 4;                            // 16-byte chunks of a row
+    static constexpr int NT = 64 * NW;                            // threads
+    static constexpr int K = (NCH + NT - 1) / NT;                 // chunks per thread
+};
+
+template <int NW>
+__device__ __forceinline__ void stage_load(const uint16_t* __restrict__ in, long img, int yy, int H, int W, int x0,
+                                           int t, uint4 (&v)[ConvRowStage<NW>::K])
+{
+    using S = ConvRowStage<NW>;
+    const bool rowok = yy >= 0 && yy < H;
+    #pragma unroll
+    for (int k = 0; k < S::K; ++k) {
+        const int e = t + k * S::NT;
+        const int x = x0 - 1 + (e >> 2);
+        v[k] = make_uint4(0, 0, 0, 0);
+        if (rowok && e < S::NCH && x >= 0 && x < W)
+            v[k] = *reinterpret_cast<const uint4*>(in + (img + (long)yy * W + x) * 32 + 8 * (e & 3));
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ void stage_store(uint4* __restrict__ buf, int t, const uint4 (&v)[ConvRowStage<NW>::K])
+{
+    using S = ConvRowStage<NW>;
+    #pragma unroll
+    for (int k = 0; k < S::K; ++k) {
+        const int e = t + k * S::NT;
+        if (e < S::NCH) buf[(e & 3) * S::NPP + (e >> 2)] = v[k];
+    }
+}
+
+template <int L, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void k_conv_mid_s(const uint16_t* __restrict__ in,
+                                                      const uint16_t* __restrict__ wt,
+                                                      const float* __restrict__ bias,
+                                                      const uint16_t* __restrict__ mask,
+                                                      uint16_t* __restrict__ out, int H, int W, int relu)
+{
+    using S = ConvRowStage<NW>;
+    __shared__ uint4 rows[2][4 * S::NPP];
+    __shared__ uint16_t sbuf[NW][16 * 32];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const int x0 = blockIdx.x * 16 * NW;                 // first column of the workgroup
+    const int xpix = x0 + wv * 16 + r16;                 // this lane's B column (pixel)
+    const int y0 = blockIdx.y * L;
+    const long img = (long)blockIdx.z * H * W;
+    bf16x8 wa[9][2];
+    #pragma unroll
+    for (int tp = 0; tp < 9; ++tp)
+        #pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+            wa[tp][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((tp * 32 + 16 * mt + r16) * 32 + 8 * g4)));
+    float bv[2][4];
+    #pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) bv[mt][i] = (MODE == 0) ? bias[16 * mt + 4 * g4 + i] : 0.0f;
+    f32x4 a0[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, a1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    f32x4 a2[2];
+    // step ir uses input row y0 - 1 + ir from buffer ir & 1
+    uint4 nx[S::K], nn[S::K];
+    stage_load<NW>(in, img, y0 - 1, H, W, x0, t, nx);
+    stage_store<NW>(rows[0], t, nx);
+    stage_load<NW>(in, img, y0, H, W, x0, t, nx);
+    const int rd = (wv * 16 + r16) + g4 * S::NPP;        // fragment dx = 0 of this lane
+    for (int ir = 0; ir < L + 2; ++ir) {
+        __syncthreads();                                 // buffer ir & 1 complete, buffer (ir + 1) & 1 free
+        const uint4* rb = rows[ir & 1];
+        bf16x8 f[3];
+        #pragma unroll
+        for (int dx = 0; dx < 3; ++dx) f[dx] = as_bf16x8(rb[rd + dx]);
+        if (ir + 2 < L + 2) stage_load<NW>(in, img, y0 + ir + 1, H, W, x0, t, nn);
+        a2[0] = a2[1] = f32x4{0, 0, 0, 0};
+        #pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            #pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                a0[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx][mt], f[dx], a0[mt], 0, 0, 0);
+                a1[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx][mt], f[dx], a1[mt], 0, 0, 0);
+                a2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx][mt], f[dx], a2[mt], 0, 0, 0);
+            }
+        }
+        const int yo = y0 + ir - 2;                      // the row a0 completes
+        if (ir >= 2 && yo < H)
+            conv_mid_store_lds<MODE>(a0, bv, mask, out, img + (long)yo * W, xpix - r16, W, r16, g4, lane, relu,
+                                     sbuf[wv]);
+        if (ir + 1 < L + 2) stage_store<NW>(rows[(ir + 1) & 1], t, nx);
+        #pragma unroll
+        for (int k = 0; k < S::K; ++k) nx[k] = nn[k];
+        a0[0] = a1[0]; a0[1] = a1[1];
+        a1[0] = a2[0]; a1[1] = a2[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // layer 5: y = xres + b + sum_t sum_ci a4[p + t][ci] w5[t][ci]   (fp32 out)
 //
 // On MFMA with the layout and rolling row walk of k_conv_mid, one 16-channel
@@ -328,6 +449,246 @@ __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a
         }
         a0 = a1;
         a1 = a2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The whole forward network in one launch (inference; no activation is kept).
+//
+// A workgroup computes a strip of 16 T computed columns (T tiles of 16
+// pixels) over L output rows of one image, walking down the rows.  Each of
+// its 4 waves owns layers: wave 0 the 1 -> 32 layer (VALU, 8 channels of a
+// pixel per lane) and the 32 -> 1 output layer (MFMA), wave k = 1, 2, 3 the 32 -> 32
+// layer k + 1 (MFMA, its 72 VGPRs of weight fragments resident).  They run
+// as a row pipeline: at step s wave 0 writes a1 row y0 - 4 + s and wave k
+// writes row y0 - 4 - 2k + s of its layer, each consuming the row its
+// producer wrote at step s - 1 from a double-buffered LDS row (bf16,
+// [channel group][pixel], 16 bytes per entry; pitch = 4 (mod 16) entries
+// keeps the fragment reads and the row writes free of bank conflicts); one
+// barrier per step; the output layer completes y row y0 - 12 + s.  Every
+// stored value comes from the same operations in the same order as the
+// layer-by-layer kernels (k_conv_in, k_conv_mid, k_conv_out: rolling
+// accumulators over the three input rows, zero rows and columns outside the
+// image), so the result is the same bits.  The strip's two edge columns see
+// zero neighbours; that error moves one column inwards per 3x3 layer, so of
+// the 16 T computed columns the middle 16 T - 8 are output (strips overlap by
+// 8 columns), and the first 12 steps fill the pipeline.
+// ---------------------------------------------------------------------------
+template <int T>
+struct FusedCnn {
+    static constexpr int NP = 16 * T + 2;                        // pixels of a row incl. the zero halo
+    static constexpr int NPP = NP + (((4 - NP) % 16) + 16) % 16;  // entry pitch of a channel group
+    static constexpr int WO = 16 * T - 8;                         // output columns of a strip
+};
+
+template <int T, int L>
+__global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x, const float* __restrict__ w1,
+                                                   const float* __restrict__ b1, const uint16_t* __restrict__ w2,
+                                                   const float* __restrict__ b2, const uint16_t* __restrict__ w3,
+                                                   const float* __restrict__ b3, const uint16_t* __restrict__ w4,
+                                                   const float* __restrict__ b4, const float* __restrict__ w5, float b5,
+                                                   float* __restrict__ y, int H, int W)
+{
+    using F = FusedCnn<T>;
+    constexpr int NPP = F::NPP, NP = F::NP;
+    __shared__ uint4 act[4][2][4 * NPP];     // rows of a1 .. a4
+    __shared__ float xr[3][NP];              // x rows y - 1, y, y + 1 of the a1 row (wave 0)
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = t >> 6;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const int c0 = blockIdx.x * F::WO - 4;   // first computed column
+    const int y0 = blockIdx.y * L;
+    const long img = (long)blockIdx.z * H * W;
+    for (int i = t; i < 4 * 2 * 4 * NPP; i += 256) (&act[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+
+    if (wv == 0) {
+        // ---- 1 -> 32 (pixel per lane) and 32 -> 1 (MFMA, one 16-channel tile: hi / lo weights) ----
+        bf16x8 wa5[9];
+        #pragma unroll
+        for (int tp = 0; tp < 9; ++tp) {
+            uint32_t u[4];
+            #pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v[2];
+                #pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const float wf = w5[tp * 32 + 8 * g4 + 2 * e + h];
+                    const float hi = (float)(__bf16)wf;
+                    v[h] = r16 == 0 ? wf : (r16 == 1 ? wf - hi : 0.0f);
+                }
+                u[e] = pack_bf16(v[0], v[1]);
+            }
+            wa5[tp] = as_bf16x8(make_uint4(u[0], u[1], u[2], u[3]));
+        }
+        // layer-1 weights and biases of this lane's 8 channels
+        float ww1[9][8], bb1[8];
+        #pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            bb1[e] = b1[8 * g4 + e];
+            #pragma unroll
+            for (int tp = 0; tp < 9; ++tp) ww1[tp][e] = w1[tp * 32 + 8 * g4 + e];
+        }
+        f32x4 o0[T], o1[T], o2[T];
+        #pragma unroll
+        for (int i = 0; i < T; ++i) o0[i] = o1[i] = f32x4{0, 0, 0, 0};
+        // x rows: lane l holds columns c0 - 1 + l + 64 k of a row (zero outside the
+        // image); each row is loaded one step before it enters the LDS ring, and
+        // the residual values of the y row one step before they are added
+        constexpr int KX = (NP + 63) / 64;
+        auto fetch_x = [&](int row, float (&v)[KX]) {
+            const bool rok = row >= 0 && row < H;
+            #pragma unroll
+            for (int k = 0; k < KX; ++k) {
+                const int i = lane + 64 * k;
+                const int c = c0 - 1 + i;
+                v[k] = (rok && i < NP && c >= 0 && c < W) ? x[img + (long)row * W + c] : 0.0f;
+            }
+        };
+        auto put_x = [&](int slot, const float (&v)[KX]) {
+            #pragma unroll
+            for (int k = 0; k < KX; ++k)
+                if (lane + 64 * k < NP) xr[slot][lane + 64 * k] = v[k];
+        };
+        auto fetch_res = [&](int row, float (&v)[T]) {
+            #pragma unroll
+            for (int i = 0; i < T; ++i) {
+                const int c = c0 + 16 * i + r16;
+                v[i] = (row >= 0 && row < H && c >= 0 && c < W) ? x[img + (long)row * W + c] : 0.0f;
+            }
+        };
+        float xn[KX], xres[T];
+        fetch_x(y0 - 5, xn);
+        put_x(0, xn);
+        fetch_x(y0 - 4, xn);
+        put_x(1, xn);
+        fetch_x(y0 - 3, xn);
+        fetch_res(y0 - 12, xres);
+        __syncthreads();
+        // one pipeline step; A completes (taps of dy = +1), Bm takes dy = 0, Cm starts
+        // with dy = -1; the three rotate over consecutive steps, so no copies (x row
+        // y0 - 5 + s + dy is in ring slot (s + dy) % 3, with PH = s % 3)
+        auto step = [&](auto ph, int s, f32x4 (&A)[T], f32x4 (&Bm)[T], f32x4 (&Cm)[T]) {
+            constexpr int PH = decltype(ph)::value;
+            put_x((PH + 2) % 3, xn);                    // x row y0 - 3 + s
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            fetch_x(y0 - 2 + s, xn);                    // for the next step
+            // a1 row r1 (k_conv_in's arithmetic: z = b; z = fma(w[t], v[t], z) for t = 0..8);
+            // lane (r16, g4): pixel 16 i + r16 of tile i, channels 8 g4 .. 8 g4 + 7
+            const int r1 = y0 - 4 + s;
+            uint4* a1w = act[0][s & 1];
+            #pragma unroll 2
+            for (int i = 0; i < T; ++i) {
+                const int pp = 16 * i + r16;
+                const int c = c0 + pp;
+                float v[9];
+                #pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+                    #pragma unroll
+                    for (int dx = 0; dx < 3; ++dx) v[3 * dy + dx] = xr[(PH + dy) % 3][pp + dx];
+                const bool ok = r1 >= 0 && r1 < H && c >= 0 && c < W;
+                uint32_t pk[4];
+                #pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float z0 = bb1[2 * e], z1 = bb1[2 * e + 1];
+                    #pragma unroll
+                    for (int tp = 0; tp < 9; ++tp) {
+                        z0 = fmaf(ww1[tp][2 * e], v[tp], z0);
+                        z1 = fmaf(ww1[tp][2 * e + 1], v[tp], z1);
+                    }
+                    pk[e] = ok ? pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f)) : 0u;
+                }
+                a1w[g4 * NPP + pp + 1] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+            }
+            // output layer on the a4 row written at step s - 1; completes y row y0 - 12 + s
+            const uint4* a4r = act[3][(s + 1) & 1];
+            const int ry = y0 - 12 + s;
+            #pragma unroll
+            for (int i = 0; i < T; ++i) {
+                bf16x8 f[3];
+                #pragma unroll
+                for (int dx = 0; dx < 3; ++dx) f[dx] = as_bf16x8(a4r[g4 * NPP + 16 * i + r16 + dx]);
+                Cm[i] = f32x4{0, 0, 0, 0};
+                #pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    A[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa5[6 + dx], f[dx], A[i], 0, 0, 0);
+                    Bm[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa5[3 + dx], f[dx], Bm[i], 0, 0, 0);
+                    Cm[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa5[dx], f[dx], Cm[i], 0, 0, 0);
+                }
+                const int c = c0 + 16 * i + r16;
+                if (s >= 12 && ry < H && g4 == 0 && 16 * i + r16 >= 4 && 16 * i + r16 < 4 + F::WO && c < W)
+                    y[img + (long)ry * W + c] = xres[i] + (b5 + (A[i][0] + A[i][1]));
+            }
+            fetch_res(ry + 1, xres);                    // for the next step
+            __syncthreads();
+        };
+        for (int s = 0; s < L + 12; s += 3) {
+            step(std::integral_constant<int, 0>{}, s, o0, o1, o2);
+            if (s + 1 < L + 12) step(std::integral_constant<int, 1>{}, s + 1, o1, o2, o0);
+            if (s + 2 < L + 12) step(std::integral_constant<int, 2>{}, s + 2, o2, o0, o1);
+        }
+    } else {
+        // ---- 32 -> 32 layer wv + 1: relu(conv(a_wv) + b) ----
+        const uint16_t* wt = wv == 1 ? w2 : (wv == 2 ? w3 : w4);
+        const float* bias = wv == 1 ? b2 : (wv == 2 ? b3 : b4);
+        bf16x8 wa[9][2];
+        #pragma unroll
+        for (int tp = 0; tp < 9; ++tp)
+            #pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+                wa[tp][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((tp * 32 + 16 * mt + r16) * 32 + 8 * g4)));
+        float bv[2][4];
+        #pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) bv[mt][i] = bias[16 * mt + 4 * g4 + i];
+        f32x4 m0[T][2], m1[T][2], m2[T][2];
+        #pragma unroll
+        for (int i = 0; i < T; ++i)
+            #pragma unroll
+            for (int mt = 0; mt < 2; ++mt) m0[i][mt] = m1[i][mt] = f32x4{0, 0, 0, 0};
+        __syncthreads();
+        // one pipeline step (accumulator roles rotate as in wave 0's steps)
+        auto step = [&](int s, f32x4 (&A)[T][2], f32x4 (&Bm)[T][2], f32x4 (&Cm)[T][2]) {
+            const uint4* in = act[wv - 1][(s + 1) & 1];   // written at step s - 1
+            uint2* ow = reinterpret_cast<uint2*>(act[wv][s & 1]);
+            const int ro = y0 - 4 - 2 * wv + s;            // the row this step completes
+            const bool rok = ro >= 0 && ro < H;
+            #pragma unroll
+            for (int i = 0; i < T; ++i) {
+                bf16x8 f[3];
+                #pragma unroll
+                for (int dx = 0; dx < 3; ++dx) f[dx] = as_bf16x8(in[g4 * NPP + 16 * i + r16 + dx]);
+                Cm[i][0] = Cm[i][1] = f32x4{0, 0, 0, 0};
+                #pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    #pragma unroll
+                    for (int mt = 0; mt < 2; ++mt) {
+                        A[i][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx][mt], f[dx], A[i][mt], 0, 0, 0);
+                        Bm[i][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx][mt], f[dx], Bm[i][mt], 0, 0, 0);
+                        Cm[i][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx][mt], f[dx], Cm[i][mt], 0, 0, 0);
+                    }
+                }
+                // D[m = co][n = pixel]: this lane holds channels 16 mt + 4 g4 .. + 3 of pixel r16
+                const int c = c0 + 16 * i + r16;
+                const bool ok = rok && c >= 0 && c < W;
+                #pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    float z[4];
+                    #pragma unroll
+                    for (int e = 0; e < 4; ++e) z[e] = fmaxf(A[i][mt][e] + bv[mt][e], 0.0f);
+                    uint2 ov = make_uint2(0, 0);
+                    if (ok) { ov.x = pack_bf16(z[0], z[1]); ov.y = pack_bf16(z[2], z[3]); }
+                    ow[((2 * mt + (g4 >> 1)) * NPP + 16 * i + r16 + 1) * 2 + (g4 & 1)] = ov;
+                }
+            }
+            __syncthreads();
+        };
+        for (int s = 0; s < L + 12; s += 3) {
+            step(s, m0, m1, m2);
+            if (s + 1 < L + 12) step(s + 1, m1, m2, m0);
+            if (s + 2 < L + 12) step(s + 2, m2, m0, m1);
+        }
     }
 }
 
@@ -548,6 +909,8 @@ __global__ void k_reduce_rows(const float* __restrict__ part, float* __restrict_
     out[i] = accumulate ? out[i] + s : s;
 }
 
+constexpr int kNConvMid = 5;
+
 int grid_for(long work, int per_block, int cap)
 {
     long g = (work + per_block - 1) / per_block;
@@ -574,13 +937,22 @@ void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const u
 {
     RN_REQUIRE(mask != nullptr || bias != nullptr, "conv_mid needs a bias (forward) or a mask (data gradient)");
     RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "conv_mid: bad batch or image size");
-    constexpr int kRows = 16;
-    const dim3 grd((unsigned)cdiv(W, 64), (unsigned)cdiv(H, kRows), (unsigned)B);
-    if (mask)
-        hipLaunchKernelGGL((k_conv_mid<kRows, 1>), grd, dim3(256), 0, st, in, wt, bias, mask, out, H, W, relu ? 1 : 0);
-    else
-        hipLaunchKernelGGL((k_conv_mid<kRows, 0>), grd, dim3(256), 0, st, in, wt, bias, mask, out, H, W, relu ? 1 : 0);
-    check_launch("conv_mid");
+    const int rl = relu ? 1 : 0;
+    // candidates: LDS-staged rows (4 or 8 waves, 16 or 32 rows per workgroup), then the per-wave DPP walk
+    auto launch = [&](int c) {
+        auto staged = [&](auto kern, int nw, int rows) {
+            const dim3 grd((unsigned)cdiv(W, 16 * nw), (unsigned)cdiv(H, rows), (unsigned)B);
+            hipLaunchKernelGGL(kern, grd, dim3(64 * nw), 0, st, in, wt, bias, mask, out, H, W, rl);
+        };
+        if (c == 0) mask ? staged(k_conv_mid_s<16, 1, 4>, 4, 16) : staged(k_conv_mid_s<16, 0, 4>, 4, 16);
+        else if (c == 1) mask ? staged(k_conv_mid_s<16, 1, 8>, 8, 16) : staged(k_conv_mid_s<16, 0, 8>, 8, 16);
+        else if (c == 2) mask ? staged(k_conv_mid_s<32, 1, 4>, 4, 32) : staged(k_conv_mid_s<32, 0, 4>, 4, 32);
+        else if (c == 3) mask ? staged(k_conv_mid_s<32, 1, 8>, 8, 32) : staged(k_conv_mid_s<32, 0, 8>, 8, 32);
+        else mask ? staged(k_conv_mid<16, 1>, 4, 16) : staged(k_conv_mid<16, 0>, 4, 16);
+        check_launch("conv_mid");
+    };
+    const int c = tune_choose({TK_CONV_MID, B, H, W, mask ? 1 : 0, rl}, kNConvMid, launch, st, true);
+    launch(c);
 }
 
 void conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, float* y, int B, int H, int W,
@@ -591,6 +963,27 @@ void conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, 
     const dim3 grd((unsigned)cdiv(W, 64), (unsigned)cdiv(H, kRows), (unsigned)B);
     hipLaunchKernelGGL(k_conv_out<kRows>, grd, dim3(256), 0, st, a4, w5, b5, xres, y, H, W);
     check_launch("conv_out");
+}
+
+void cnn_forward(const float* x, const float* w1, const float* b1, const uint16_t* w2, const float* b2,
+                 const uint16_t* w3, const float* b3, const uint16_t* w4, const float* b4, const float* w5, float b5,
+                 float* y, int B, int H, int W, hipStream_t st)
+{
+    RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "cnn_forward: bad batch or image size");
+    // candidates: strips of T = 6, 5, 4 tiles of 16 columns, 64 or 128 rows per workgroup
+    auto launch = [&](int c) {
+        auto go = [&](auto kern, int wo, int rows) {
+            const dim3 grd((unsigned)cdiv(W, wo), (unsigned)cdiv(H, rows), (unsigned)B);
+            hipLaunchKernelGGL(kern, grd, dim3(256), 0, st, x, w1, b1, w2, b2, w3, b3, w4, b4, w5, b5, y, H, W);
+        };
+        if (c == 0) go(k_cnn_fwd<6, 64>, FusedCnn<6>::WO, 64);
+        else if (c == 1) go(k_cnn_fwd<5, 64>, FusedCnn<5>::WO, 64);
+        else if (c == 2) go(k_cnn_fwd<4, 64>, FusedCnn<4>::WO, 64);
+        else go(k_cnn_fwd<6, 128>, FusedCnn<6>::WO, 128);
+        check_launch("cnn_forward");
+    };
+    const int c = tune_choose({TK_CNN_FWD, B, H, W, 0, 0}, 4, launch, st, true);
+    launch(c);
 }
 
 void conv_out_dgrad(const float* dy, const float* w5, const uint16_t* a4, uint16_t* dz4, int B, int H, int W,

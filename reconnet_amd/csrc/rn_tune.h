@@ -41,6 +41,8 @@ enum TuneKernel {
     TK_DD_BP_Q = 10,
     TK_DD_FBP_Q = 11,
     TK_DD_FP_Q = 12,
+    TK_CONV_MID = 13,
+    TK_CNN_FWD = 14,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

@@ -66,22 +66,55 @@ def test_conv_in_exact(shape):
     assert np.array_equal(out.float().cpu().numpy(), ref.astype(np.float32))
 
 
+N_CONV_MID = 5      # launch candidates of the 32 -> 32 layer (LDS-staged rows x 4, per-wave walk)
+
+
+def _conv_mid(a, w, b, mask, mode, shape, cand=None):
+    from reconnet_amd._lib import autotune_pin
+
+    B, H, W = shape
+    out = torch.empty(B, H, W, 32, dtype=torch.bfloat16, device="cuda")
+    tb, tm = _f(b), _bf(mask)
+    if cand is not None:
+        autotune_pin("conv_mid", cand)
+    try:
+        st = _h().rn_cnn_conv_mid(_p(_bf(a)), _p(_bf(w)), _p(tb) if mode else None, None if mode else _p(tm),
+                                  _p(out), B, H, W, mode, _st())
+    finally:
+        if cand is not None:
+            autotune_pin("conv_mid", None)
+    assert st == 0
+    return out.float().cpu().numpy()
+
+
+@pytest.mark.parametrize("cand", [None] + list(range(N_CONV_MID)))
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("mode", [1, 0])
-def test_conv_mid_exact(shape, mode):
+def test_conv_mid_exact(shape, mode, cand):
     B, H, W = shape
     rng = np.random.default_rng(12 + mode)
     a, w = _ints(rng, -3, 3, (B, H, W, 32)), _ints(rng, -2, 2, (9, 32, 32))
     b = _ints(rng, -4, 4, 32)
     mask = _ints(rng, -1, 1, (B, H, W, 32))
-    out = torch.empty(B, H, W, 32, dtype=torch.bfloat16, device="cuda")
-    tb, tm = _f(b), _bf(mask)
-    st = _h().rn_cnn_conv_mid(_p(_bf(a)), _p(_bf(w)), _p(tb) if mode else None, None if mode else _p(tm),
-                              _p(out), B, H, W, mode, _st())
-    assert st == 0
+    out = _conv_mid(a, w, b, mask, mode, shape, cand)
     z = ocnn.conv_taps(a, w)
     ref = np.maximum(z + b, 0) if mode else z * (mask > 0)
-    assert np.array_equal(out.float().cpu().numpy(), ocnn.bf16_round(ref).astype(np.float32))
+    assert np.array_equal(out, ocnn.bf16_round(ref).astype(np.float32))
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_conv_mid_launch_shapes_agree_bitwise(mode):
+    # real-valued inputs: every candidate runs the same MFMA sequence per output
+    shape = (2, 45, 150)
+    rng = np.random.default_rng(40 + mode)
+    a = rng.normal(0, 1, shape + (32,)).astype(np.float32)
+    w = rng.normal(0, 0.1, (9, 32, 32)).astype(np.float32)
+    b = rng.normal(0, 0.1, 32).astype(np.float32)
+    mask = rng.normal(0, 1, shape + (32,)).astype(np.float32)
+    outs = [_conv_mid(a, w, b, mask, mode, shape, c) for c in range(N_CONV_MID)]
+    assert np.abs(outs[0]).max() > 0.1
+    for c in range(1, N_CONV_MID):
+        assert np.array_equal(outs[c], outs[0]), c
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -198,3 +231,31 @@ def test_training_step_reduces_loss():
         opt.step()
         net.refresh()
     assert losses[-1] < 0.5 * losses[0]
+
+
+@pytest.mark.parametrize("shape", [(1, 9, 70), (2, 17, 33), (1, 64, 64), (3, 130, 200), (1, 64, 89), (1, 200, 97)])
+def test_fused_forward_equals_layer_by_layer_bitwise(shape):
+    # one launch of the whole network against conv_in, 3 x conv_mid, conv_out:
+    # strips of 88 output columns overlapping by 8, 64-row segments, borders
+    from reconnet_amd.cnn import PriorCNN, init_params
+
+    B, H, W = shape
+    p = init_params(seed=3)
+    for k in ("b1", "b2", "b3", "b4"):
+        p[k] = torch.randn(p[k].shape, generator=torch.Generator().manual_seed(len(k) + B)).cuda() * 0.1
+    p["b5"] = torch.tensor(0.01).cuda()
+    net = PriorCNN(p, scale=50.0)
+    g = torch.Generator(device="cuda").manual_seed(H * W)
+    x = torch.rand((B, H, W), generator=g, device="cuda") * 0.04
+    ref = net.forward(x, fused=False)
+    assert (ref - x).abs().max() > 1e-4
+    from reconnet_amd._lib import autotune_pin
+
+    for cand in [None, 0, 1, 2, 3]:
+        if cand is not None:
+            autotune_pin("cnn_forward", cand)
+        try:
+            out = net.forward(x)
+        finally:
+            autotune_pin("cnn_forward", None)
+        assert torch.equal(out, ref), cand
