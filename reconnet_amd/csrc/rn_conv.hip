@@ -162,134 +162,79 @@ __device__ __forceinline__ void row_fragments(const RowLoad& r, bf16x8 (&f)[3])
 }
 
 // ---------------------------------------------------------------------------
-// 32 -> 32 layer on MFMA.  wt: [9][32 co][32 ci] bf16 (tap t = 3*dy + dx).
+// 32 -> 32 layer on MFMA: v_mfma_f32_32x32x16_bf16, which on gfx950 moves
+// twice the flops per cycle of the 16x16x32 form (measured 2.17 against 1.22
+// PFLOP/s on MI355X).  wt: [9][32 co][32 ci] bf16 (tap t = 3*dy + dx).
 // mode 0: out = relu?(acc + bias); mode 1: out = acc * (mask > 0)
 //
-// One wave walks down a column of 16 pixels x 32 channels of one image over L
-// output rows; the four waves of a workgroup take adjacent 16-pixel columns,
-// so the grid is (W / 64, H / L, B) and needs no index division.  Each input
-// row is read once (row_fragments: 16 pixels x 32 channels, 16 bytes per lane,
-// the column-shifted copies by DPP) and feeds the three output rows that use it
-// (taps dy = 2, 1, 0 of rows y - 1, y, y + 1) through three rolling
-// accumulators; the oldest is complete after its third row and is stored.
-// The weight fragments of all 9 taps and both 16-channel output tiles stay in
-// registers for the L rows.
-// ---------------------------------------------------------------------------
-// One completed output row of a wave: out = relu?(acc + bias) (mode 0) or
-// acc * (mask > 0) (mode 1), as bf16.  The 16 pixels x 32 channels are
-// gathered in the wave's 1 KB LDS row buffer `sb` and written with one
-// contiguous 16-byte store per lane (pixel lane / 4, channels 8 (lane % 4) ..),
-// 1 KB per instruction instead of 8-byte pieces of every pixel record.
-template <int MODE>
-__device__ __forceinline__ void conv_mid_store_lds(const f32x4 (&acc)[2], const float (&bv)[2][4],
-                                                   const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
-                                                   long rowbase, int xseg, int W, int r16, int g4, int lane, int relu,
-                                                   uint16_t* sb)
-{
-    constexpr int mode = MODE;
-    #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        const int c0 = 16 * mt + 4 * g4;
-        float z[4];
-        if (mode == 0) {
-            #pragma unroll
-            for (int i = 0; i < 4; ++i) { z[i] = acc[mt][i] + bv[mt][i]; if (relu) z[i] = fmaxf(z[i], 0.0f); }
-        } else {
-            uint2 m = make_uint2(0, 0);
-            if (xseg + r16 < W) m = *reinterpret_cast<const uint2*>(mask + (rowbase + xseg + r16) * 32 + c0);
-            const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16),
-                                    (uint16_t)(m.y & 0xffff), (uint16_t)(m.y >> 16)};
-            #pragma unroll
-            for (int i = 0; i < 4; ++i) z[i] = bf2f(mk[i]) > 0.0f ? acc[mt][i] : 0.0f;
-        }
-        uint2 ov;
-        ov.x = pack_bf16(z[0], z[1]);
-        ov.y = pack_bf16(z[2], z[3]);
-        *reinterpret_cast<uint2*>(sb + r16 * 32 + c0) = ov;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const uint4 v = *reinterpret_cast<const uint4*>(sb + lane * 8);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (xseg + lane / 4 < W) *reinterpret_cast<uint4*>(out + (rowbase + xseg) * 32 + lane * 8) = v;
-}
-
-template <int L, int MODE>
-__global__ __launch_bounds__(256, 3) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
-                                                  const float* __restrict__ bias, const uint16_t* __restrict__ mask,
-                                                  uint16_t* __restrict__ out, int H, int W, int relu)
-{
-    constexpr int mode = MODE;
-    __shared__ uint16_t sbuf[4][16 * 32];
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int r16 = lane & 15, g4 = lane >> 4;
-    const int xpix = (blockIdx.x * 4 + wv) * 16 + r16;   // this lane's B column (pixel)
-    const bool store = xpix < W;
-    const int y0 = blockIdx.y * L;
-    const long img = (long)blockIdx.z * H * W;
-    // weight fragments: A[m = co][k = ci], lane holds co = r16 (+16 mt), ci = 8 g4 .. 8 g4 + 7
-    bf16x8 wa[9][2];
-    #pragma unroll
-    for (int t = 0; t < 9; ++t)
-        #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-            wa[t][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((t * 32 + 16 * mt + r16) * 32 + 8 * g4)));
-    // D[m = co][n = pixel]: lane holds co = 16 mt + 4 g4 + i for pixel r16
-    float bv[2][4];
-    #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-        #pragma unroll
-        for (int i = 0; i < 4; ++i) bv[mt][i] = (mode == 0) ? bias[16 * mt + 4 * g4 + i] : 0.0f;
-    // a0: output row yy - 1 (gets tap dy = 2 and completes), a1: row yy (dy = 1), a2: row yy + 1 (dy = 0)
-    f32x4 a0[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, a1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    f32x4 a2[2];
-    // the next input row's loads are issued before this row's MFMAs
-    RowLoad nxt = row_load(in, img, y0 - 1, H, W, xpix, r16, g4);
-    #pragma unroll 3
-    for (int ir = 0; ir < L + 2; ++ir) {
-        const int yy = y0 + ir - 1;                      // input row
-        const RowLoad cur = nxt;
-        if (ir + 1 < L + 2) nxt = row_load(in, img, yy + 1, H, W, xpix, r16, g4);
-        bf16x8 f[3];
-        row_fragments(cur, f);
-        a2[0] = a2[1] = f32x4{0, 0, 0, 0};
-        #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                a0[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx][mt], f[dx], a0[mt], 0, 0, 0);
-                a1[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx][mt], f[dx], a1[mt], 0, 0, 0);
-                a2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx][mt], f[dx], a2[mt], 0, 0, 0);
-            }
-        }
-        const int yo = yy - 1;                           // the row a0 completes
-        if (ir >= 2 && yo < H)
-            conv_mid_store_lds<MODE>(a0, bv, mask, out, img + (long)yo * W, xpix - r16, W, r16, g4, lane, relu,
-                                     sbuf[wv]);
-        a0[0] = a1[0]; a0[1] = a1[1];
-        a1[0] = a2[0]; a1[1] = a2[1];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// 32 -> 32 layer with the input rows staged in LDS.  Same walk, fragments and
-// MFMA sequence as k_conv_mid (so the same bits), but the NW waves of a
-// workgroup (16 NW adjacent pixel columns) share each input row: the row's
-// 16 NW + 2 pixels (the columns and a 1-pixel halo, 64 bytes each) are loaded
-// once per workgroup with coalesced 16-byte loads, two rows ahead of their
-// use, into one of two LDS row buffers, and every wave reads its three
-// column-shifted fragments from there (one ds_read_b128 each).  The vector
-// memory path then moves each input byte about once per workgroup row instead
-// of once per wave plus two edge loads.
+// Operands for tap t and input-channel half kc (ci 16 kc .. 16 kc + 15):
+// A[m = co][k] on lane l: co = l % 32, ci = 16 kc + 8 (l / 32) .. + 7 (one
+// 16-byte load of wt); B[k][n] on lane l: pixel n = l % 32 of a 32-pixel tile,
+// the same 8 channels; D on lane l, register i: co = 8 (i / 4) + 4 (l / 32) +
+// i % 4 of pixel l % 32.  Every output accumulates taps 0 .. 8, each with
+// kc = 0 then 1, in that order, in the layer-by-layer and the fused kernels
+// alike, so they give the same bits.
 //
-// LDS row layout: [channel group g][pixel], 16 bytes per entry, pitch NPP
-// with NPP = 4 (mod 16) entries: the 16 lanes of a fragment read (same g,
-// consecutive pixels) and of a staging write (4 pixels x 4 groups) hit
-// distinct banks.
+// Activation rows in LDS: [channel group g][pixel] 16-byte entries (g = 8
+// channels) with a pitch of 4 (mod 16) entries, so the 32 lanes of a fragment
+// read (same g, consecutive pixels) and a staging write (4 pixels x 4 groups
+// per 16 lanes) hit distinct banks.
 // ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x16 zero16()
+{
+    f32x16 z;
+    #pragma unroll
+    for (int i = 0; i < 16; ++i) z[i] = 0.0f;
+    return z;
+}
+
+// the 18 weight fragments of a 32 -> 32 layer
+__device__ __forceinline__ void load_w32(const uint16_t* __restrict__ wt, int lane, bf16x8 (&wa)[9][2])
+{
+    #pragma unroll
+    for (int tp = 0; tp < 9; ++tp)
+        #pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+            wa[tp][kc] = as_bf16x8(*reinterpret_cast<const uint4*>(
+                wt + ((tp * 32 + (lane & 31)) * 32 + 16 * kc + 8 * (lane >> 5))));
+}
+
+// One input row of a 32-pixel tile: f[dx][kc] are its fragments at column
+// offset dx - 1; it adds taps 6 .. 8 to A (output row above), 3 .. 5 to Bm
+// (same row) and 0 .. 2 to Cm (row below).
+__device__ __forceinline__ void mid32_row(const bf16x8 (&wa)[9][2], const bf16x8 (&f)[3][2], f32x16& A, f32x16& Bm,
+                                          f32x16& Cm)
+{
+    #pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+        #pragma unroll
+        for (int kc = 0; kc < 2; ++kc) {
+            A = mfma32(wa[6 + dx][kc], f[dx][kc], A);
+            Bm = mfma32(wa[3 + dx][kc], f[dx][kc], Bm);
+            Cm = mfma32(wa[dx][kc], f[dx][kc], Cm);
+        }
+}
+
+// fragments of the tile whose dx = 0 pixel is entry p of the row buffer
+template <int NPP>
+__device__ __forceinline__ void frags32(const uint4* __restrict__ rb, int p, int lane, bf16x8 (&f)[3][2])
+{
+    #pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+        #pragma unroll
+        for (int kc = 0; kc < 2; ++kc) f[dx][kc] = as_bf16x8(rb[(2 * kc + (lane >> 5)) * NPP + p + dx]);
+}
+
 template <int NW>
 struct ConvRowStage {
-    static constexpr int NP = 16 * NW + 2;                        // pixels of a staged row
+    static constexpr int NP = 32 * NW + 2;                        // pixels of a staged row (1-pixel halo)
     static constexpr int NPP = NP + (((4 - NP) % 16) + 16) % 16;  // pitch = 4 (mod 16)
     static constexpr int NCH = NP * 4;                            // 16-byte chunks of a row
     static constexpr int NT = 64 * NW;                            // threads
@@ -323,132 +268,199 @@ __device__ __forceinline__ void stage_store(uint4* __restrict__ buf, int t, cons
     }
 }
 
+// One completed output row of a wave's 32 pixels: relu?(acc + bias) (mode 0)
+// or acc * (mask > 0) (mode 1) as bf16, gathered in the wave's LDS tile
+// (80-byte pixel pitch: conflict-free 8-byte writes) and written as two
+// contiguous 16-byte stores per lane.
+template <int MODE>
+__device__ __forceinline__ void store32(const f32x16& acc, const float (&bv)[16], const uint16_t* __restrict__ mask,
+                                        uint16_t* __restrict__ out, long rowbase, int xw, int W, int lane, int relu,
+                                        uint4* __restrict__ tile)
+{
+    const int n = lane & 31, h = lane >> 5;
+    uint2* tw = reinterpret_cast<uint2*>(tile);
+    #pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float z[4];
+        if (MODE == 0) {
+            #pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                z[e] = acc[4 * q + e] + bv[4 * q + e];
+                if (relu) z[e] = fmaxf(z[e], 0.0f);
+            }
+        } else {
+            uint2 m = make_uint2(0, 0);
+            if (xw + n < W) m = *reinterpret_cast<const uint2*>(mask + (rowbase + xw + n) * 32 + 8 * q + 4 * h);
+            const uint16_t mk[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16), (uint16_t)(m.y & 0xffff),
+                                    (uint16_t)(m.y >> 16)};
+            #pragma unroll
+            for (int e = 0; e < 4; ++e) z[e] = bf2f(mk[e]) > 0.0f ? acc[4 * q + e] : 0.0f;
+        }
+        tw[n * 10 + 2 * q + h] = make_uint2(pack_bf16(z[0], z[1]), pack_bf16(z[2], z[3]));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    #pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int e = lane + 64 * k;                 // 16-byte chunk: pixel e / 4, channels 8 (e % 4) ..
+        const uint4 v = tile[(e >> 2) * 5 + (e & 3)];
+        if (xw + (e >> 2) < W) *reinterpret_cast<uint4*>(out + (rowbase + xw) * 32 + (long)e * 8) = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// A workgroup of NW waves takes 32 NW adjacent columns and walks down L output
+// rows of one image.  Each input row (the columns and a 1-pixel halo) is
+// loaded once per workgroup, two rows ahead of its use, into one of two LDS
+// row buffers, and feeds the three output rows that use it through three
+// rolling accumulators; the oldest is complete after its third row and is
+// stored.  The 18 weight fragments stay in registers.
 template <int L, int MODE, int NW>
-__global__ __launch_bounds__(64 * NW) void k_conv_mid_s(const uint16_t* __restrict__ in,
-                                                      const uint16_t* __restrict__ wt,
-                                                      const float* __restrict__ bias,
-                                                      const uint16_t* __restrict__ mask,
-                                                      uint16_t* __restrict__ out, int H, int W, int relu)
+__global__ __launch_bounds__(64 * NW) void k_conv_mid(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+                                                    const float* __restrict__ bias,
+                                                    const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
+                                                    int H, int W, int relu)
 {
     using S = ConvRowStage<NW>;
     __shared__ uint4 rows[2][4 * S::NPP];
-    __shared__ uint16_t sbuf[NW][16 * 32];
+    __shared__ uint4 tiles[NW][32 * 5];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;
-    const int r16 = lane & 15, g4 = lane >> 4;
-    const int x0 = blockIdx.x * 16 * NW;                 // first column of the workgroup
-    const int xpix = x0 + wv * 16 + r16;                 // this lane's B column (pixel)
+    const int x0 = blockIdx.x * 32 * NW;                 // first column of the workgroup
+    const int xw = x0 + 32 * wv;                         // first column of this wave
     const int y0 = blockIdx.y * L;
     const long img = (long)blockIdx.z * H * W;
     bf16x8 wa[9][2];
+    load_w32(wt, lane, wa);
+    float bv[16];
     #pragma unroll
-    for (int tp = 0; tp < 9; ++tp)
-        #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-            wa[tp][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((tp * 32 + 16 * mt + r16) * 32 + 8 * g4)));
-    float bv[2][4];
-    #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-        #pragma unroll
-        for (int i = 0; i < 4; ++i) bv[mt][i] = (MODE == 0) ? bias[16 * mt + 4 * g4 + i] : 0.0f;
-    f32x4 a0[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, a1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    f32x4 a2[2];
+    for (int i = 0; i < 16; ++i) bv[i] = (MODE == 0) ? bias[8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)] : 0.0f;
+    f32x16 a0 = zero16(), a1 = zero16(), a2 = zero16();
     // step ir uses input row y0 - 1 + ir from buffer ir & 1
     uint4 nx[S::K], nn[S::K];
     stage_load<NW>(in, img, y0 - 1, H, W, x0, t, nx);
     stage_store<NW>(rows[0], t, nx);
     stage_load<NW>(in, img, y0, H, W, x0, t, nx);
-    const int rd = (wv * 16 + r16) + g4 * S::NPP;        // fragment dx = 0 of this lane
-    for (int ir = 0; ir < L + 2; ++ir) {
+    const int p0 = 32 * wv + (lane & 31);                // entry of the pixel left of this lane's pixel
+    auto step = [&](int ir, f32x16& A, f32x16& Bm, f32x16& Cm) {
         __syncthreads();                                 // buffer ir & 1 complete, buffer (ir + 1) & 1 free
-        const uint4* rb = rows[ir & 1];
-        bf16x8 f[3];
-        #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) f[dx] = as_bf16x8(rb[rd + dx]);
+        bf16x8 f[3][2];
+        frags32<S::NPP>(rows[ir & 1], p0, lane, f);
         if (ir + 2 < L + 2) stage_load<NW>(in, img, y0 + ir + 1, H, W, x0, t, nn);
-        a2[0] = a2[1] = f32x4{0, 0, 0, 0};
-        #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                a0[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx][mt], f[dx], a0[mt], 0, 0, 0);
-                a1[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx][mt], f[dx], a1[mt], 0, 0, 0);
-                a2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx][mt], f[dx], a2[mt], 0, 0, 0);
-            }
-        }
-        const int yo = y0 + ir - 2;                      // the row a0 completes
-        if (ir >= 2 && yo < H)
-            conv_mid_store_lds<MODE>(a0, bv, mask, out, img + (long)yo * W, xpix - r16, W, r16, g4, lane, relu,
-                                     sbuf[wv]);
+        Cm = zero16();
+        mid32_row(wa, f, A, Bm, Cm);
+        const int yo = y0 + ir - 2;                      // the row A completes
+        if (ir >= 2 && yo < H) store32<MODE>(A, bv, mask, out, img + (long)yo * W, xw, W, lane, relu, tiles[wv]);
         if (ir + 1 < L + 2) stage_store<NW>(rows[(ir + 1) & 1], t, nx);
         #pragma unroll
         for (int k = 0; k < S::K; ++k) nx[k] = nn[k];
-        a0[0] = a1[0]; a0[1] = a1[1];
-        a1[0] = a2[0]; a1[1] = a2[1];
+    };
+    for (int ir = 0; ir < L + 2; ir += 3) {
+        step(ir, a0, a1, a2);
+        if (ir + 1 < L + 2) step(ir + 1, a1, a2, a0);
+        if (ir + 2 < L + 2) step(ir + 2, a2, a0, a1);
     }
 }
 
 // ---------------------------------------------------------------------------
-// layer 5: y = xres + b + sum_t sum_ci a4[p + t][ci] w5[t][ci]   (fp32 out)
+// layer 5: y = xres + b + sum_t sum_ci a4[p + off_t][ci] w5[t][ci]   (fp32 out)
 //
-// On MFMA with the layout and rolling row walk of k_conv_mid, one 16-channel
-// output tile of which two rows are used: the FP32 weights are split into
-// w5 = hi + lo with hi = bf16(w5) and lo = bf16(w5 - hi), held as output
-// channels 0 and 1 of the A fragment, so y = xres + b + (acc[0] + acc[1])
-// keeps the weights to about 16 significant bits.
+// As a 1x1 convolution on MFMA followed by a 3x3 sum on the VALU.  The FP32
+// weights are split into w5 = hi + lo (hi = bf16(w5), lo = bf16(w5 - hi),
+// about 16 significant bits together) and held as the 18 rows m = 2 t + h of
+// a 32-row A operand (rows 18 .. 31 zero), so one v_mfma_f32_32x32x16_bf16
+// pair per 32 pixels gives U[p][2 t + h] = sum_ci a4[p][ci] w5_h[t][ci] for
+// all 9 taps; then S_h = sum over t = 0 .. 8 in order of U[p + off_t][2 t + h]
+// and y = xres + (b + (S_0 + S_1)).  The layer-by-layer and the fused kernels
+// both compute exactly this.
 // ---------------------------------------------------------------------------
-template <int L>
-__global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a, const float* __restrict__ w,
-                                                  float b5, const float* __restrict__ xres, float* __restrict__ y,
-                                                  int H, int W)
+// A fragments (kc = 0, 1) of the split layer-5 weights
+__device__ __forceinline__ void load_w5(const float* __restrict__ w, int lane, bf16x8 (&wa)[2])
 {
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int r16 = lane & 15, g4 = lane >> 4;
-    const int xpix = (blockIdx.x * 4 + wv) * 16 + r16;
-    const int y0 = blockIdx.y * L;
-    const long img = (long)blockIdx.z * H * W;
-    // A[m = co][k = ci]: row 0 = hi, row 1 = lo, rows 2..15 = 0
-    bf16x8 wa[9];
+    const int m = lane & 31, t = m >> 1;
     #pragma unroll
-    for (int t = 0; t < 9; ++t) {
+    for (int kc = 0; kc < 2; ++kc) {
         uint32_t u[4];
         #pragma unroll
         for (int e = 0; e < 4; ++e) {
             float v[2];
             #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const float wf = w[t * 32 + 8 * g4 + 2 * e + h];
+            for (int q = 0; q < 2; ++q) {
+                const int ci = 16 * kc + 8 * (lane >> 5) + 2 * e + q;
+                const float wf = m < 18 ? w[t * 32 + ci] : 0.0f;
                 const float hi = (float)(__bf16)wf;
-                v[h] = r16 == 0 ? wf : (r16 == 1 ? wf - hi : 0.0f);
+                v[q] = (m & 1) ? wf - hi : hi;
             }
             u[e] = pack_bf16(v[0], v[1]);
         }
-        wa[t] = as_bf16x8(make_uint4(u[0], u[1], u[2], u[3]));
+        wa[kc] = as_bf16x8(make_uint4(u[0], u[1], u[2], u[3]));
     }
-    f32x4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0}, a2;
-    #pragma unroll 3
+}
+
+// the U values (rows m < 18) of D on this lane into a U row [pixel][18]
+__device__ __forceinline__ void put_u(const f32x16& U, float* __restrict__ urow, int p, int lane)
+{
+    #pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int m = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        if (m < 18) urow[p * 18 + m] = U[i];
+    }
+}
+
+// S_0 + S_1 of the pixel whose left neighbour is entry p of the U rows
+// r0 (row above), r1, r2 (row below)
+__device__ __forceinline__ float sum_u(const float* __restrict__ r0, const float* __restrict__ r1,
+                                       const float* __restrict__ r2, int p)
+{
+    const float* rr[3] = {r0, r1, r2};
+    float s0 = 0.0f, s1 = 0.0f;
+    #pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const float* u = rr[t / 3] + (p + t % 3) * 18 + 2 * t;
+        s0 += u[0];
+        s1 += u[1];
+    }
+    return s0 + s1;
+}
+
+// One wave: U for the 32 columns xw - 1 .. xw + 30 of each input row, outputs
+// for the 30 columns xw .. xw + 29 of each of L rows (3 U rows in LDS).
+template <int L>
+__global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a, const float* __restrict__ w,
+                                                  float b5, const float* __restrict__ xres, float* __restrict__ y,
+                                                  int H, int W)
+{
+    __shared__ float ur[4][3][32 * 18];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int n = lane & 31, h = lane >> 5;
+    const int xw = (blockIdx.x * 4 + wv) * 30;
+    const int col = xw - 1 + n;                         // this lane's U column
+    const int y0 = blockIdx.y * L;
+    const long img = (long)blockIdx.z * H * W;
+    bf16x8 wa[2];
+    load_w5(w, lane, wa);
     for (int ir = 0; ir < L + 2; ++ir) {
-        const int yy = y0 + ir - 1;
-        const RowLoad cur = row_load(a, img, yy, H, W, xpix, r16, g4);
-        bf16x8 f[3];
-        row_fragments(cur, f);
-        a2 = f32x4{0, 0, 0, 0};
+        const int yy = y0 + ir - 1;                     // input row
+        bf16x8 f[2];
         #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx], f[dx], a0, 0, 0, 0);
-            a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx], f[dx], a1, 0, 0, 0);
-            a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx], f[dx], a2, 0, 0, 0);
+        for (int kc = 0; kc < 2; ++kc) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < H && col >= 0 && col < W)
+                v = *reinterpret_cast<const uint4*>(a + (img + (long)yy * W + col) * 32 + 16 * kc + 8 * h);
+            f[kc] = as_bf16x8(v);
         }
-        // D[m = co][n = pixel]: lanes 0..15 (g4 = 0) hold co 0..3 of pixel r16
+        f32x16 U = mfma32(wa[0], f[0], zero16());
+        U = mfma32(wa[1], f[1], U);
+        put_u(U, ur[wv][ir % 3], n, lane);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const int yo = yy - 1;
-        if (ir >= 2 && yo < H && xpix < W && g4 == 0) {
-            const long p = img + (long)yo * W + xpix;
-            y[p] = (xres ? xres[p] : 0.0f) + (b5 + (a0[0] + a0[1]));
+        if (ir >= 2 && yo < H && lane < 30 && xw + lane < W) {
+            const float sv = sum_u(ur[wv][(ir + 1) % 3], ur[wv][(ir + 2) % 3], ur[wv][ir % 3], lane);
+            const long q = img + (long)yo * W + xw + lane;
+            y[q] = (xres ? xres[q] : 0.0f) + (b5 + sv);
         }
-        a0 = a1;
-        a1 = a2;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
 }
 
@@ -490,9 +502,11 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
                                                    float* __restrict__ y, int H, int W)
 {
     using F = FusedCnn<T>;
+    static_assert(T % 2 == 0, "32-pixel tiles of the 32 -> 32 layers");
     constexpr int NPP = F::NPP, NP = F::NP;
     __shared__ uint4 act[4][2][4 * NPP];     // rows of a1 .. a4
     __shared__ float xr[3][NP];              // x rows y - 1, y, y + 1 of the a1 row (wave 0)
+    __shared__ float ur[3][NP * 18];         // U rows of the output layer (wave 0)
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wv = t >> 6;
@@ -503,24 +517,10 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
     for (int i = t; i < 4 * 2 * 4 * NPP; i += 256) (&act[0][0][0])[i] = make_uint4(0, 0, 0, 0);
 
     if (wv == 0) {
-        // ---- 1 -> 32 (pixel per lane) and 32 -> 1 (MFMA, one 16-channel tile: hi / lo weights) ----
-        bf16x8 wa5[9];
-        #pragma unroll
-        for (int tp = 0; tp < 9; ++tp) {
-            uint32_t u[4];
-            #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                float v[2];
-                #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const float wf = w5[tp * 32 + 8 * g4 + 2 * e + h];
-                    const float hi = (float)(__bf16)wf;
-                    v[h] = r16 == 0 ? wf : (r16 == 1 ? wf - hi : 0.0f);
-                }
-                u[e] = pack_bf16(v[0], v[1]);
-            }
-            wa5[tp] = as_bf16x8(make_uint4(u[0], u[1], u[2], u[3]));
-        }
+        // ---- 1 -> 32 (VALU) and 32 -> 1 (1x1 MFMA + 3x3 sum, as k_conv_out) ----
+        bf16x8 wa5[2];
+        load_w5(w5, lane, wa5);
+        for (int i = lane; i < 3 * NP * 18; i += 64) (&ur[0][0])[i] = 0.0f;
         // layer-1 weights and biases of this lane's 8 channels
         float ww1[9][8], bb1[8];
         #pragma unroll
@@ -529,13 +529,12 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
             #pragma unroll
             for (int tp = 0; tp < 9; ++tp) ww1[tp][e] = w1[tp * 32 + 8 * g4 + e];
         }
-        f32x4 o0[T], o1[T], o2[T];
-        #pragma unroll
-        for (int i = 0; i < T; ++i) o0[i] = o1[i] = f32x4{0, 0, 0, 0};
         // x rows: lane l holds columns c0 - 1 + l + 64 k of a row (zero outside the
         // image); each row is loaded one step before it enters the LDS ring, and
-        // the residual values of the y row one step before they are added
+        // the residual values of the y row (columns c0 + l + 64 k) one step before
+        // they are added
         constexpr int KX = (NP + 63) / 64;
+        constexpr int KY = (16 * T + 63) / 64;
         auto fetch_x = [&](int row, float (&v)[KX]) {
             const bool rok = row >= 0 && row < H;
             #pragma unroll
@@ -550,14 +549,14 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
             for (int k = 0; k < KX; ++k)
                 if (lane + 64 * k < NP) xr[slot][lane + 64 * k] = v[k];
         };
-        auto fetch_res = [&](int row, float (&v)[T]) {
+        auto fetch_res = [&](int row, float (&v)[KY]) {
             #pragma unroll
-            for (int i = 0; i < T; ++i) {
-                const int c = c0 + 16 * i + r16;
-                v[i] = (row >= 0 && row < H && c >= 0 && c < W) ? x[img + (long)row * W + c] : 0.0f;
+            for (int k = 0; k < KY; ++k) {
+                const int c = c0 + lane + 64 * k;
+                v[k] = (row >= 0 && row < H && c >= 0 && c < W) ? x[img + (long)row * W + c] : 0.0f;
             }
         };
-        float xn[KX], xres[T];
+        float xn[KX], xres[KY];
         fetch_x(y0 - 5, xn);
         put_x(0, xn);
         fetch_x(y0 - 4, xn);
@@ -565,10 +564,9 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
         fetch_x(y0 - 3, xn);
         fetch_res(y0 - 12, xres);
         __syncthreads();
-        // one pipeline step; A completes (taps of dy = +1), Bm takes dy = 0, Cm starts
-        // with dy = -1; the three rotate over consecutive steps, so no copies (x row
-        // y0 - 5 + s + dy is in ring slot (s + dy) % 3, with PH = s % 3)
-        auto step = [&](auto ph, int s, f32x4 (&A)[T], f32x4 (&Bm)[T], f32x4 (&Cm)[T]) {
+        // one pipeline step (x row y0 - 5 + s + dy in ring slot (s + dy) % 3, U row
+        // y0 - 13 + s + dy of the output layer in slot (s + 1 + dy) % 3; PH = s % 3)
+        auto step = [&](auto ph, int s) {
             constexpr int PH = decltype(ph)::value;
             put_x((PH + 2) % 3, xn);                    // x row y0 - 3 + s
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -600,86 +598,73 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
                 }
                 a1w[g4 * NPP + pp + 1] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
             }
-            // output layer on the a4 row written at step s - 1; completes y row y0 - 12 + s
+            // output layer: U row of the a4 row y0 - 11 + s written at step s - 1,
+            // then the 3x3 sum of y row y0 - 12 + s
             const uint4* a4r = act[3][(s + 1) & 1];
+            const int n = lane & 31, h = lane >> 5;
+            #pragma unroll
+            for (int j = 0; j < T / 2; ++j) {
+                const bf16x8 f0 = as_bf16x8(a4r[h * NPP + 32 * j + n + 1]);
+                const bf16x8 f1 = as_bf16x8(a4r[(2 + h) * NPP + 32 * j + n + 1]);
+                f32x16 U = mfma32(wa5[0], f0, zero16());
+                U = mfma32(wa5[1], f1, U);
+                put_u(U, ur[PH], 32 * j + n + 1, lane);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             const int ry = y0 - 12 + s;
             #pragma unroll
-            for (int i = 0; i < T; ++i) {
-                bf16x8 f[3];
-                #pragma unroll
-                for (int dx = 0; dx < 3; ++dx) f[dx] = as_bf16x8(a4r[g4 * NPP + 16 * i + r16 + dx]);
-                Cm[i] = f32x4{0, 0, 0, 0};
-                #pragma unroll
-                for (int dx = 0; dx < 3; ++dx) {
-                    A[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa5[6 + dx], f[dx], A[i], 0, 0, 0);
-                    Bm[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa5[3 + dx], f[dx], Bm[i], 0, 0, 0);
-                    Cm[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa5[dx], f[dx], Cm[i], 0, 0, 0);
+            for (int k = 0; k < KY; ++k) {
+                const int pp = lane + 64 * k;           // computed column c0 + pp
+                if (s >= 12 && ry < H && pp >= 4 && pp < 4 + F::WO && c0 + pp < W) {
+                    const float sv = sum_u(ur[(PH + 1) % 3], ur[(PH + 2) % 3], ur[PH], pp);
+                    y[img + (long)ry * W + c0 + pp] = xres[k] + (b5 + sv);
                 }
-                const int c = c0 + 16 * i + r16;
-                if (s >= 12 && ry < H && g4 == 0 && 16 * i + r16 >= 4 && 16 * i + r16 < 4 + F::WO && c < W)
-                    y[img + (long)ry * W + c] = xres[i] + (b5 + (A[i][0] + A[i][1]));
             }
             fetch_res(ry + 1, xres);                    // for the next step
             __syncthreads();
         };
         for (int s = 0; s < L + 12; s += 3) {
-            step(std::integral_constant<int, 0>{}, s, o0, o1, o2);
-            if (s + 1 < L + 12) step(std::integral_constant<int, 1>{}, s + 1, o1, o2, o0);
-            if (s + 2 < L + 12) step(std::integral_constant<int, 2>{}, s + 2, o2, o0, o1);
+            step(std::integral_constant<int, 0>{}, s);
+            if (s + 1 < L + 12) step(std::integral_constant<int, 1>{}, s + 1);
+            if (s + 2 < L + 12) step(std::integral_constant<int, 2>{}, s + 2);
         }
     } else {
-        // ---- 32 -> 32 layer wv + 1: relu(conv(a_wv) + b) ----
-        const uint16_t* wt = wv == 1 ? w2 : (wv == 2 ? w3 : w4);
-        const float* bias = wv == 1 ? b2 : (wv == 2 ? b3 : b4);
+        // ---- 32 -> 32 layer wv + 1: relu(conv(a_wv) + b), T / 2 tiles of 32 pixels ----
+        constexpr int T2 = T / 2;
         bf16x8 wa[9][2];
+        load_w32(wv == 1 ? w2 : (wv == 2 ? w3 : w4), lane, wa);
+        const float* bias = wv == 1 ? b2 : (wv == 2 ? b3 : b4);
+        float bv[16];
         #pragma unroll
-        for (int tp = 0; tp < 9; ++tp)
-            #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-                wa[tp][mt] = as_bf16x8(*reinterpret_cast<const uint4*>(wt + ((tp * 32 + 16 * mt + r16) * 32 + 8 * g4)));
-        float bv[2][4];
+        for (int i = 0; i < 16; ++i) bv[i] = bias[8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)];
+        const int n = lane & 31, h = lane >> 5;
+        f32x16 m0[T2], m1[T2], m2[T2];
         #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-            #pragma unroll
-            for (int i = 0; i < 4; ++i) bv[mt][i] = bias[16 * mt + 4 * g4 + i];
-        f32x4 m0[T][2], m1[T][2], m2[T][2];
-        #pragma unroll
-        for (int i = 0; i < T; ++i)
-            #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) m0[i][mt] = m1[i][mt] = f32x4{0, 0, 0, 0};
+        for (int j = 0; j < T2; ++j) m0[j] = m1[j] = m2[j] = zero16();
         __syncthreads();
         // one pipeline step (accumulator roles rotate as in wave 0's steps)
-        auto step = [&](int s, f32x4 (&A)[T][2], f32x4 (&Bm)[T][2], f32x4 (&Cm)[T][2]) {
+        auto step = [&](int s, f32x16 (&A)[T2], f32x16 (&Bm)[T2], f32x16 (&Cm)[T2]) {
             const uint4* in = act[wv - 1][(s + 1) & 1];   // written at step s - 1
             uint2* ow = reinterpret_cast<uint2*>(act[wv][s & 1]);
             const int ro = y0 - 4 - 2 * wv + s;            // the row this step completes
             const bool rok = ro >= 0 && ro < H;
             #pragma unroll
-            for (int i = 0; i < T; ++i) {
-                bf16x8 f[3];
-                #pragma unroll
-                for (int dx = 0; dx < 3; ++dx) f[dx] = as_bf16x8(in[g4 * NPP + 16 * i + r16 + dx]);
-                Cm[i][0] = Cm[i][1] = f32x4{0, 0, 0, 0};
-                #pragma unroll
-                for (int dx = 0; dx < 3; ++dx) {
-                    #pragma unroll
-                    for (int mt = 0; mt < 2; ++mt) {
-                        A[i][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[6 + dx][mt], f[dx], A[i][mt], 0, 0, 0);
-                        Bm[i][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[3 + dx][mt], f[dx], Bm[i][mt], 0, 0, 0);
-                        Cm[i][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dx][mt], f[dx], Cm[i][mt], 0, 0, 0);
-                    }
-                }
-                // D[m = co][n = pixel]: this lane holds channels 16 mt + 4 g4 .. + 3 of pixel r16
-                const int c = c0 + 16 * i + r16;
+            for (int j = 0; j < T2; ++j) {
+                bf16x8 f[3][2];
+                frags32<NPP>(in, 32 * j + n, lane, f);
+                Cm[j] = zero16();
+                mid32_row(wa, f, A[j], Bm[j], Cm[j]);
+                // this lane holds channels 8 q + 4 h .. + 3 of pixel n of the tile
+                const int c = c0 + 32 * j + n;
                 const bool ok = rok && c >= 0 && c < W;
                 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt) {
+                for (int q = 0; q < 4; ++q) {
                     float z[4];
                     #pragma unroll
-                    for (int e = 0; e < 4; ++e) z[e] = fmaxf(A[i][mt][e] + bv[mt][e], 0.0f);
+                    for (int e = 0; e < 4; ++e) z[e] = fmaxf(A[j][4 * q + e] + bv[4 * q + e], 0.0f);
                     uint2 ov = make_uint2(0, 0);
                     if (ok) { ov.x = pack_bf16(z[0], z[1]); ov.y = pack_bf16(z[2], z[3]); }
-                    ow[((2 * mt + (g4 >> 1)) * NPP + 16 * i + r16 + 1) * 2 + (g4 & 1)] = ov;
+                    ow[(q * NPP + 32 * j + n + 1) * 2 + h] = ov;
                 }
             }
             __syncthreads();
@@ -938,17 +923,17 @@ void conv_mid(const uint16_t* in, const uint16_t* wt, const float* bias, const u
     RN_REQUIRE(mask != nullptr || bias != nullptr, "conv_mid needs a bias (forward) or a mask (data gradient)");
     RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "conv_mid: bad batch or image size");
     const int rl = relu ? 1 : 0;
-    // candidates: LDS-staged rows (4 or 8 waves, 16 or 32 rows per workgroup), then the per-wave DPP walk
+    // candidates: workgroups of 2, 4 or 8 waves of 32 columns, 16 or 32 rows
     auto launch = [&](int c) {
-        auto staged = [&](auto kern, int nw, int rows) {
-            const dim3 grd((unsigned)cdiv(W, 16 * nw), (unsigned)cdiv(H, rows), (unsigned)B);
+        auto go = [&](auto kern, int nw, int rows) {
+            const dim3 grd((unsigned)cdiv(W, 32 * nw), (unsigned)cdiv(H, rows), (unsigned)B);
             hipLaunchKernelGGL(kern, grd, dim3(64 * nw), 0, st, in, wt, bias, mask, out, H, W, rl);
         };
-        if (c == 0) mask ? staged(k_conv_mid_s<16, 1, 4>, 4, 16) : staged(k_conv_mid_s<16, 0, 4>, 4, 16);
-        else if (c == 1) mask ? staged(k_conv_mid_s<16, 1, 8>, 8, 16) : staged(k_conv_mid_s<16, 0, 8>, 8, 16);
-        else if (c == 2) mask ? staged(k_conv_mid_s<32, 1, 4>, 4, 32) : staged(k_conv_mid_s<32, 0, 4>, 4, 32);
-        else if (c == 3) mask ? staged(k_conv_mid_s<32, 1, 8>, 8, 32) : staged(k_conv_mid_s<32, 0, 8>, 8, 32);
-        else mask ? staged(k_conv_mid<16, 1>, 4, 16) : staged(k_conv_mid<16, 0>, 4, 16);
+        if (c == 0) mask ? go(k_conv_mid<16, 1, 4>, 4, 16) : go(k_conv_mid<16, 0, 4>, 4, 16);
+        else if (c == 1) mask ? go(k_conv_mid<16, 1, 2>, 2, 16) : go(k_conv_mid<16, 0, 2>, 2, 16);
+        else if (c == 2) mask ? go(k_conv_mid<32, 1, 4>, 4, 32) : go(k_conv_mid<32, 0, 4>, 4, 32);
+        else if (c == 3) mask ? go(k_conv_mid<32, 1, 2>, 2, 32) : go(k_conv_mid<32, 0, 2>, 2, 32);
+        else mask ? go(k_conv_mid<16, 1, 8>, 8, 16) : go(k_conv_mid<16, 0, 8>, 8, 16);
         check_launch("conv_mid");
     };
     const int c = tune_choose({TK_CONV_MID, B, H, W, mask ? 1 : 0, rl}, kNConvMid, launch, st, true);
@@ -960,7 +945,7 @@ void conv_out(const uint16_t* a4, const float* w5, float b5, const float* xres, 
 {
     RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "conv_out: bad batch or image size");
     constexpr int kRows = 16;
-    const dim3 grd((unsigned)cdiv(W, 64), (unsigned)cdiv(H, kRows), (unsigned)B);
+    const dim3 grd((unsigned)cdiv(W, 4 * 30), (unsigned)cdiv(H, kRows), (unsigned)B);
     hipLaunchKernelGGL(k_conv_out<kRows>, grd, dim3(256), 0, st, a4, w5, b5, xres, y, H, W);
     check_launch("conv_out");
 }
@@ -970,16 +955,16 @@ void cnn_forward(const float* x, const float* w1, const float* b1, const uint16_
                  float* y, int B, int H, int W, hipStream_t st)
 {
     RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "cnn_forward: bad batch or image size");
-    // candidates: strips of T = 6, 5, 4 tiles of 16 columns, 64 or 128 rows per workgroup
+    // candidates: strips of T = 4 or 6 tiles of 16 columns, 32, 64 or 128 rows per workgroup
     auto launch = [&](int c) {
         auto go = [&](auto kern, int wo, int rows) {
             const dim3 grd((unsigned)cdiv(W, wo), (unsigned)cdiv(H, rows), (unsigned)B);
             hipLaunchKernelGGL(kern, grd, dim3(256), 0, st, x, w1, b1, w2, b2, w3, b3, w4, b4, w5, b5, y, H, W);
         };
-        if (c == 0) go(k_cnn_fwd<6, 64>, FusedCnn<6>::WO, 64);
-        else if (c == 1) go(k_cnn_fwd<5, 64>, FusedCnn<5>::WO, 64);
-        else if (c == 2) go(k_cnn_fwd<4, 64>, FusedCnn<4>::WO, 64);
-        else go(k_cnn_fwd<6, 128>, FusedCnn<6>::WO, 128);
+        if (c == 0) go(k_cnn_fwd<4, 64>, FusedCnn<4>::WO, 64);
+        else if (c == 1) go(k_cnn_fwd<6, 64>, FusedCnn<6>::WO, 64);
+        else if (c == 2) go(k_cnn_fwd<4, 128>, FusedCnn<4>::WO, 128);
+        else go(k_cnn_fwd<4, 32>, FusedCnn<4>::WO, 32);
         check_launch("cnn_forward");
     };
     const int c = tune_choose({TK_CNN_FWD, B, H, W, 0, 0}, 4, launch, st, true);
