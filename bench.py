@@ -43,6 +43,7 @@ Prints one JSON line.
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -60,6 +61,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
+# helical workload (3 turns, pitch 1): raw flat-detector views and the rebinned parallel-cone rows
+_ZROT = 64 * 1.0 * 595.0 / 1085.6
+HELICAL = {"nu": 512, "nview": 2160, "nv": 64,
+           "rebin": dict(du_f=1.2, dv_f=1.0, beta0=0.0, dbeta=math.pi / 360, nu_p=512, nview_p=2160, nv_p=64,
+                         du_p=1.2 * 595.0 / 1085.6, dv_p=1.0, theta0=0.0, dtheta=math.pi / 360, dsd=1085.6,
+                         dso=595.0),
+           "bp": dict(nview_per_pi=360, theta0=0.0, vol_z0=-11.0, zrot=_ZROT, nx=256, ny=256, nz=128, dx=1.0,
+                      dy=1.0, dz=1.0, du=1.2 * 595.0 / 1085.6, dv=1.0, dsd=1085.6, dso=595.0)}
+
 JUDGE_BETA = 0.01
 SPEC_BETA = 1e-4
 
@@ -540,6 +550,22 @@ def config_gpu_timings(torch, dev, reps=5):
     rec("ragged16_siddon_fp", gpu_ms(lambda: rb.forward(flat, "siddon")), nbr)
     rec("ragged16_siddon_bp", gpu_ms(lambda: rb.backward(sino, "siddon")), nbr)
     del rb
+
+    # helical 256^2 x 128 at 1 mm, 3 turns of 720 views at pitch 1 (SID 595, SDD 1085.6 mm):
+    # flat 64 x 512 views (1.0 x 1.2 mm) -> rebinned parallel-cone rows -> weighted BP
+    from reconnet_amd.helical import helical_backproject, rebin_fan_to_parallel
+
+    hw = HELICAL
+    fan = torch.rand((1, hw["nu"], hw["nview"], hw["nv"], 1), generator=torch.Generator(device=dev).manual_seed(5),
+                     device=dev)
+    par = rebin_fan_to_parallel(fan, **hw["rebin"])
+    nbh = 4.0 * (2 * hw["nu"] * hw["nview"] * hw["nv"])
+    rec("helical_rebin", gpu_ms(lambda: rebin_fan_to_parallel(fan, out=par, **hw["rebin"])), nbh)
+    vol = helical_backproject(par, **hw["bp"])
+    nbb = 4.0 * (hw["nu"] * hw["nview"] * hw["nv"] + 256 * 256 * 128)
+    rec("helical_bp", gpu_ms(lambda: helical_backproject(par, out=vol, **hw["bp"])), nbb)
+    out["helical_bp"]["voxel_views_per_s"] = 256 * 256 * 128 * hw["bp"]["nview_per_pi"] * 3 / (
+        out["helical_bp"]["gpu_ms"] * 1e-3)
     return out
 
 
@@ -596,6 +622,15 @@ def cpu_config_timings(configs, threads):
             lambda: orc.siddon_fan_bp(yi, d, qo, (q.nx, q.ny, 1), precision="f32"), reps=1, warm=0)
     for k in ("dd_fp", "siddon_fp", "siddon_bp"):
         t["ragged16_" + k] = tot[k]
+    # helical BP: 2 of the 128 slices (z = 29, 30 mm: whole pi-windows), FP64 oracle, scaled by 64
+    hb = HELICAL["bp"]
+    par = np.random.default_rng(5).uniform(0, 1, (1, HELICAL["nu"], HELICAL["nview"], HELICAL["nv"], 1))
+    par = par.astype(np.float32)
+    hg = dict(dx=hb["dx"], dy=hb["dy"], dz=hb["dz"], du=hb["du"], dv=hb["dv"], dsd=hb["dsd"], dso=hb["dso"])
+    t["helical_bp"] = 64 * median_time(lambda: orc.helical_bp(par, hg, (hb["nx"], hb["ny"], 2), hb["nview_per_pi"],
+                                                              hb["theta0"], hb["vol_z0"] + 40, hb["zrot"]),
+                                       reps=1, warm=0)
+    del par
     for key, sec in t.items():
         if key in configs:
             configs[key]["cpu_ms"] = 1000.0 * sec
@@ -603,7 +638,8 @@ def cpu_config_timings(configs, threads):
     configs["cpu_notes"] = ("FP32 OpenMP oracle, %d threads, median of 3 after a warm-up; cone pair timed on "
                             "36 of 720 views and scaled by 20; uneven batch: one slice of each size class, scaled by "
                             "the class's slice count (the oracle's DD adjoint is not instantiated in FP32, so "
-                            "ragged16_dd_adjoint has no CPU figure)"
+                            "ragged16_dd_adjoint has no CPU figure); helical BP: FP64 oracle on 2 of 128 slices, "
+                            "scaled by 64"
                             % threads)
 
 

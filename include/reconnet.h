@@ -283,6 +283,14 @@ void rn_cone_plan_destroy(void* plan);
 int rn_cone_forward(void* plan, int kind, const float* img, float* prj, int nBatches, int nChannels, void* stream);
 int rn_cone_backward(void* plan, int kind, const float* prj, float* img, int nBatches, int nChannels, void* stream);
 
+/* cHelicalBPFromParallelConebeamBackprojection (Recon3D/HelicalBP.cu:132) on
+ * device pointers and a caller stream: prj [B][nu][nview][nv][C] parallel-cone
+ * data, img [B][nx][ny][nz][C].  Its scratch (the projection transposes) is
+ * kept per host thread: calls from one thread on different streams must not
+ * overlap. */
+int rn_helical_bp(const float* prj, float* img, int nBatches, int nChannels, int nx, int ny, int nz,
+        float dx, float dy, float dz, int nu, int nview, int nv, float du, float dv, float off_u, float off_v,
+        float dsd, float dso, int nviewPerPI, float theta0, float volZ0, float zrot, int mPI, float Q, void* stream);
 /* Helical fan -> parallel-cone rebinning (no reference counterpart: the
  * reference's helical BP, Recon3D/HelicalBP.cu:10-83, consumes rebinned data).
  * fan: [B][nu_f][nview_f][nv_f][C], flat-detector views of a helical scan,
@@ -313,8 +321,8 @@ int rn_autotune_reset(void);
 /* Force candidate `candidate` of kernel `kernel` (1 Siddon FP, 2 Siddon BP,
  * 3 DD FP, 4 DD adjoint, 5 DD BP, 7 DD FBP BP; 8 / 9 Siddon FP / BP,
  * 10 / 11 DD BP / FBP BP and 12 DD FP on the interleaved 4-slice layout used
- * for 4 or more slices; 13 the CNN's 32 -> 32 convolution, 14 the one-launch CNN forward) for every
- * shape, bypassing the
+ * for 4 or more slices; 13 the CNN's 32 -> 32 convolution, 14 the one-launch CNN forward, 15 the
+ * helical backprojection) for every shape, bypassing the
  * timing; candidate < 0 removes the pin.  A launch of a kernel with fewer
  * candidates fails with an error.  Lets tests run every launch shape. */
 int rn_autotune_pin(int kernel, int candidate);

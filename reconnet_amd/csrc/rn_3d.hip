@@ -16,6 +16,7 @@
 
 #include "rn_3d.h"
 #include "rn_fan.h"
+#include "rn_tune.h"
 
 #include <cmath>
 #include <vector>
@@ -502,65 +503,126 @@ __global__ void k_cone_update(float* __restrict__ x, const float* __restrict__ b
 }
 
 // ---- helical ----
-
+//
+// Weighted helical backprojection of parallel-cone data
+// (bpHelicalParallelConebeam, Recon3D/HelicalBP.cu:10-83).  For a voxel at
+// height z the views of one pi-window, i = iStart(z) .. iStart(z) + nviewPerPI - 1,
+// each combine their conjugates k = i + m nviewPerPI (|m| <= mPI) as
+// sum_m w_k P_k / (sum_m w_k + 1e-6), with P_k the bilinear sample at
+//   u = x sin t_k - y cos t_k,
+//   v = (z - zs_k(u)) dsd / (sqrt(dso^2 - u^2) - (x cos t_k + y sin t_k)),
+//   zs_k(u) = zrot (k / (2 nviewPerPI) - asin(u / dso) / (2 pi)),
+// and w_k the row taper of q = |2 v / (dv nv)|.
+//
+// Everything about a view except v is independent of z: a lane owns a
+// column of ZC voxels (x, y fixed) and walks the union of their pi-windows;
+// for each view it evaluates t_k, u, the magnification dsd / (...), zs_k and
+// the column interpolation once, and then each of its voxels whose window
+// contains the view costs one FMA for v, the taper and a bilinear fetch.  A
+// wave takes an 8 x 8 block of columns (its fetches land in one detector
+// neighbourhood).  The sums run in the order above (i ascending, m
+// ascending), the window start in the reference's float arithmetic.
+// ---------------------------------------------------------------------------
 // prj_k: [slice][nview][nv][nu]
-__global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* __restrict__ prj, float* __restrict__ img,
-                                                    int nC)
+// per view k: (sin t_k, cos t_k, zrot k / (2 nviewPerPI)), t_k = theta0 + k PI / nviewPerPI, in FP64
+__global__ void k_helical_views(HelicalGeo h, float4* __restrict__ vt)
 {
-    const int iy = blockIdx.x * 16 + threadIdx.x;
-    const int ix = blockIdx.y * 16 + threadIdx.y;
-    const int bz = blockIdx.z;   // (b*nC + c)*nz + iz
-    const int iz = bz % h.nz, bc = bz / h.nz, c = bc % nC, b = bc / nC;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= h.nview) return;
+    const double PI = (double)3.141592657f;
+    const double t = (double)h.theta0 + k * PI / h.nviewPerPI;
+    vt[k] = make_float4((float)sin(t), (float)cos(t), (float)((double)h.zrot * k / (2.0 * h.nviewPerPI)), 0.0f);
+}
+
+template <int ZC>
+__global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* __restrict__ prj,
+                                                    const float4* __restrict__ vt, float* __restrict__ img, int nC,
+                                                    int nzc)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int iy = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
+    const int ix = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
+    const int zb = blockIdx.z % nzc, bc = blockIdx.z / nzc;
+    const int c = bc % nC, b = bc / nC;
     if (ix >= h.nx || iy >= h.ny) return;
-    const float PI = 3.141592657f;
+    const float PI = 3.141592657f;                      // the reference's PI (Projector.h)
     const float x = (ix - (h.nx - 1) / 2.0f) * h.dx, y = (iy - (h.ny - 1) / 2.0f) * h.dy;
-    const float z = iz * h.dz + h.volZ0;
-    const int iStart = (int)((z / h.zrot * 2 * PI - PI / 4) * h.nviewPerPI / PI);
-    const float* pb = prj + (long)(b * nC + c) * h.nview * h.nv * h.nu;
-    const int period = 2 * h.nviewPerPI;     // views per turn
-    const float dth = PI / h.nviewPerPI;
-    float imgVal = 0.f;
-    for (int i = iStart; i < iStart + h.nviewPerPI; ++i) {
-        float totalW = 0.f, val = 0.f;
-        for (int k = i - h.mPI * h.nviewPerPI; k <= i + h.mPI * h.nviewPerPI; k += h.nviewPerPI) {
-            if (k < 0 || k >= h.nview) continue;
-            // theta = theta0 + k pi / nviewPerPI, reduced by whole turns before sincos
-            const int kr = k % period;
-            float st, ct;
-            sincosf(h.theta0 + kr * dth, &st, &ct);
-            const float u = x * st - y * ct;
-            const float turnsz = h.zrot * ((float)k / (float)period);
-            const float v = (z - turnsz + h.zrot * asinf(u / h.dso) / (2 * PI)) * h.dsd /
-                            (sqrtf(h.dso * h.dso - u * u) - (x * ct + y * st));
-            const float q = fabsf(v / h.dv / h.nv * 2.0f);
-            float w;
-            if (q <= h.Q) w = 1.f;
-            else if (q < 1.f) { w = cosf((q - h.Q) / (1 - h.Q) * PI / 2); w *= w; }
-            else w = 0.f;
-            const float pu = u / h.du + (h.nu - 1.0f) / 2.0f + h.off_u;
-            const float pv = v / h.dv + (h.nv - 1.0f) / 2.0f + h.off_v;
-            // bilinear fetch with border 0 (texture at (pv+.5, k+.5, pu+.5))
-            const float fu = floorf(pu), fv = floorf(pv);
-            const int u0 = (int)fu, v0 = (int)fv;
-            const float au = pu - fu, av = pv - fv;
-            const float* pk = pb + (long)k * h.nv * h.nu;
-            float smp = 0.f;
-            for (int dv = 0; dv < 2; ++dv) {
-                const int vv = v0 + dv;
-                if (vv < 0 || vv >= h.nv) continue;
-                const float wv = dv ? av : 1.f - av;
-                for (int du = 0; du < 2; ++du) {
-                    const int uu = u0 + du;
-                    if (uu < 0 || uu >= h.nu) continue;
-                    smp += wv * (du ? au : 1.f - au) * pk[(long)vv * h.nu + uu];
-                }
-            }
-            val += smp * w;
-            totalW += w;
+    const int z0 = zb * ZC;
+    float zc[ZC];
+    int is[ZC];
+    int iLo = 0x7fffffff, iHi = -0x7fffffff;
+    #pragma unroll
+    for (int j = 0; j < ZC; ++j) {
+        zc[j] = (z0 + j) * h.dz + h.volZ0;
+        is[j] = (int)((zc[j] / h.zrot * 2 * PI - PI / 4) * h.nviewPerPI / PI);
+        if (z0 + j < h.nz) {
+            iLo = min(iLo, is[j]);
+            iHi = max(iHi, is[j] + h.nviewPerPI);
         }
-        imgVal += val / (totalW + 1e-6f);
     }
-    img[((((long)b * h.nx + ix) * h.ny + iy) * h.nz + iz) * nC + c] = imgVal;
+    // only views with a conjugate inside the data can contribute
+    iLo = max(iLo, -h.mPI * h.nviewPerPI);
+    iHi = min(iHi, h.nview + h.mPI * h.nviewPerPI);
+    const float* pb = prj + (long)(b * nC + c) * h.nview * h.nv * h.nu;
+    const float zr2pi = h.zrot / (2 * PI);
+    const float qs = 2.0f / (h.dv * h.nv);
+    const float cu = (h.nu - 1.0f) / 2.0f + h.off_u, cv = (h.nv - 1.0f) / 2.0f + h.off_v;
+    float acc[ZC];
+    #pragma unroll
+    for (int j = 0; j < ZC; ++j) acc[j] = 0.0f;
+    for (int i = iLo; i < iHi; ++i) {
+        float val[ZC], tw[ZC];
+        #pragma unroll
+        for (int j = 0; j < ZC; ++j) val[j] = tw[j] = 0.0f;
+        for (int m = -h.mPI; m <= h.mPI; ++m) {
+            const int k = i + m * h.nviewPerPI;
+            if (k < 0 || k >= h.nview) continue;
+            const float4 tv = vt[k];                          // k is wave-uniform: scalar loads
+            const float st = tv.x, ct = tv.y;
+            const float u = x * st - y * ct;
+            const float mag = h.dsd / (sqrtf(h.dso * h.dso - u * u) - (x * ct + y * st));
+            const float zs = tv.z - zr2pi * asinf(u / h.dso);
+            const float pu = u / h.du + cu;
+            const float fu = floorf(pu);
+            const int u0 = (int)fu;
+            const float au = pu - fu;
+            const float wu0 = (u0 >= 0 && u0 < h.nu) ? 1.0f - au : 0.0f;
+            const float wu1 = (u0 + 1 >= 0 && u0 + 1 < h.nu) ? au : 0.0f;
+            const int ua = min(max(u0, 0), h.nu - 1), ub = min(max(u0 + 1, 0), h.nu - 1);
+            const float* pk = pb + (long)k * h.nv * h.nu;
+            #pragma unroll
+            for (int j = 0; j < ZC; ++j) {
+                if (i < is[j] || i >= is[j] + h.nviewPerPI) continue;
+                const float v = (zc[j] - zs) * mag;
+                const float q = fabsf(v * qs);
+                float w;
+                if (q <= h.Q) w = 1.0f;
+                else if (q < 1.0f) { w = cosf((q - h.Q) / (1 - h.Q) * PI / 2); w *= w; }
+                else w = 0.0f;
+                const float pv = v / h.dv + cv;
+                const float fv = floorf(pv);
+                const int v0 = (int)fv;
+                const float av = pv - fv;
+                float smp = 0.0f;
+                if (v0 >= 0 && v0 < h.nv) {
+                    const float* r = pk + (long)v0 * h.nu;
+                    smp += (1.0f - av) * (wu0 * r[ua] + wu1 * r[ub]);
+                }
+                if (v0 + 1 >= 0 && v0 + 1 < h.nv) {
+                    const float* r = pk + (long)(v0 + 1) * h.nu;
+                    smp += av * (wu0 * r[ua] + wu1 * r[ub]);
+                }
+                val[j] += smp * w;
+                tw[j] += w;
+            }
+        }
+        #pragma unroll
+        for (int j = 0; j < ZC; ++j)
+            if (i >= is[j] && i < is[j] + h.nviewPerPI) acc[j] += val[j] / (tw[j] + 1e-6f);
+    }
+    #pragma unroll
+    for (int j = 0; j < ZC; ++j)
+        if (z0 + j < h.nz) img[((((long)b * h.nx + ix) * h.ny + iy) * h.nz + z0 + j) * nC + c] = acc[j];
 }
 
 Grid3K grid_of(const ArbGeo& g) { return Grid3K{g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.cx, g.cy, g.cz}; }
@@ -805,9 +867,13 @@ void helical_rebin(const RebinGeo& r, int nB, int nC, const float* fan, float* p
 void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st)
 {
     const size_t per = (size_t)h.nu * h.nview * h.nv;
-    work.ensure(2 * (size_t)nB * nC * per * sizeof(float));
+    const size_t vt_off = (2 * (size_t)nB * nC * per + 3) / 4 * 4;          // 16-byte aligned view table
+    work.ensure((vt_off + 4 * (size_t)h.nview) * sizeof(float));
     float* t1 = work.as<float>();
     float* t2 = t1 + (size_t)nB * nC * per;
+    float4* vt = reinterpret_cast<float4*>(t1 + vt_off);
+    hipLaunchKernelGGL(k_helical_views, dim3(cdiv(h.nview, 256)), dim3(256), 0, st, h, vt);
+    check_launch("helical_views");
     // [b][u][view][v][c] -> [(b, c)][u][view*v] -> [(b, c)][view][v][u]
     const float* base = prj;
     if (nC > 1) {
@@ -816,9 +882,21 @@ void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* im
         base = t2;
     }
     launch_transpose(base, t1, h.nu, h.nview * h.nv, nB * nC, st);
-    dim3 blk(16, 16), grd(cdiv(h.ny, 16), cdiv(h.nx, 16), nB * nC * h.nz);
-    hipLaunchKernelGGL(k_helical_bp, grd, blk, 0, st, h, t1, img, nC);
-    check_launch("helical_bp");
+    RN_REQUIRE((long)nB * nC * h.nz < 0x7fffffffL, "helical_bp: grid too large");
+    // candidates: columns of 4, 8 or 16 voxels per lane
+    auto launch = [&](int cand) {
+        auto go = [&](auto kern, int zc) {
+            const int nzc = cdiv(h.nz, zc);
+            const dim3 grd(cdiv(h.ny, 16), cdiv(h.nx, 16), nB * nC * nzc);
+            hipLaunchKernelGGL(kern, grd, dim3(256), 0, st, h, t1, vt, img, nC, nzc);
+        };
+        if (cand == 0) go(k_helical_bp<4>, 4);
+        else if (cand == 1) go(k_helical_bp<8>, 8);
+        else go(k_helical_bp<16>, 16);
+        check_launch("helical_bp");
+    };
+    const int c = tune_choose({TK_HELICAL_BP, h.nx * h.ny, h.nz, h.nu * h.nv, h.nview, nB * nC}, 3, launch, st, true);
+    launch(c);
 }
 
 } // namespace rn

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -480,6 +481,7 @@ extern "C" void cHelicalBPFromParallelConebeamBackprojection(float* pImg, const 
         check_bc(nBatches, nChannels);
         RN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nview > 0 && nv > 0, "dimensions must be positive");
         RN_REQUIRE(nviewPerPI > 0, "nviewPerPI must be positive");
+        RN_REQUIRE(zrot != 0.0f && std::isfinite(zrot) && mPI >= 0, "zrot must be non-zero and mPI >= 0");
         ThreadCtx& c = ctx();
         const size_t nimg = (size_t)nBatches * nx * ny * nz * nChannels, nprj = (size_t)nBatches * nu * nview * nv * nChannels;
         h2d(c.a, pPrj, nprj, c.st());
@@ -494,6 +496,22 @@ extern "C" void cHelicalBPFromParallelConebeamBackprojection(float* pImg, const 
 // =====================================================================
 // Additive API
 // =====================================================================
+
+extern "C" int rn_helical_bp(const float* prj, float* img, int nBatches, int nChannels, int nx, int ny, int nz,
+        float dx, float dy, float dz, int nu, int nview, int nv, float du, float dv, float off_u, float off_v,
+        float dsd, float dso, int nviewPerPI, float theta0, float volZ0, float zrot, int mPI, float Q, void* stream)
+{
+    return guarded_status("rn_helical_bp", [&] {
+        RN_REQUIRE(prj && img, "null pointer");
+        check_bc(nBatches, nChannels);
+        RN_REQUIRE(nx > 0 && ny > 0 && nz > 0 && nu > 0 && nview > 0 && nv > 0, "dimensions must be positive");
+        RN_REQUIRE(nviewPerPI > 0, "nviewPerPI must be positive");
+        RN_REQUIRE(zrot != 0.0f && std::isfinite(zrot) && mPI >= 0, "zrot must be non-zero and mPI >= 0");
+        HelicalGeo h{nx, ny, nz, dx, dy, dz, nu, nview, nv, du, dv, off_u, off_v, dsd, dso, nviewPerPI, theta0, volZ0,
+                     zrot, mPI, Q};
+        helical_bp(h, nBatches, nChannels, prj, img, ctx().g, static_cast<hipStream_t>(stream));
+    });
+}
 
 extern "C" int rn_helical_rebin(const float* fan, float* par, int nBatches, int nChannels,
         int nu_f, int nview_f, int nv_f, float du_f, float dv_f, float off_u_f, float off_v_f,

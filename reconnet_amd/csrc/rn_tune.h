@@ -43,6 +43,7 @@ enum TuneKernel {
     TK_DD_FP_Q = 12,
     TK_CONV_MID = 13,
     TK_CNN_FWD = 14,
+    TK_HELICAL_BP = 15,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

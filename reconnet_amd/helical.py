@@ -20,7 +20,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-__all__ = ["helical_scan_vectors", "rebin_fan_to_parallel"]
+__all__ = ["helical_scan_vectors", "rebin_fan_to_parallel", "helical_backproject"]
 
 
 def helical_scan_vectors(nview, dso, dsd, zrot, beta0=0.0, dbeta=None, theta0=0.0):
@@ -48,6 +48,9 @@ def _sigs():
         h.rn_helical_rebin.restype = ci
         h.rn_helical_rebin.argtypes = [P, P, ci, ci, ci, ci, ci, cf, cf, cf, cf, cf, cf, ci, ci, ci, cf, cf, cf, cf,
                                        cf, cf, cf, cf, P]
+        h.rn_helical_bp.restype = ci
+        h.rn_helical_bp.argtypes = [P, P, ci, ci, ci, ci, ci, cf, cf, cf, ci, ci, ci, cf, cf, cf, cf, cf, cf, ci, cf,
+                                    cf, cf, ci, cf, P]
         h._rebin_sigs = True
     return h
 
@@ -72,4 +75,25 @@ def rebin_fan_to_parallel(fan, *, du_f, dv_f, beta0, dbeta, nu_p, nview_p, nv_p,
     check(_sigs().rn_helical_rebin(_ptr(fan), _ptr(out), B, C, nu_f, nview_f, nv_f, du_f, dv_f, off_u_f, off_v_f,
                                    beta0, dbeta, nu_p, nview_p, nv_p, du_p, dv_p, off_u_p, off_v_p, theta0, dtheta,
                                    dsd, dso, _stream()), "rn_helical_rebin")
+    return out
+
+
+def helical_backproject(par, nview_per_pi, theta0, vol_z0, zrot, *, nx, ny, nz, dx, dy, dz, du, dv, dsd, dso,
+                        off_u=0.0, off_v=0.0, m_pi=1, Q=0.5, out=None):
+    """cHelicalBPFromParallelConebeamBackprojection on the GPU: par float32
+    CUDA tensor [B, nu, nview, nv, C] of parallel-cone data (normally ramp
+    filtered along u); returns the volume [B, nx, ny, nz, C]."""
+    import torch
+
+    from .torch_ops import _ptr, _stream
+
+    if par.dim() != 5 or par.dtype != torch.float32 or not par.is_cuda:
+        raise ValueError("par must be a float32 CUDA tensor [B, nu, nview, nv, C]")
+    par = par.contiguous()
+    B, nu, nview, nv, C = par.shape
+    if out is None:
+        out = torch.empty((B, nx, ny, nz, C), dtype=torch.float32, device=par.device)
+    check(_sigs().rn_helical_bp(_ptr(par), _ptr(out), B, C, nx, ny, nz, dx, dy, dz, nu, nview, nv, du, dv, off_u,
+                                off_v, dsd, dso, nview_per_pi, theta0, vol_z0, zrot, m_pi, Q, _stream()),
+          "rn_helical_bp")
     return out

@@ -491,19 +491,48 @@ def test_cone_ossqs_parity():
     assert rel_l2(out, ref) <= TOL
 
 
-def test_helical_bp_parity():
+# (mPI = 0 is left out: a lone view in the row taper has w ~ 1e-7 and adds
+# w P / (w + 1e-6), whose value follows the last bits of cos near pi / 2)
+@pytest.mark.parametrize("cand", [None, 0, 1, 2])
+@pytest.mark.parametrize("m_pi,Q,nC", [(1, 0.5, 1), (2, 0.3, 2), (1, 0.8, 1)])
+def test_helical_bp_parity(cand, m_pi, Q, nC):
+    from reconnet_amd._lib import autotune_pin
     from reconnet_amd.recon import ReconNet
 
     npp, turns, nu, nv = 48, 3, 80, 24
     r = ReconNet()
-    r.nx, r.ny, r.nz, r.dx, r.dy, r.dz = 32, 32, 12, 1.0, 1.0, 1.0
+    r.nx, r.ny, r.nz, r.dx, r.dy, r.dz = 32, 30, 13, 1.0, 1.0, 1.0
     r.du, r.dv, r.off_u, r.off_v, r.dsd, r.dso = 1.0, 1.0, 0.25, -0.5, 1000.0, 600.0
     rng = np.random.default_rng(9)
-    prj = rng.uniform(0, 1, (1, nu, 2 * npp * turns, nv, 1)).astype(np.float32)
-    out = r.cHelicalBP4d(prj, npp, 0.1, 8.0, 12.0)
+    prj = rng.uniform(0, 1, (1, nu, 2 * npp * turns, nv, nC)).astype(np.float32)
+    if cand is not None:
+        autotune_pin("helical_bp", cand)
+    try:
+        out = r.cHelicalBP4d(prj, npp, 0.1, 8.0, 12.0, mPI=m_pi, Q=Q)
+    finally:
+        autotune_pin("helical_bp", None)
     g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.0, dv=1.0, off_u=0.25, off_v=-0.5, dsd=1000.0, dso=600.0)
-    ref = orc.helical_bp(prj, g, (32, 32, 12), npp, 0.1, 8.0, 12.0)
+    ref = orc.helical_bp(prj, g, (32, 30, 13), npp, 0.1, 8.0, 12.0, m_pi=m_pi, Q=Q)
+    assert np.abs(ref).max() > 0.1
     assert rel_l2(out, ref) <= TOL
+
+
+def test_helical_bp_device_entry_equals_host_abi():
+    import torch
+
+    from reconnet_amd.helical import helical_backproject
+    from reconnet_amd.recon import ReconNet
+
+    npp, nu, nv = 48, 80, 24
+    r = ReconNet()
+    r.nx, r.ny, r.nz, r.dx, r.dy, r.dz = 32, 30, 13, 1.0, 1.0, 1.0
+    r.du, r.dv, r.off_u, r.off_v, r.dsd, r.dso = 1.0, 1.0, 0.25, -0.5, 1000.0, 600.0
+    prj = np.random.default_rng(19).uniform(0, 1, (2, nu, 2 * npp * 3, nv, 1)).astype(np.float32)
+    host = r.cHelicalBP4d(prj, npp, 0.1, 8.0, 12.0)
+    dev = helical_backproject(torch.from_numpy(prj).cuda(), npp, 0.1, 8.0, 12.0, nx=32, ny=30, nz=13, dx=1.0, dy=1.0,
+                              dz=1.0, du=1.0, dv=1.0, dsd=1000.0, dso=600.0, off_u=0.25, off_v=-0.5)
+    assert np.abs(host).max() > 0.1
+    assert np.array_equal(dev.cpu().numpy(), host)
 
 
 def test_helical_bp_full_size_slices_match_oracle():
