@@ -536,6 +536,9 @@ def config_gpu_timings(torch, dev, reps=5):
     nbc = 4.0 * (c["nx"] * c["ny"] * c["nz"] + c["nu"] * c["nview"] * c["nv"])
     rec("cone_dd_fp", gpu_ms(lambda: cp.forward(v, "dd")), nbc)
     rec("cone_dd_adjoint", gpu_ms(lambda: cp.backward(pr, "dd")), nbc)
+    prs = cp.forward(v, "siddon")
+    rec("cone_siddon_fp", gpu_ms(lambda: cp.forward(v, "siddon")), nbc)
+    rec("cone_siddon_bp", gpu_ms(lambda: cp.backward(prs, "siddon")), nbc)
     del cp
 
     # uneven batch: 6 x 256^2/360, 6 x 384^2/720, 4 x 512^2/1152

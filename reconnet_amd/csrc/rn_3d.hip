@@ -146,21 +146,161 @@ __global__ __launch_bounds__(256) void k_siddon_arb_fp(ArbGeo a, Grid3K g, int p
     prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c] = val;
 }
 
+// Siddon backprojection as a gather, the transpose of k_siddon_arb_fp: a
+// voxel collects, per view, the rays of the detector cells that lie in the
+// projection of its 8 corners, each weighted by the ray's chord through the
+// voxel.  The ray is the walk's: from its start point (the source; for
+// parallel views the detector point moved back by m along invRayDir) through
+// the grid and on past the detector point, and it is used only if it enters
+// the grid before the detector point (the walk's amin < amax with amax =
+// min(1, exit)); moving the start point to the grid (MoveSourceDstNearGrid)
+// keeps it outside the grid, so it changes no chord.  One wave holds a
+// column of 64 voxels along z with (x, y) wave-uniform; no atomics, every
+// voxel written once.
+
+// [enter, exit] of the ray o + t d in the box [lo, hi] (empty: enter > exit);
+// an axis with d = 0 keeps the interval if lo <= o < hi (the walk's voxel
+// index of a stationary coordinate)
+__device__ __forceinline__ void slab(const F3& o, const F3& d, const F3& lo, const F3& hi, float& t0, float& t1)
+{
+    t0 = -INFINITY;
+    t1 = INFINITY;
+    const float oc[3] = {o.x, o.y, o.z}, dc[3] = {d.x, d.y, d.z};
+    const float lc[3] = {lo.x, lo.y, lo.z}, hc[3] = {hi.x, hi.y, hi.z};
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (dc[k] != 0.0f) {
+            const float inv = __builtin_amdgcn_rcpf(dc[k]);
+            const float a0 = (lc[k] - oc[k]) * inv, a1 = (hc[k] - oc[k]) * inv;
+            t0 = fmaxf(t0, fminf(a0, a1));
+            t1 = fminf(t1, fmaxf(a0, a1));
+        } else if (!(oc[k] >= lc[k] && oc[k] < hc[k])) {
+            t1 = -INFINITY;
+        }
+    }
+}
+
+// Per view, in FP64: the detector normal N = U x V, the dual basis (Ud, Vd) of
+// (U, V) and the projection constants: cone A = (C - S).N, su, sv = (S - C).Ud,
+// (S - C).Vd; parallel sdn, sdu, sdv = invRayDir.(N, Ud, Vd) and the start
+// distance m of GetSrcDstForParallel (SiddonParallel.cu:11-32).
+__global__ void k_siddon_arb_views(ArbGeo a, Grid3K g, int parallel, const float* C, const float* U, const float* V,
+                                   const float* S, float4* __restrict__ vt)
+{
+    const int view = blockIdx.x * blockDim.x + threadIdx.x;
+    if (view >= a.nview) return;
+    double c[3], u[3], v[3], sd[3];
+    for (int k = 0; k < 3; ++k) {
+        c[k] = C[3 * view + k]; u[k] = U[3 * view + k]; v[k] = V[3 * view + k]; sd[k] = S[3 * view + k];
+    }
+    const double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2], vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    const double idet = 1.0 / (uu * vv - uv * uv);
+    double ud[3], vd[3];
+    for (int k = 0; k < 3; ++k) { ud[k] = (vv * u[k] - uv * v[k]) * idet; vd[k] = (uu * v[k] - uv * u[k]) * idet; }
+    auto dot = [](const double* p, const double* q) { return p[0] * q[0] + p[1] * q[1] + p[2] * q[2]; };
+    double w, wu, wv, m = 0.0;
+    if (!parallel) {
+        const double cs[3] = {c[0] - sd[0], c[1] - sd[1], c[2] - sd[2]};
+        w = dot(cs, n); wu = -dot(cs, ud); wv = -dot(cs, vd);
+    } else {
+        w = dot(sd, n); wu = dot(sd, ud); wv = dot(sd, vd);
+        // the walk's start distance, in its float arithmetic
+        const float fx = g.cx - C[3 * view], fy = g.cy - C[3 * view + 1], fz = g.cz - C[3 * view + 2];
+        const float ex = g.nx * g.dx, ey = g.ny * g.dy, ez = g.nz * g.dz;
+        m = fmaxf(sqrtf(fx * fx + fy * fy + fz * fz), sqrtf(ex * ex + ey * ey + ez * ez) / 2) * 1.1f;
+    }
+    vt[4 * view + 0] = make_float4((float)n[0], (float)n[1], (float)n[2], (float)w);
+    vt[4 * view + 1] = make_float4((float)ud[0], (float)ud[1], (float)ud[2], (float)wu);
+    vt[4 * view + 2] = make_float4((float)vd[0], (float)vd[1], (float)vd[2], (float)wv);
+    vt[4 * view + 3] = make_float4((float)m, 0.0f, 0.0f, 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ prj,
                                                        float* __restrict__ img, const float* C, const float* U,
-                                                       const float* V, const float* S, int nC)
+                                                       const float* V, const float* S, const float4* __restrict__ vt,
+                                                       int nC)
 {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int view = blockIdx.y;
-    const int bc = blockIdx.z;
-    const int c = bc % nC, biv = bc / nC, iv = biv % a.nv, b = biv / a.nv;
-    if (iu >= a.nu) return;
-    const float p = prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c];
-    if (p == 0.f) return;
-    F3 s, t;
-    arb_ray(a, g, parallel != 0, C, U, V, S, view, iu, iv, s, t);
-    float* base = img + (long)b * g.nx * g.ny * g.nz * nC + c;
-    siddon_walk(s, t, g, [&](long vox, float len) { unsafeAtomicAdd(base + vox * nC, p * len); });
+    const int iz = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iy = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (int)(threadIdx.x >> 6));
+    const int ix = blockIdx.z % g.nx, bc = blockIdx.z / g.nx;
+    if (iz >= g.nz || iy >= g.ny) return;
+    const int cc = bc % nC, b = bc / nC;
+    const float* q = prj + (long)b * a.nu * a.nview * a.nv * nC + cc;
+    const F3 vlo{g.cx + (ix - g.nx / 2.0f) * g.dx, g.cy + (iy - g.ny / 2.0f) * g.dy, g.cz + (iz - g.nz / 2.0f) * g.dz};
+    const F3 vhi{vlo.x + g.dx, vlo.y + g.dy, vlo.z + g.dz};
+    const F3 glo{g.cx - g.nx / 2.0f * g.dx, g.cy - g.ny / 2.0f * g.dy, g.cz - g.nz / 2.0f * g.dz};
+    const F3 ghi{g.cx + g.nx / 2.0f * g.dx, g.cy + g.ny / 2.0f * g.dy, g.cz + g.nz / 2.0f * g.dz};
+    const float cu = (a.nu - 1) / 2.0f + a.off_u, cv = (a.nv - 1) / 2.0f + a.off_v;
+    auto dot = [](const F3& p, const F3& r) { return p.x * r.x + p.y * r.y + p.z * r.z; };
+    float acc = 0.f;
+    for (int view = 0; view < a.nview; ++view) {
+        const F3 Cv = ld3(C, view), Uv = ld3(U, view), Vv = ld3(V, view), Sv = ld3(S, view);
+        const float4 t0v = vt[4 * view], t1v = vt[4 * view + 1], t2v = vt[4 * view + 2];
+        const F3 N{t0v.x, t0v.y, t0v.z}, Ud{t1v.x, t1v.y, t1v.z}, Vd{t2v.x, t2v.y, t2v.z};
+        // the voxel's corners on the detector (u, v in mm)
+        float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+        bool ok = true;
+        float m = 0.f;
+        if (!parallel) {
+            const float A = t0v.w, su = t1v.w, sv = t2v.w;
+            #pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const F3 w{((k & 1) ? vhi.x : vlo.x) - Sv.x, ((k & 2) ? vhi.y : vlo.y) - Sv.y,
+                           ((k & 4) ? vhi.z : vlo.z) - Sv.z};
+                const float den = dot(w, N);
+                ok = ok && (den * A > 0.f);
+                const float t = A * __builtin_amdgcn_rcpf(den);
+                const float u = fmaf(t, dot(w, Ud), su), v = fmaf(t, dot(w, Vd), sv);
+                umin = fminf(umin, u); umax = fmaxf(umax, u);
+                vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+            }
+        } else {
+            const float sdn = t0v.w, sdu = t1v.w, sdv = t2v.w;
+            ok = sdn != 0.f;
+            const float isdn = -__builtin_amdgcn_rcpf(sdn);
+            #pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const F3 w{((k & 1) ? vhi.x : vlo.x) - Cv.x, ((k & 2) ? vhi.y : vlo.y) - Cv.y,
+                           ((k & 4) ? vhi.z : vlo.z) - Cv.z};
+                const float tau = dot(w, N) * isdn;
+                const float u = fmaf(tau, sdu, dot(w, Ud)), v = fmaf(tau, sdv, dot(w, Vd));
+                umin = fminf(umin, u); umax = fmaxf(umax, u);
+                vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+            }
+            m = vt[4 * view + 3].x;
+        }
+        if (!ok) continue;
+        // cells whose centre ray meets the corners' bounding box
+        const float fu0 = umin / a.du + cu, fu1 = umax / a.du + cu, fv0 = vmin / a.dv + cv, fv1 = vmax / a.dv + cv;
+        const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0), iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
+        const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0), iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
+        for (int iu = iu0; iu <= iu1; ++iu) {
+            const float uc = (iu - cu) * a.du;
+            const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
+            for (int iv = iv0; iv <= iv1; ++iv) {
+                const float vc = (iv - cv) * a.dv;
+                const F3 dst{Cv.x + Uv.x * uc + Vv.x * vc, Cv.y + Uv.y * uc + Vv.y * vc, Cv.z + Uv.z * uc + Vv.z * vc};
+                const F3 o = parallel ? F3{dst.x + Sv.x * m, dst.y + Sv.y * m, dst.z + Sv.z * m} : Sv;
+                const F3 d{dst.x - o.x, dst.y - o.y, dst.z - o.z};
+                float t0, t1;
+                slab(o, d, vlo, vhi, t0, t1);
+                t0 = fmaxf(t0, 0.f);
+                if (!(t1 > t0)) continue;
+                if (t0 >= 1.f) {
+                    // a voxel beyond the detector point: the ray counts only if it
+                    // entered the grid before that point
+                    float g0, g1;
+                    slab(o, d, glo, ghi, g0, g1);
+                    if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
+                }
+                const float len = (t1 - t0) * sqrtf(dot(d, d));
+                acc = fmaf(len, qc[(long)iv * nC], acc);
+            }
+        }
+    }
+    img[((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc] = acc;
 }
 
 // ---- DD tomosynthesis ----
@@ -325,14 +465,31 @@ __device__ __forceinline__ ConeRay cone_ray(const ArbGeo& a, const Grid3K& g, co
 }
 
 // vol, prj: reference layouts [b][nx][ny][nz][c], [b][nu][nview][nv][c]
+// XCD > 0: 1-D grid (padded to a multiple of XCD) renumbered so that each XCD
+// walks a contiguous range of (row block, column, view): the adjacent detector
+// columns of a view, whose footprints share plane cells, meet in one L2.
+template <int XCD>
 __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const float* __restrict__ vol,
                                                    float* __restrict__ prj, const float* Cp, const float* Up,
                                                    const float* Vp, const float* Sp, const int* __restrict__ cls,
-                                                   int nC)
+                                                   int nC, int nblocks)
 {
-    const int iv = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iu = blockIdx.y;
-    const int view = blockIdx.z % a.nview, bc = blockIdx.z / a.nview;
+    int bx, iu, bz;
+    if (XCD > 0) {
+        const int per = gridDim.x / XCD;
+        const int l = (blockIdx.x % XCD) * per + blockIdx.x / XCD;
+        if (l >= nblocks) return;
+        const int nvb = (a.nv + 63) / 64;
+        bx = l % nvb;
+        iu = (l / nvb) % a.nu;
+        bz = l / (nvb * a.nu);
+    } else {
+        bx = blockIdx.x;
+        iu = blockIdx.y;
+        bz = blockIdx.z;
+    }
+    const int iv = bx * 64 + threadIdx.x;
+    const int view = bz % a.nview, bc = bz / a.nview;
     if (iv >= a.nv) return;
     const bool xv = cls[view] == 0;
     const ConeRay r = cone_ray(a, g, ld3(Cp, view), ld3(Up, view), ld3(Vp, view), ld3(Sp, view), xv, iu, iv);
@@ -645,13 +802,16 @@ void siddon_arb_bp(const ArbGeo& g, bool parallel, int nB, int nC, const float* 
                    const float* detCenter, const float* detU, const float* detV, const float* srcOrDir,
                    DevBuf& work, hipStream_t st)
 {
-    (void)work;
     RN_REQUIRE(detCenter && detU && detV && srcOrDir, "null geometry array");
-    const size_t nimg = (size_t)nB * g.nx * g.ny * g.nz * nC;
-    RN_HIP_CHECK(hipMemsetAsync(img, 0, nimg * sizeof(float), st));
-    dim3 blk(256), grd(cdiv(g.nu, 256), g.nview, nB * g.nv * nC);
+    RN_REQUIRE((long)g.nx * nB * nC < 0x7fffffffL, "siddon_arb_bp: grid too large");
+    work.ensure((size_t)g.nview * 4 * sizeof(float4));
+    float4* vt = work.as<float4>();
+    hipLaunchKernelGGL(k_siddon_arb_views, dim3(cdiv(g.nview, 256)), dim3(256), 0, st, g, grid_of(g),
+                       parallel ? 1 : 0, detCenter, detU, detV, srcOrDir, vt);
+    check_launch("siddon_arb_views");
+    dim3 blk(256), grd(cdiv(g.nz, 64), cdiv(g.ny, 4), g.nx * nB * nC);
     hipLaunchKernelGGL(k_siddon_arb_bp, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, prj, img, detCenter, detU,
-                       detV, srcOrDir, nC);
+                       detV, srcOrDir, vt, nC);
     check_launch("siddon_arb_bp");
 }
 
@@ -727,10 +887,22 @@ void ConePlan::dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t s
     const ArbGeo& g = geo;
     RN_REQUIRE(nB > 0 && nC > 0 && (long)g.nview * nB * nC <= 65535, "cone batch too large for one launch");
     const float* v = vec_.as<float>();
-    dim3 blk(64), grd(cdiv(g.nv, 64), g.nu, g.nview * nB * nC);
-    hipLaunchKernelGGL(k_dd_cone_fp, grd, blk, 0, st, g, grid_of(g), img, prj, v, v + 3 * g.nview, v + 6 * g.nview,
-                       v + 9 * g.nview, cls_.as<int>(), nC);
-    check_launch("dd_cone_fp");
+    const long nblocks = (long)cdiv(g.nv, 64) * g.nu * g.nview * nB * nC;
+    RN_REQUIRE(nblocks + 8 < 0x7fffffffL, "dd_cone_fp: grid too large");
+    auto launch = [&](int c) {
+        if (c == 0) {
+            const long n8 = (nblocks + 7) / 8 * 8;
+            hipLaunchKernelGGL(k_dd_cone_fp<8>, dim3((unsigned)n8), dim3(64), 0, st, g, grid_of(g), img, prj, v,
+                               v + 3 * g.nview, v + 6 * g.nview, v + 9 * g.nview, cls_.as<int>(), nC, (int)nblocks);
+        } else {
+            dim3 grd(cdiv(g.nv, 64), g.nu, g.nview * nB * nC);
+            hipLaunchKernelGGL(k_dd_cone_fp<0>, grd, dim3(64), 0, st, g, grid_of(g), img, prj, v, v + 3 * g.nview,
+                               v + 6 * g.nview, v + 9 * g.nview, cls_.as<int>(), nC, (int)nblocks);
+        }
+        check_launch("dd_cone_fp");
+    };
+    const int c = tune_choose({TK_DD_CONE_FP, g.nx * g.ny, g.nz, g.nu * g.nv, g.nview, nB * nC}, 2, launch, st, true);
+    launch(c);
 }
 
 void ConePlan::dd_adj(int nB, int nC, const float* prj, float* img, hipStream_t st) const

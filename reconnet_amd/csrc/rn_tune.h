@@ -44,6 +44,7 @@ enum TuneKernel {
     TK_CONV_MID = 13,
     TK_CNN_FWD = 14,
     TK_HELICAL_BP = 15,
+    TK_DD_CONE_FP = 16,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

@@ -5,8 +5,9 @@ Batch: 16 slices, six 256^2 / 360 views, six 384^2 / 720 views, four 512^2 /
 1152 views (736 bins).  Loss 0.5 ||Ax - y||^2 in FP32.  Checks: per-slice
 results of the persistent launch equal independent single-geometry launches
 bit for bit; the gradient equals A^T(Ax - y) from the FP64 oracle (Siddon,
-where the oracle holds the exact transpose); central finite differences
-(h = 1e-3) on 64 random pixels.
+where the oracle holds the exact transpose; DD through <g, z> = <r, A z> with
+the oracle's forward operator); central finite differences (h = 1e-3) on 64
+random pixels, for both.
 """
 
 import numpy as np
@@ -92,6 +93,48 @@ def test_gradient_matches_oracle_and_finite_differences(batch):
             fd.append(float((lp - lm) / (2 * h)))
     fd = np.array(fd)
     ga = g.cpu().numpy()[idx].astype(np.float64)
+    assert rel_l2(ga, fd) <= 1e-2
+
+
+def test_dd_gradient_through_uneven_batch_matches_oracle_and_finite_differences(batch):
+    # DD: the gradient A^T (A x - y) of the persistent launch, against the FP64
+    # oracle's DD projector through <g, z> = <r, A z> for random z (the adjoint
+    # must be the transpose of the oracle's forward operator), and central
+    # finite differences of the loss on 64 random pixels
+    from reconnet_amd.ragged import RaggedProjection
+
+    geos, rb, imgs = batch
+    rng = np.random.default_rng(43)
+    x0 = rb.flatten_images(imgs)
+    meas = rb.forward(x0, "dd") * 1.01 + 0.01
+    x = (x0 + 0.002 * torch.from_numpy(rng.normal(size=x0.numel()).astype(np.float32)).cuda()).requires_grad_(True)
+    r = RaggedProjection.apply(x, rb, "dd") - meas
+    loss = 0.5 * (r * r).sum()
+    loss.backward()
+    g = x.grad.detach()
+    rs = rb.split_sinos(r.detach())
+    gs = rb.split_images(g)
+    for s in (0, 7, 12):
+        geo = geos[s]
+        z = rng.uniform(-1, 1, (geo.nx, geo.ny)).astype(np.float32)
+        az = orc.dd_fan_fp(z[None, :, :, None, None], geo.angles(), geo.oracle_geo(), geo.nu)[0, :, :, 0, 0]
+        lhs = float((gs[s].cpu().numpy().astype(np.float64) * z).sum())
+        rhs = float((rs[s].cpu().numpy().astype(np.float64) * az).sum())
+        assert abs(lhs - rhs) <= 1e-4 * np.sqrt((gs[s].double() ** 2).sum().item() * (z.astype(np.float64) ** 2).sum())
+    idx = rng.choice(x0.numel(), size=64, replace=False)
+    h = 1e-3
+    with torch.no_grad():
+        base = x.detach().clone()
+        fd = []
+        for i in idx:
+            xp = base.clone(); xp[i] += h
+            xm = base.clone(); xm[i] -= h
+            lp = 0.5 * ((rb.forward(xp, "dd") - meas).double() ** 2).sum()
+            lm = 0.5 * ((rb.forward(xm, "dd") - meas).double() ** 2).sum()
+            fd.append(float((lp - lm) / (2 * h)))
+    fd = np.array(fd)
+    ga = g.cpu().numpy()[idx].astype(np.float64)
+    assert np.abs(ga).max() > 1e-3
     assert rel_l2(ga, fd) <= 1e-2
 
 

@@ -178,20 +178,29 @@ def test_fused_update_matches_separate_update(beta):
         eng.close()
 
 
-@pytest.mark.parametrize("ns,rect", [(4, False), (7, False), (6, True)])
-def test_interleaved_engine_matches_plain_layout(ns, rect, monkeypatch):
+@pytest.mark.parametrize("ns,rect,pin", [(4, False, None), (7, False, None), (6, True, None), (6, True, 0)])
+def test_interleaved_engine_matches_plain_layout(ns, rect, pin, monkeypatch):
     # the same passes with the engine's plain one-slice layout
     # (RECONNET_SQS_INTERLEAVE=0) and with the interleaved 4-slice layout,
-    # also on a rectangular grid whose two orientations have different pitches
+    # also on a rectangular grid whose two orientations have different pitches;
+    # pin = 0 forces the staged backprojection shape, whose epilogue carries the
+    # update, with a partly filled 4-slice group (6 slices) on nx != ny
+    from reconnet_amd._lib import autotune_pin
+
     outs = []
-    for flag in ("0", "1"):
-        monkeypatch.setenv("RECONNET_SQS_INTERLEAVE", flag)
-        eng, x0 = _engine(ns, rect=rect)
-        eng.set_image(x0)
-        for _ in range(2):
-            eng.run_pass(beta=1e-3, eps=1e-4, lb=0.0)
-        outs.append(eng.image())
-        eng.close()
+    try:
+        if pin is not None:
+            autotune_pin("siddon_bp_q", pin)
+        for flag in ("0", "1"):
+            monkeypatch.setenv("RECONNET_SQS_INTERLEAVE", flag)
+            eng, x0 = _engine(ns, rect=rect)
+            eng.set_image(x0)
+            for _ in range(2):
+                eng.run_pass(beta=1e-3, eps=1e-4, lb=0.0)
+            outs.append(eng.image())
+            eng.close()
+    finally:
+        autotune_pin("siddon_bp_q", None)
     err = float((outs[0] - outs[1]).abs().max() / outs[0].abs().max())
     assert err <= 1e-6, err
 
