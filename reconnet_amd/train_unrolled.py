@@ -1,7 +1,6 @@
 """Train the learned unrolled OS-SQS network on one GPU.
 
-    python -m reconnet_amd.train_unrolled --out reconnet_amd/weights/unrolled_fan512.npz \
-        --log reconnet_amd/weights/train_log_fan512.json
+    python -m reconnet_amd.train_unrolled --train 2000 --out reconnet_amd/weights/unrolled_fan512.npz
 
 Data: seeded random-ellipse phantoms (seeds 0 .. N-1) on the 512^2 fan
 geometry (0.7 mm pixels, 1152 views, 736 bins), projected by the library's
@@ -64,12 +63,12 @@ def masked_rmse_hu(x, truth, mask):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=3000)
+    ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--val", type=int, default=32)
-    ap.add_argument("--iters", type=int, default=8)
+    ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--subsets", type=int, default=12)
     ap.add_argument("--i0", type=float, default=2.5e4)
-    ap.add_argument("--epochs", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=16)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--scale", type=float, default=50.0)

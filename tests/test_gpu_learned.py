@@ -1,6 +1,6 @@
 """What the trained prior gains: on held-out slices (seeds 100000+, the dose
-the weights were trained for), the learned reconstruction (one stage per
-trained network) against the same number of OS-SQS data passes without the CNN, and against plain SQS-TV
+the weights were trained for), the 10-iteration learned reconstruction against
+the same 10 OS-SQS data passes without the CNN, and against plain SQS-TV
 (beta = 1e-4, delta = 1e-3) after the same passes, all from the FBP image."""
 
 import os
