@@ -129,15 +129,19 @@ __device__ void siddon_walk(F3 src, F3 dst, const Grid3K& g, F f)
     }
 }
 
+// One ray per lane; a wave holds 64 adjacent detector rows of one column, so
+// its rays share their in-plane path when the rows run along z and read
+// adjacent z voxels (z is the innermost image index), and its outputs are
+// adjacent; 4 columns per workgroup.
 __global__ __launch_bounds__(256) void k_siddon_arb_fp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ img,
                                                        float* __restrict__ prj, const float* C, const float* U,
                                                        const float* V, const float* S, int nC)
 {
-    const int iu = blockIdx.x * blockDim.x + threadIdx.x;
-    const int view = blockIdx.y;
-    const int bc = blockIdx.z;   // b * nv * nC + iv * nC + c  (iv folded in)
-    const int c = bc % nC, biv = bc / nC, iv = biv % a.nv, b = biv / a.nv;
-    if (iu >= a.nu) return;
+    const int iv = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int iu = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int view = blockIdx.z % a.nview, bc = blockIdx.z / a.nview;
+    const int c = bc % nC, b = bc / nC;
+    if (iu >= a.nu || iv >= a.nv) return;
     F3 s, t;
     arb_ray(a, g, parallel != 0, C, U, V, S, view, iu, iv, s, t);
     const float* base = img + (long)b * g.nx * g.ny * g.nz * nC + c;
@@ -276,7 +280,54 @@ __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int p
         const float fu0 = umin / a.du + cu, fu1 = umax / a.du + cu, fv0 = vmin / a.dv + cv, fv1 = vmax / a.dv + cv;
         const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0), iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
         const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0), iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
-        for (int iu = iu0; iu <= iu1; ++iu) {
+        // cone views whose detector rows run along z and columns lie in a
+        // horizontal plane: a column's rays share their in-plane direction, so
+        // the x and y slabs are computed once per column and each row adds its
+        // z slab (the same numbers as the general path below)
+        const bool vz = !parallel && Vv.x == 0.f && Vv.y == 0.f && Uv.z == 0.f;
+        for (int iu = iu0; iu <= iu1 && vz; ++iu) {
+            const float uc = (iu - cu) * a.du;
+            const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
+            const float dx = fmaf(Uv.x, uc, Cv.x) - Sv.x, dy = fmaf(Uv.y, uc, Cv.y) - Sv.y;
+            float xy0 = -INFINITY, xy1 = INFINITY;
+            if (dx != 0.f) {
+                const float inv = __builtin_amdgcn_rcpf(dx);
+                const float a0 = (vlo.x - Sv.x) * inv, a1 = (vhi.x - Sv.x) * inv;
+                xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
+            } else if (!(Sv.x >= vlo.x && Sv.x < vhi.x)) {
+                xy1 = -INFINITY;
+            }
+            if (dy != 0.f) {
+                const float inv = __builtin_amdgcn_rcpf(dy);
+                const float a0 = (vlo.y - Sv.y) * inv, a1 = (vhi.y - Sv.y) * inv;
+                xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
+            } else if (!(Sv.y >= vlo.y && Sv.y < vhi.y)) {
+                xy1 = -INFINITY;
+            }
+            if (!(xy1 > fmaxf(xy0, 0.f))) continue;
+            const float dxy2 = fmaf(dx, dx, dy * dy);
+            for (int iv = iv0; iv <= iv1; ++iv) {
+                const float vc = (iv - cv) * a.dv;
+                const float dz = fmaf(Vv.z, vc, Cv.z) - Sv.z;
+                float t0 = xy0, t1 = xy1;
+                if (dz != 0.f) {
+                    const float inv = __builtin_amdgcn_rcpf(dz);
+                    const float a0 = (vlo.z - Sv.z) * inv, a1 = (vhi.z - Sv.z) * inv;
+                    t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
+                } else if (!(Sv.z >= vlo.z && Sv.z < vhi.z)) {
+                    continue;
+                }
+                t0 = fmaxf(t0, 0.f);
+                if (!(t1 > t0)) continue;
+                if (t0 >= 1.f) {
+                    float g0, g1;
+                    slab(Sv, F3{dx, dy, dz}, glo, ghi, g0, g1);
+                    if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
+                }
+                acc = fmaf((t1 - t0) * sqrtf(fmaf(dz, dz, dxy2)), qc[(long)iv * nC], acc);
+            }
+        }
+        for (int iu = iu0; iu <= iu1 && !vz; ++iu) {
             const float uc = (iu - cu) * a.du;
             const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
             for (int iv = iv0; iv <= iv1; ++iv) {
@@ -792,7 +843,8 @@ void siddon_arb_fp(const ArbGeo& g, bool parallel, int nB, int nC, const float* 
 {
     (void)work;
     RN_REQUIRE(detCenter && detU && detV && srcOrDir, "null geometry array");
-    dim3 blk(256), grd(cdiv(g.nu, 256), g.nview, nB * g.nv * nC);
+    RN_REQUIRE((long)g.nview * nB * nC <= 65535 && cdiv(g.nu, 4) <= 65535, "siddon_arb_fp: grid too large");
+    dim3 blk(256), grd(cdiv(g.nv, 64), cdiv(g.nu, 4), g.nview * nB * nC);
     hipLaunchKernelGGL(k_siddon_arb_fp, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, img, prj, detCenter, detU,
                        detV, srcOrDir, nC);
     check_launch("siddon_arb_fp");
@@ -803,7 +855,7 @@ void siddon_arb_bp(const ArbGeo& g, bool parallel, int nB, int nC, const float* 
                    DevBuf& work, hipStream_t st)
 {
     RN_REQUIRE(detCenter && detU && detV && srcOrDir, "null geometry array");
-    RN_REQUIRE((long)g.nx * nB * nC < 0x7fffffffL, "siddon_arb_bp: grid too large");
+    RN_REQUIRE((long)g.nx * nB * nC <= 65535 && cdiv(g.ny, 4) <= 65535, "siddon_arb_bp: grid too large");
     work.ensure((size_t)g.nview * 4 * sizeof(float4));
     float4* vt = work.as<float4>();
     hipLaunchKernelGGL(k_siddon_arb_views, dim3(cdiv(g.nview, 256)), dim3(256), 0, st, g, grid_of(g),
@@ -1054,11 +1106,11 @@ void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* im
         base = t2;
     }
     launch_transpose(base, t1, h.nu, h.nview * h.nv, nB * nC, st);
-    RN_REQUIRE((long)nB * nC * h.nz < 0x7fffffffL, "helical_bp: grid too large");
     // candidates: columns of 4, 8 or 16 voxels per lane
     auto launch = [&](int cand) {
         auto go = [&](auto kern, int zc) {
             const int nzc = cdiv(h.nz, zc);
+            RN_REQUIRE((long)nB * nC * nzc <= 65535, "helical_bp: batch too large for one launch");
             const dim3 grd(cdiv(h.ny, 16), cdiv(h.nx, 16), nB * nC * nzc);
             hipLaunchKernelGGL(kern, grd, dim3(256), 0, st, h, t1, vt, img, nC, nzc);
         };
