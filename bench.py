@@ -525,6 +525,8 @@ def config_gpu_timings(torch, dev, reps=5):
     yb = p.forward(xb, "dd")
     rec("fan256x32_dd_fp", gpu_ms(lambda: p.forward(xb, "dd")), 32 * nb)
     rec("fan256x32_dd_bp", gpu_ms(lambda: p.backward(yb, "dd")), 32 * nb)
+    rec("fan256x32_pixel_bp", gpu_ms(lambda: p.backward(yb, "pixel")), 32 * nb)
+    rec("fan256x32_dd_adjoint", gpu_ms(lambda: p.backward(yb, "dd_adjoint")), 32 * nb)
     del p
 
     # cone 256^2 x 128, 720 views, 64 x 512
@@ -598,6 +600,7 @@ def cpu_config_timings(configs, threads):
     }
     t["fan256x32_dd_fp"] = 32 * t["fan256_dd_fp"]
     t["fan256x32_dd_bp"] = 32 * t["fan256_dd_bp"]
+    t["fan256x32_pixel_bp"] = 32 * t["fan256_pixel_bp"]
     c = CONE_WORKLOAD
     sub = 20
     vecs = [a[::sub].copy() for a in circular_cone_vectors(c["nview"], c["dso"], c["dsd"])]

@@ -679,17 +679,26 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_dd_fan_bp_ql(FanK g
 }
 
 // bpFanKernel3D with isFBP = true (the only mode the reference launches).
-template <int BX, int BY>
+// Pixel-driven FBP backprojection (PixelDriven.cu:11-75): one pixel of S
+// slices per lane; the view's detector coordinate (an atan), the linear
+// interpolation weights and the FBP weight dso^2 / (rx^2 + (dso + ry)^2) are
+// computed once and applied to the S slices, each with the arithmetic of a
+// one-slice call (a tail group reads its last slice again and discards it).
+template <int BX, int BY, int S>
 __global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* __restrict__ prj,
-                                                      float* __restrict__ img)
+                                                      float* __restrict__ img, int ns)
 {
     const int iy = blockIdx.x * BX + threadIdx.x;
     const int ix = blockIdx.y * BY + threadIdx.y;
-    const int s = blockIdx.z;
+    const int s0 = blockIdx.z * S;
     if (ix >= g.nx || iy >= g.ny) return;
+    const int nsl = min(S, ns - s0);
     const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
     const float inv_da = 1.0f / g.da;
-    float acc = 0.0f;
+    const long sstride = (long)g.nview * g.nu;
+    float acc[S];
+    #pragma unroll
+    for (int q = 0; q < S; ++q) acc[q] = 0.0f;
     for (int view = 0; view < g.nview; ++view) {
         const float ct = g.vcos[view], st = g.vsin[view];
         const float rx = x * ct + y * st, ry = -x * st + y * ct;
@@ -698,12 +707,20 @@ __global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* _
         const float f0 = floorf(pu);
         const int i0 = (int)f0;
         const float f = pu - f0;
-        const float* p = prj + ((long)s * g.nview + view) * g.nu;
-        const float v0 = (i0 >= 0 && i0 < g.nu) ? p[i0] : 0.0f;
-        const float v1 = (i0 + 1 >= 0 && i0 + 1 < g.nu) ? p[i0 + 1] : 0.0f;
-        acc += ((1.0f - f) * v0 + f * v1) * (g.dso * g.dso / (rx * rx + d * d));
+        const float w = g.dso * g.dso / (rx * rx + d * d);
+        const bool in0 = i0 >= 0 && i0 < g.nu, in1 = i0 + 1 >= 0 && i0 + 1 < g.nu;
+        const float* p = prj + ((long)s0 * g.nview + view) * g.nu;
+        #pragma unroll
+        for (int q = 0; q < S; ++q) {
+            const float* pq = p + (long)min(q, nsl - 1) * sstride;
+            const float v0 = in0 ? pq[i0] : 0.0f;
+            const float v1 = in1 ? pq[i0 + 1] : 0.0f;
+            acc[q] += ((1.0f - f) * v0 + f * v1) * w;
+        }
     }
-    img[(long)s * g.nx * g.ny + (long)ix * g.ny + iy] = acc;
+    #pragma unroll
+    for (int q = 0; q < S; ++q)
+        if (q < nsl) img[(long)(s0 + q) * g.nx * g.ny + (long)ix * g.ny + iy] = acc[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1355,11 +1372,26 @@ void launch_dd_fan_bp_q(const FanK& g, const float4* wprj_q, float* img_xy, int 
     check_launch("dd_fan_bp_q");
 }
 
+template <int BX, int BY, int S>
+void go_pixel_bp(const FanK& g, const float* prj_vu, float* img_xy, int nslices, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_pixel_fan_bp<BX, BY, S>), dim3(cdiv(g.ny, BX), cdiv(g.nx, BY), cdiv(nslices, S)),
+                       dim3(BX, BY), 0, st, g, prj_vu, img_xy, nslices);
+}
+
+using PixBpFn = void (*)(const FanK&, const float*, float*, int, hipStream_t);
+const PixBpFn kPixBp[] = {go_pixel_bp<16, 16, 1>, go_pixel_bp<8, 8, 1>,   go_pixel_bp<16, 16, 2>,
+                          go_pixel_bp<16, 16, 4>, go_pixel_bp<8, 8, 4>,   go_pixel_bp<16, 16, 8>,
+                          go_pixel_bp<8, 32, 8>,  go_pixel_bp<16, 16, 16>};
+constexpr int kNPixBp = sizeof(kPixBp) / sizeof(kPixBp[0]);
+
 void launch_pixel_fan_bp(const FanK& g, const float* prj_vu, float* img_xy, int nslices, hipStream_t st)
 {
     if (nslices <= 0) return;
-    dim3 blk(16, 16), grd(cdiv(g.ny, 16), cdiv(g.nx, 16), nslices);
-    hipLaunchKernelGGL((k_pixel_fan_bp<16, 16>), grd, blk, 0, st, g, prj_vu, img_xy);
+    RN_REQUIRE(cdiv(g.nx, 8) <= 65535 && cdiv(nslices, 1) <= 65535, "pixel_fan_bp: grid too large");
+    const int c = tune_choose({TK_PIXEL_BP, g.nx, g.ny, g.nu, g.nview, nslices}, kNPixBp,
+                              [&](int i) { kPixBp[i](g, prj_vu, img_xy, nslices, st); }, st, true);
+    kPixBp[c](g, prj_vu, img_xy, nslices, st);
     check_launch("pixel_fan_bp");
 }
 

@@ -45,6 +45,7 @@ enum TuneKernel {
     TK_CNN_FWD = 14,
     TK_HELICAL_BP = 15,
     TK_DD_CONE_FP = 16,
+    TK_PIXEL_BP = 17,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

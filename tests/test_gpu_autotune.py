@@ -65,6 +65,7 @@ def test_every_launch_shape_agrees(ns):
         plan.backward(y, "dd")
         plan.backward(y, "dd_fbp")
         plan.backward(y, "dd_adjoint")
+        plan.backward(y, "pixel")
         for (kernel, *_), (_, ms) in autotune().items():
             counts[kernel] = len(ms)
         assert counts.get(1, 0) >= 13 and counts.get(2, 0) >= 13
@@ -72,14 +73,15 @@ def test_every_launch_shape_agrees(ns):
         dd_fp, dd_bp, dd_fbp = ((("dd_fp", 3), ("dd_bp", 5), ("dd_fbp", 7)) if ns < 4 else
                                 (("dd_fp_q", 12), ("dd_bp_q", 10), ("dd_fbp_q", 11)))
         assert counts.get(dd_bp[1], 0) >= 5 and counts.get(dd_fbp[1], 0) >= 5, counts
-        assert counts.get(dd_fp[1], 0) >= 8 and counts.get(4, 0) >= 8, counts
+        assert counts.get(dd_fp[1], 0) >= 8 and counts.get(4, 0) >= 8 and counts.get(17, 0) >= 8, counts
         refs = {}
         for kname, kid, run in (("siddon_fp", 1, lambda: plan.forward(x, "siddon")),
                                 (dd_fp[0], dd_fp[1], lambda: plan.forward(x, "dd")),
                                 ("dd_adjoint", 4, lambda: plan.backward(y, "dd_adjoint")),
                                 ("siddon_bp", 2, lambda: plan.backward(y, "siddon")),
                                 (dd_bp[0], dd_bp[1], lambda: plan.backward(y, "dd")),
-                                (dd_fbp[0], dd_fbp[1], lambda: plan.backward(y, "dd_fbp"))):
+                                (dd_fbp[0], dd_fbp[1], lambda: plan.backward(y, "dd_fbp")),
+                                ("pixel_bp", 17, lambda: plan.backward(y, "pixel"))):
             for c in range(counts[kid]):
                 autotune_pin(kname, c)
                 out = run()
@@ -96,7 +98,7 @@ def test_every_launch_shape_agrees(ns):
             plan.forward(x, "siddon")
     finally:
         for k in ("siddon_fp", "siddon_bp", "dd_fp", "dd_fp_q", "dd_adjoint", "dd_bp", "dd_fbp", "dd_bp_q",
-                  "dd_fbp_q"):
+                  "dd_fbp_q", "pixel_bp"):
             autotune_pin(k, None)
         autotune(enable=True, reset=True)
 
