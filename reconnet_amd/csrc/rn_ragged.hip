@@ -4,11 +4,14 @@
 // Each slice has its own geometry, angle set and trig tables.  One launch of a
 // fixed grid serves the whole batch: every wave repeatedly takes the next work
 // item from a global counter (a 64-bin strip of one view for the FP, an 8 x 8
-// pixel tile for the BP), finds its slice in the descriptor table and runs the
-// same per-ray / per-pixel code as the single-geometry kernels, so the results
-// are identical to independent per-slice launches.  Descriptors are ordered
-// by decreasing cost so the heaviest slices start first.  Every wave leaves the
-// loop once the counter passes the item total.
+// pixel tile for the BP), finds its descriptor and runs the same per-ray /
+// per-pixel code as the single-geometry kernels, so the results are identical
+// to independent per-slice launches.  Consecutive slices of one geometry and
+// angle set share a descriptor, up to kRagS of them: a lane then carries the
+// ray or pixel through all of them, its geometry work done once (each slice
+// with the arithmetic of a one-slice call).  Descriptors are ordered by
+// decreasing cost so the heaviest start first.  Every wave leaves the loop once
+// the counter passes the item total.
 
 #include "rn_ragged.h"
 
@@ -23,12 +26,14 @@ namespace {
 
 using namespace fan;
 
+constexpr int kRagS = 4;   // slices of one geometry per descriptor
+
 struct RagSlice {
     FanK g;
     const float* pa;     // FP: RX (DD) or Pyx (Siddon); BP: prj [nview][nu]
     const float* pb;     // FP: RY (DD) or Pxy (Siddon)
     float* out;          // FP: prj [nu][nview]; BP: img [nx][ny]
-    int item0, tiles, nitems, pad;
+    int item0, tiles, nitems, nsl;   // nsl consecutive slices, nx * ny (image) / nu * nview (sinogram) apart
 };
 
 __device__ __forceinline__ int find_slice(const RagSlice* __restrict__ sl, int nsl, int item)
@@ -56,16 +61,19 @@ __global__ __launch_bounds__(256) void k_ragged_fp(const RagSlice* __restrict__ 
         const int local = item - d.item0;
         const int view = local / d.tiles, iu = (local % d.tiles) * 64 + lane;
         const bool active = iu < d.g.nu;
+        const long plane = (long)d.g.nx * d.g.ny, sino = (long)d.g.nu * d.g.nview;
         // every lane takes part (wave-uniform line loops)
-        float v;
-        if (siddon) {
-            v = siddon_fp_ray<float>(d.g, d.pa, d.pb, view, active ? iu : d.g.nu - 1, active);
-        } else {
-            float acc[1];
-            dd_fp_ray_slices<float, 1>(d.g, d.pa, d.pb, 1, 1, view, active ? iu : d.g.nu - 1, active, acc);
-            v = acc[0];
+        float acc[kRagS];
+        if (siddon)
+            siddon_fp_ray_slices<float, kRagS>(d.g, d.pa, d.pb, plane, d.nsl, view, active ? iu : d.g.nu - 1, active,
+                                               acc);
+        else
+            dd_fp_ray_slices<float, kRagS>(d.g, d.pa, d.pb, plane, d.nsl, view, active ? iu : d.g.nu - 1, active, acc);
+        if (active) {
+            #pragma unroll
+            for (int q = 0; q < kRagS; ++q)
+                if (q < d.nsl) d.out[q * sino + (long)iu * d.g.nview + view] = acc[q];
         }
-        if (active) d.out[(long)iu * d.g.nview + view] = v;
     }
 }
 
@@ -80,16 +88,17 @@ __global__ __launch_bounds__(256) void k_ragged_bp(const RagSlice* __restrict__ 
         const int local = item - d.item0;
         const int ix = (local / d.tiles) * 8 + (lane >> 3), iy = (local % d.tiles) * 8 + (lane & 7);
         if (ix < d.g.nx && iy < d.g.ny) {
-            float v;
-            if (siddon) {
-                v = siddon_bp_pixel<float>(d.g, d.pa, 0, d.g.nview, ix, iy);
-            } else {
-                float acc[1];
-                if (d.g.unit_angle) dd_adj_pixel_slices<true, 1>(d.g, d.pa, 0, 1, 0, d.g.nview, ix, iy, acc);
-                else dd_adj_pixel_slices<false, 1>(d.g, d.pa, 0, 1, 0, d.g.nview, ix, iy, acc);
-                v = acc[0];
-            }
-            d.out[(long)ix * d.g.ny + iy] = v;
+            const long plane = (long)d.g.nx * d.g.ny, sino = (long)d.g.nu * d.g.nview;
+            float acc[kRagS];
+            if (siddon)
+                siddon_bp_pixel_slices<float, 4, kRagS>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
+            else if (d.g.unit_angle)
+                dd_adj_pixel_slices<true, kRagS>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
+            else
+                dd_adj_pixel_slices<false, kRagS>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
+            #pragma unroll
+            for (int q = 0; q < kRagS; ++q)
+                if (q < d.nsl) d.out[q * plane + (long)ix * d.g.ny + iy] = acc[q];
         }
     }
 }
@@ -110,7 +119,8 @@ int persistent_blocks()
 struct RaggedBatch::Impl {
     std::vector<FanTables> tables;
     std::vector<long> img_off, sino_off;
-    std::vector<int> order;       // slices by decreasing cost
+    std::vector<int> first, count;   // descriptor k: slices first[k] .. first[k] + count[k] - 1
+    std::vector<int> order;          // descriptors by decreasing cost
     long img_total = 0, sino_total = 0;
     DevBuf desc, counter, wa, wb, wc;
     std::vector<RagSlice> host;
@@ -141,14 +151,34 @@ void RaggedBatch::setup(const std::vector<FanGeo>& geos, const std::vector<const
     }
     p_->img_total = io;
     p_->sino_total = so;
-    p_->order.resize(n);
+    // runs of consecutive slices with one geometry and angle set, in chunks of kRagS
+    auto same = [&](int a, int b) {
+        const FanGeo& x = geos[a];
+        const FanGeo& y = geos[b];
+        return x.nx == y.nx && x.ny == y.ny && x.nz == y.nz && x.dx == y.dx && x.dy == y.dy && x.dz == y.dz &&
+               x.nu == y.nu && x.nview == y.nview && x.nv == y.nv && x.da == y.da && x.dv == y.dv &&
+               x.off_a == y.off_a && x.off_v == y.off_v && x.dsd == y.dsd && x.dso == y.dso &&
+               std::equal(angles[a], angles[a] + x.nview, angles[b]);
+    };
+    p_->first.clear();
+    p_->count.clear();
+    for (int s = 0; s < n; s++) {
+        if (!p_->count.empty() && p_->count.back() < kRagS && same(p_->first.back(), s)) {
+            ++p_->count.back();
+        } else {
+            p_->first.push_back(s);
+            p_->count.push_back(1);
+        }
+    }
+    const int nd = (int)p_->first.size();
+    p_->order.resize(nd);
     std::iota(p_->order.begin(), p_->order.end(), 0);
     std::stable_sort(p_->order.begin(), p_->order.end(), [&](int a, int b) {
-        const double ca = (double)geos[a].nx * geos[a].ny * geos[a].nview;
-        const double cb = (double)geos[b].nx * geos[b].ny * geos[b].nview;
-        return ca > cb;
+        const FanGeo& ga = geos[p_->first[a]];
+        const FanGeo& gb = geos[p_->first[b]];
+        return (double)ga.nx * ga.ny * ga.nview > (double)gb.nx * gb.ny * gb.nview;
     });
-    p_->desc.alloc(n * sizeof(RagSlice));
+    p_->desc.alloc(nd * sizeof(RagSlice));
     p_->counter.alloc(sizeof(int));
     p_->wa.alloc(io * sizeof(float));
     p_->wb.alloc(std::max(io, so) * sizeof(float));
@@ -161,29 +191,30 @@ long RaggedBatch::sino_elems() const { return p_->sino_total; }
 
 void RaggedBatch::forward(bool siddon, const float* img, float* prj, hipStream_t st)
 {
-    const int n = size();
+    const int nd = (int)p_->first.size();
     float* A = p_->wa.as<float>();
     float* Bb = p_->wb.as<float>();
-    p_->host.assign(n, RagSlice{});
+    p_->host.assign(nd, RagSlice{});
     int item = 0;
-    for (int k = 0; k < n; k++) {
-        const int s = p_->order[k];
+    for (int k = 0; k < nd; k++) {
+        const int s = p_->first[p_->order[k]], cnt = p_->count[p_->order[k]];
         const FanGeo& g = p_->tables[s].geo;
         const float* P = img + p_->img_off[s];
         float* a = A + p_->img_off[s];
         float* b = Bb + p_->img_off[s];
         RagSlice& d = p_->host[k];
         d.g = p_->tables[s].k;
+        d.nsl = cnt;
         if (siddon) {
-            launch_transpose(P, a, g.nx, g.ny, 1, st);      // rows contiguous
+            launch_transpose(P, a, g.nx, g.ny, cnt, st);    // rows contiguous
             d.pa = a;
             d.pb = P;
         } else {
             // RY[x][y] = (P[x-1][y] + P[x][y]) / 2 ;  RX = transpose((P[x][y-1] + P[x][y]) / 2)
             float* c = p_->wc.as<float>() + p_->img_off[s];
-            launch_avg_prev(P, b, g.nx, g.ny, 1, 0, st);
-            launch_avg_prev(P, c, g.nx, g.ny, 1, 1, st);
-            launch_transpose(c, a, g.nx, g.ny, 1, st);
+            launch_avg_prev(P, b, g.nx, g.ny, cnt, 0, st);
+            launch_avg_prev(P, c, g.nx, g.ny, cnt, 1, st);
+            launch_transpose(c, a, g.nx, g.ny, cnt, st);
             d.pa = a;
             d.pb = b;
         }
@@ -193,9 +224,9 @@ void RaggedBatch::forward(bool siddon, const float* img, float* prj, hipStream_t
         d.item0 = item;
         item += d.nitems;
     }
-    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), n * sizeof(RagSlice), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), nd * sizeof(RagSlice), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p_->counter.as<int>());
-    hipLaunchKernelGGL(k_ragged_fp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), n, item,
+    hipLaunchKernelGGL(k_ragged_fp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), nd, item,
                        p_->counter.as<int>(), siddon ? 1 : 0);
     check_launch("ragged_fp");
     RN_HIP_CHECK(hipStreamSynchronize(st));
@@ -203,17 +234,18 @@ void RaggedBatch::forward(bool siddon, const float* img, float* prj, hipStream_t
 
 void RaggedBatch::backward(bool siddon, const float* prj, float* img, hipStream_t st)
 {
-    const int n = size();
+    const int nd = (int)p_->first.size();
     float* V = p_->wb.as<float>();   // view-major copies of the sinograms
-    p_->host.assign(n, RagSlice{});
+    p_->host.assign(nd, RagSlice{});
     int item = 0;
-    for (int k = 0; k < n; k++) {
-        const int s = p_->order[k];
+    for (int k = 0; k < nd; k++) {
+        const int s = p_->first[p_->order[k]], cnt = p_->count[p_->order[k]];
         const FanGeo& g = p_->tables[s].geo;
         float* v = V + p_->sino_off[s];
-        launch_transpose(prj + p_->sino_off[s], v, g.nu, g.nview, 1, st);
+        launch_transpose(prj + p_->sino_off[s], v, g.nu, g.nview, cnt, st);
         RagSlice& d = p_->host[k];
         d.g = p_->tables[s].k;
+        d.nsl = cnt;
         d.pa = v;
         d.pb = nullptr;
         d.out = img + p_->img_off[s];
@@ -222,9 +254,9 @@ void RaggedBatch::backward(bool siddon, const float* prj, float* img, hipStream_
         d.item0 = item;
         item += d.nitems;
     }
-    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), n * sizeof(RagSlice), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), nd * sizeof(RagSlice), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p_->counter.as<int>());
-    hipLaunchKernelGGL(k_ragged_bp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), n, item,
+    hipLaunchKernelGGL(k_ragged_bp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), nd, item,
                        p_->counter.as<int>(), siddon ? 1 : 0);
     check_launch("ragged_bp");
     RN_HIP_CHECK(hipStreamSynchronize(st));
