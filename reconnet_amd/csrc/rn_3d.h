@@ -11,7 +11,14 @@ struct ArbGeo {
     float dx, dy, dz, cx, cy, cz;
     int nu, nview, nv;
     float du, dv, off_u, off_v;
+    // every view has its detector rows along z and its columns in a
+    // horizontal plane (detV = (0, 0, v), detU.z = 0; arb_rows_along_z):
+    // the cone backprojection then runs a kernel without the general-view
+    // path.  Set by callers that hold the views on the host; 0 is always safe.
+    int rows_z = 0;
 };
+
+bool arb_rows_along_z(const float* detU, const float* detV, int nview);
 
 // Siddon with per-view detCenter / detU / detV and either invRayDir
 // (parallel, SiddonParallel.cu) or the source position (cone, SiddonCone.cu).

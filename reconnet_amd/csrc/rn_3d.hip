@@ -184,6 +184,41 @@ __device__ __forceinline__ void slab(const F3& o, const F3& d, const F3& lo, con
     }
 }
 
+// [s0, s1] on the ray p(s) = s d - r (a voxel-centred frame) inside the box
+// |p| <= h, as slab() with the half-open faces: an axis with d = 0 keeps the
+// interval if -h < r <= h.  r is small (the ray passes within a few voxels of
+// the centre), so the parameters carry float precision relative to the chord
+// instead of relative to the source distance.
+__device__ __forceinline__ void slab_local(const F3& r, const F3& d, const F3& h, float& s0, float& s1)
+{
+    s0 = -INFINITY;
+    s1 = INFINITY;
+    const float rc[3] = {r.x, r.y, r.z}, dc[3] = {d.x, d.y, d.z}, hc[3] = {h.x, h.y, h.z};
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (dc[k] != 0.0f) {
+            const float inv = __builtin_amdgcn_rcpf(dc[k]);
+            const float a0 = (rc[k] - hc[k]) * inv, a1 = (rc[k] + hc[k]) * inv;
+            s0 = fmaxf(s0, fminf(a0, a1));
+            s1 = fminf(s1, fmaxf(a0, a1));
+        } else if (!(rc[k] > -hc[k] && rc[k] <= hc[k])) {
+            s1 = -INFINITY;
+        }
+    }
+}
+
+// the wave's copy of a value every active lane holds alike (an SGPR operand)
+__device__ __forceinline__ float wave_uniform(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double wave_uniform(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Per view, in FP64: the detector normal N = U x V, the dual basis (Ud, Vd) of
 // (U, V) and the projection constants: cone A = (C - S).N, su, sv = (S - C).Ud,
 // (S - C).Vd; parallel sdn, sdu, sdv = invRayDir.(N, Ud, Vd) and the start
@@ -221,6 +256,7 @@ __global__ void k_siddon_arb_views(ArbGeo a, Grid3K g, int parallel, const float
     vt[4 * view + 3] = make_float4((float)m, 0.0f, 0.0f, 0.0f);
 }
 
+template <bool ROWS_Z>
 __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ prj,
                                                        float* __restrict__ img, const float* C, const float* U,
                                                        const float* V, const float* S, const float4* __restrict__ vt,
@@ -232,18 +268,120 @@ __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int p
     if (iz >= g.nz || iy >= g.ny) return;
     const int cc = bc % nC, b = bc / nC;
     const float* q = prj + (long)b * a.nu * a.nview * a.nv * nC + cc;
-    const F3 vlo{g.cx + (ix - g.nx / 2.0f) * g.dx, g.cy + (iy - g.ny / 2.0f) * g.dy, g.cz + (iz - g.nz / 2.0f) * g.dz};
-    const F3 vhi{vlo.x + g.dx, vlo.y + g.dy, vlo.z + g.dz};
-    const F3 glo{g.cx - g.nx / 2.0f * g.dx, g.cy - g.ny / 2.0f * g.dy, g.cz - g.nz / 2.0f * g.dz};
-    const F3 ghi{g.cx + g.nx / 2.0f * g.dx, g.cy + g.ny / 2.0f * g.dy, g.cz + g.nz / 2.0f * g.dz};
-    const float cu = (a.nu - 1) / 2.0f + a.off_u, cv = (a.nv - 1) / 2.0f + a.off_v;
+    // wave-uniform values are kept in scalar registers (wave_uniform)
+    const float vlz = g.cz + (iz - g.nz / 2.0f) * g.dz;
+    const F3 vlo{wave_uniform(g.cx + (ix - g.nx / 2.0f) * g.dx), wave_uniform(g.cy + (iy - g.ny / 2.0f) * g.dy), vlz};
+    const F3 vhi{wave_uniform(vlo.x + g.dx), wave_uniform(vlo.y + g.dy), vlz + g.dz};
+    const F3 glo{wave_uniform(g.cx - g.nx / 2.0f * g.dx), wave_uniform(g.cy - g.ny / 2.0f * g.dy),
+                 wave_uniform(g.cz - g.nz / 2.0f * g.dz)};
+    const F3 ghi{wave_uniform(g.cx + g.nx / 2.0f * g.dx), wave_uniform(g.cy + g.ny / 2.0f * g.dy),
+                 wave_uniform(g.cz + g.nz / 2.0f * g.dz)};
+    // the voxel centre in FP64 and its half widths: chords are measured from a
+    // point of the ray next to the voxel (slab_local), because parameters
+    // counted from the source (~600 mm away) lose ~5e-5 mm to float rounding
+    const double vcx = wave_uniform(g.cx + (ix + 0.5 - g.nx / 2.0) * g.dx);
+    const double vcy = wave_uniform(g.cy + (iy + 0.5 - g.ny / 2.0) * g.dy);
+    const double vcz = g.cz + (iz + 0.5 - g.nz / 2.0) * g.dz;
+    const F3 hw{wave_uniform(0.5f * g.dx), wave_uniform(0.5f * g.dy), wave_uniform(0.5f * g.dz)};
+    const float cu = wave_uniform((a.nu - 1) / 2.0f + a.off_u), cv = wave_uniform((a.nv - 1) / 2.0f + a.off_v);
     auto dot = [](const F3& p, const F3& r) { return p.x * r.x + p.y * r.y + p.z * r.z; };
     float acc = 0.f;
     for (int view = 0; view < a.nview; ++view) {
         const F3 Cv = ld3(C, view), Uv = ld3(U, view), Vv = ld3(V, view), Sv = ld3(S, view);
         const float4 t0v = vt[4 * view], t1v = vt[4 * view + 1], t2v = vt[4 * view + 2];
         const F3 N{t0v.x, t0v.y, t0v.z}, Ud{t1v.x, t1v.y, t1v.z}, Vd{t2v.x, t2v.y, t2v.z};
-        // the voxel's corners on the detector (u, v in mm)
+        // cone views whose detector rows run along z and columns lie in a
+        // horizontal plane (V = (0, 0, v), U.z = 0): N and Ud are horizontal
+        // and Vd = (0, 0, 1/v), so a voxel's detector columns depend on (x, y)
+        // only -- the column range and each column's in-plane ray direction
+        // are the wave's (the wave holds one (x, y)) -- and its rows scale
+        // with z: each row adds its z slab to the column's x and y slabs
+        if (ROWS_Z || (!parallel && Vv.x == 0.f && Vv.y == 0.f && Uv.z == 0.f)) {
+            const float A = t0v.w, su = t1v.w, sv = t2v.w;
+            float umin = INFINITY, umax = -INFINITY, tmin = INFINITY, tmax = -INFINITY;
+            bool ok = true;
+            #pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float wxk = ((k & 1) ? vhi.x : vlo.x) - Sv.x, wyk = ((k & 2) ? vhi.y : vlo.y) - Sv.y;
+                const float den = fmaf(wxk, N.x, wyk * N.y);
+                ok = ok && (den * A > 0.f);
+                const float t = A * __builtin_amdgcn_rcpf(den);
+                const float u = fmaf(t, fmaf(wxk, Ud.x, wyk * Ud.y), su);
+                umin = fminf(umin, u); umax = fmaxf(umax, u);
+                tmin = fminf(tmin, t); tmax = fmaxf(tmax, t);
+            }
+            if (!ok) continue;
+            const float z0 = (vlo.z - Sv.z) * Vd.z, z1 = (vhi.z - Sv.z) * Vd.z;
+            const float vmin = fminf(fminf(tmin * z0, tmax * z0), fminf(tmin * z1, tmax * z1)) + sv;
+            const float vmax = fmaxf(fmaxf(tmin * z0, tmax * z0), fmaxf(tmin * z1, tmax * z1)) + sv;
+            const float fu0 = umin / a.du + cu, fu1 = umax / a.du + cu, fv0 = vmin / a.dv + cv, fv1 = vmax / a.dv + cv;
+            const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0);
+            const int iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
+            const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0);
+            const int iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
+            const double wx = wave_uniform(vcx - Sv.x), wy = wave_uniform(vcy - Sv.y), wz = vcz - Sv.z;
+            // rows in float: the voxel's z offset from the source split in two
+            // floats, the detector centre's height above the source rounded once
+            const float wz_hi = (float)wz, wz_lo = (float)(wz - wz_hi), czs = (float)((double)Cv.z - Sv.z);
+            for (int iu = iu0; iu <= iu1; ++iu) {
+                const double uc = (double)(iu - cu) * a.du;
+                const double dxd = wave_uniform(fma((double)Uv.x, uc, (double)Cv.x) - Sv.x);
+                const double dyd = wave_uniform(fma((double)Uv.y, uc, (double)Cv.y) - Sv.y);
+                const float dx = wave_uniform((float)dxd), dy = wave_uniform((float)dyd);
+                const float dxy2 = wave_uniform(fmaf(dx, dx, dy * dy));
+                // tp: the column's in-plane closest approach to the voxel (any
+                // nearby value serves: r is exact for the tp used)
+                const float tp = wave_uniform((float)(wx * dxd + wy * dyd) * __builtin_amdgcn_rcpf(dxy2));
+                const float rx = wave_uniform((float)fma(-(double)tp, dxd, wx));
+                const float ry = wave_uniform((float)fma(-(double)tp, dyd, wy));
+                float xy0 = -INFINITY, xy1 = INFINITY;
+                if (dx != 0.f) {
+                    const float inv = __builtin_amdgcn_rcpf(dx);
+                    const float a0 = (rx - hw.x) * inv, a1 = (rx + hw.x) * inv;
+                    xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
+                } else if (!(rx > -hw.x && rx <= hw.x)) {
+                    xy1 = -INFINITY;
+                }
+                if (dy != 0.f) {
+                    const float inv = __builtin_amdgcn_rcpf(dy);
+                    const float a0 = (ry - hw.y) * inv, a1 = (ry + hw.y) * inv;
+                    xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
+                } else if (!(ry > -hw.y && ry <= hw.y)) {
+                    xy1 = -INFINITY;
+                }
+                xy0 = wave_uniform(xy0);
+                xy1 = wave_uniform(xy1);
+                if (!(xy1 > fmaxf(xy0, -tp))) continue;
+                const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
+                for (int iv = iv0; iv <= iv1; ++iv) {
+                    const float dz = fmaf(Vv.z, (iv - cv) * a.dv, czs), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
+                    float s0 = xy0, s1 = xy1;
+                    if (dz != 0.f) {
+                        const float inv = __builtin_amdgcn_rcpf(dz);
+                        const float a0 = (rz - hw.z) * inv, a1 = (rz + hw.z) * inv;
+                        s0 = fmaxf(s0, fminf(a0, a1)); s1 = fminf(s1, fmaxf(a0, a1));
+                    } else if (!(rz > -hw.z && rz <= hw.z)) {
+                        continue;
+                    }
+                    s0 = fmaxf(s0, -tp);
+                    if (!(s1 > s0)) continue;
+                    if (s0 >= 1.f - tp) {
+                        // a voxel beyond the detector point: the ray counts only
+                        // if it entered the grid before that point
+                        float g0, g1;
+                        slab(Sv, F3{dx, dy, dz}, glo, ghi, g0, g1);
+                        if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
+                    }
+                    acc = fmaf((s1 - s0) * sqrtf(fmaf(dz, dz, dxy2)), qc[(long)iv * nC], acc);
+                }
+            }
+            continue;
+        }
+        // any other view (not instantiated when the caller has found every
+        // view to be of the kind above: the FP64 ray set-up below would
+        // double the register count): the voxel's eight corners on the
+        // detector (u, v in mm)
+        if constexpr (ROWS_Z) continue;
         float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
         bool ok = true;
         float m = 0.f;
@@ -280,74 +418,36 @@ __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int p
         const float fu0 = umin / a.du + cu, fu1 = umax / a.du + cu, fv0 = vmin / a.dv + cv, fv1 = vmax / a.dv + cv;
         const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0), iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
         const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0), iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
-        // cone views whose detector rows run along z and columns lie in a
-        // horizontal plane: a column's rays share their in-plane direction, so
-        // the x and y slabs are computed once per column and each row adds its
-        // z slab (the same numbers as the general path below)
-        const bool vz = !parallel && Vv.x == 0.f && Vv.y == 0.f && Uv.z == 0.f;
-        for (int iu = iu0; iu <= iu1 && vz; ++iu) {
-            const float uc = (iu - cu) * a.du;
-            const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
-            const float dx = fmaf(Uv.x, uc, Cv.x) - Sv.x, dy = fmaf(Uv.y, uc, Cv.y) - Sv.y;
-            float xy0 = -INFINITY, xy1 = INFINITY;
-            if (dx != 0.f) {
-                const float inv = __builtin_amdgcn_rcpf(dx);
-                const float a0 = (vlo.x - Sv.x) * inv, a1 = (vhi.x - Sv.x) * inv;
-                xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
-            } else if (!(Sv.x >= vlo.x && Sv.x < vhi.x)) {
-                xy1 = -INFINITY;
-            }
-            if (dy != 0.f) {
-                const float inv = __builtin_amdgcn_rcpf(dy);
-                const float a0 = (vlo.y - Sv.y) * inv, a1 = (vhi.y - Sv.y) * inv;
-                xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
-            } else if (!(Sv.y >= vlo.y && Sv.y < vhi.y)) {
-                xy1 = -INFINITY;
-            }
-            if (!(xy1 > fmaxf(xy0, 0.f))) continue;
-            const float dxy2 = fmaf(dx, dx, dy * dy);
-            for (int iv = iv0; iv <= iv1; ++iv) {
-                const float vc = (iv - cv) * a.dv;
-                const float dz = fmaf(Vv.z, vc, Cv.z) - Sv.z;
-                float t0 = xy0, t1 = xy1;
-                if (dz != 0.f) {
-                    const float inv = __builtin_amdgcn_rcpf(dz);
-                    const float a0 = (vlo.z - Sv.z) * inv, a1 = (vhi.z - Sv.z) * inv;
-                    t0 = fmaxf(t0, fminf(a0, a1)); t1 = fminf(t1, fmaxf(a0, a1));
-                } else if (!(Sv.z >= vlo.z && Sv.z < vhi.z)) {
-                    continue;
-                }
-                t0 = fmaxf(t0, 0.f);
-                if (!(t1 > t0)) continue;
-                if (t0 >= 1.f) {
-                    float g0, g1;
-                    slab(Sv, F3{dx, dy, dz}, glo, ghi, g0, g1);
-                    if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
-                }
-                acc = fmaf((t1 - t0) * sqrtf(fmaf(dz, dz, dxy2)), qc[(long)iv * nC], acc);
-            }
-        }
-        for (int iu = iu0; iu <= iu1 && !vz; ++iu) {
-            const float uc = (iu - cu) * a.du;
+        for (int iu = iu0; iu <= iu1; ++iu) {
+            const double uc = (double)(iu - cu) * a.du;
             const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
             for (int iv = iv0; iv <= iv1; ++iv) {
-                const float vc = (iv - cv) * a.dv;
-                const F3 dst{Cv.x + Uv.x * uc + Vv.x * vc, Cv.y + Uv.y * uc + Vv.y * vc, Cv.z + Uv.z * uc + Vv.z * vc};
-                const F3 o = parallel ? F3{dst.x + Sv.x * m, dst.y + Sv.y * m, dst.z + Sv.z * m} : Sv;
-                const F3 d{dst.x - o.x, dst.y - o.y, dst.z - o.z};
-                float t0, t1;
-                slab(o, d, vlo, vhi, t0, t1);
-                t0 = fmaxf(t0, 0.f);
-                if (!(t1 > t0)) continue;
-                if (t0 >= 1.f) {
+                const double vc = (double)(iv - cv) * a.dv;
+                const double dst[3] = {Cv.x + Uv.x * uc + Vv.x * vc, Cv.y + Uv.y * uc + Vv.y * vc,
+                                       Cv.z + Uv.z * uc + Vv.z * vc};
+                const double o[3] = {parallel ? dst[0] + (double)Sv.x * m : (double)Sv.x,
+                                     parallel ? dst[1] + (double)Sv.y * m : (double)Sv.y,
+                                     parallel ? dst[2] + (double)Sv.z * m : (double)Sv.z};
+                const double dd[3] = {dst[0] - o[0], dst[1] - o[1], dst[2] - o[2]};
+                const double w[3] = {vcx - o[0], vcy - o[1], vcz - o[2]};
+                const float tp = (float)((w[0] * dd[0] + w[1] * dd[1] + w[2] * dd[2]) /
+                                         (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]));
+                const F3 r{(float)fma(-(double)tp, dd[0], w[0]), (float)fma(-(double)tp, dd[1], w[1]),
+                           (float)fma(-(double)tp, dd[2], w[2])};
+                const F3 d{(float)dd[0], (float)dd[1], (float)dd[2]};
+                float s0, s1;
+                slab_local(r, d, hw, s0, s1);
+                s0 = fmaxf(s0, -tp);
+                if (!(s1 > s0)) continue;
+                if (s0 >= 1.f - tp) {
                     // a voxel beyond the detector point: the ray counts only if it
                     // entered the grid before that point
+                    const F3 of{(float)o[0], (float)o[1], (float)o[2]};
                     float g0, g1;
-                    slab(o, d, glo, ghi, g0, g1);
+                    slab(of, d, glo, ghi, g0, g1);
                     if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
                 }
-                const float len = (t1 - t0) * sqrtf(dot(d, d));
-                acc = fmaf(len, qc[(long)iv * nC], acc);
+                acc = fmaf((s1 - s0) * sqrtf(dot(d, d)), qc[(long)iv * nC], acc);
             }
         }
     }
@@ -837,6 +937,14 @@ Grid3K grid_of(const ArbGeo& g) { return Grid3K{g.nx, g.ny, g.nz, g.dx, g.dy, g.
 
 } // namespace
 
+bool arb_rows_along_z(const float* detU, const float* detV, int nview)
+{
+    if (!detU || !detV) return false;
+    for (int v = 0; v < nview; v++)
+        if (detV[3 * v] != 0.f || detV[3 * v + 1] != 0.f || detU[3 * v + 2] != 0.f) return false;
+    return true;
+}
+
 void siddon_arb_fp(const ArbGeo& g, bool parallel, int nB, int nC, const float* img, float* prj,
                    const float* detCenter, const float* detU, const float* detV, const float* srcOrDir,
                    DevBuf& work, hipStream_t st)
@@ -862,7 +970,8 @@ void siddon_arb_bp(const ArbGeo& g, bool parallel, int nB, int nC, const float* 
                        parallel ? 1 : 0, detCenter, detU, detV, srcOrDir, vt);
     check_launch("siddon_arb_views");
     dim3 blk(256), grd(cdiv(g.nz, 64), cdiv(g.ny, 4), g.nx * nB * nC);
-    hipLaunchKernelGGL(k_siddon_arb_bp, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, prj, img, detCenter, detU,
+    hipLaunchKernelGGL(!parallel && g.rows_z ? k_siddon_arb_bp<true> : k_siddon_arb_bp<false>, grd, blk, 0, st, g,
+                       grid_of(g), parallel ? 1 : 0, prj, img, detCenter, detU,
                        detV, srcOrDir, vt, nC);
     check_launch("siddon_arb_bp");
 }
@@ -932,6 +1041,7 @@ void ConePlan::setup(const ArbGeo& g, const float* C, const float* U, const floa
     RN_HIP_CHECK(hipMemcpyAsync(cls_.get(), cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, st));
     RN_HIP_CHECK(hipStreamSynchronize(st));
     geo = g;
+    geo.rows_z = arb_rows_along_z(U, V, n) ? 1 : 0;
 }
 
 void ConePlan::dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t st) const

@@ -359,10 +359,13 @@ void arb_call(const char* name, bool forward, int kind, float* out, const float*
         if (forward) siddon_arb_fp(g, true, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, d2, d3, c.g, c.st());
         else siddon_arb_bp(g, true, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, d2, d3, c.g, c.st());
         break;
-    case 1: // Siddon cone: v3 = src
-        if (forward) siddon_arb_fp(g, false, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, d2, d3, c.g, c.st());
-        else siddon_arb_bp(g, false, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, d2, d3, c.g, c.st());
+    case 1: { // Siddon cone: v3 = src
+        ArbGeo gz = g;
+        gz.rows_z = arb_rows_along_z(v1, v2, g.nview) ? 1 : 0;
+        if (forward) siddon_arb_fp(gz, false, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, d2, d3, c.g, c.st());
+        else siddon_arb_bp(gz, false, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, d2, d3, c.g, c.st());
         break;
+    }
     case 2: // DD tomo: v0 = detCenter, v1 = src
         if (forward) dd_tomo_fp(g, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, c.g, c.st());
         else dd_tomo_bp(g, nB, nC, c.a.as<float>(), c.b.as<float>(), d0, d1, c.g, c.st());
@@ -458,6 +461,7 @@ extern "C" void cOSSQSConeAbitrary(float* res, const float* img, const float* pr
         check_bc(nBatches, nChannels);
         RN_REQUIRE(nviewPerSubset > 0 && nSubsets > 0, "subset sizes must be positive");
         ArbGeo g = make_arb(nx, ny, nz, dx, dy, dz, cx, cy, cz, nu, nview, nv, du, dv, off_u, off_v);
+        g.rows_z = arb_rows_along_z(detU, detV, nview) ? 1 : 0;
         ThreadCtx& c = ctx();
         const size_t nimg = (size_t)nBatches * nx * ny * nz * nChannels;
         const size_t nprj = (size_t)nBatches * nu * nview * nv * nChannels, nvec = 3 * (size_t)nview;

@@ -420,8 +420,17 @@ def _arb_recon(nx, ny, nz, nu, nv, du, dv, rotview):
     return r
 
 
-def test_siddon_cone_parity():
+@pytest.mark.parametrize("tilt", ["none", "all", "every_other"])
+def test_siddon_cone_parity(tilt):
+    # "all" / "every_other": the detector of those views turned by 0.3 rad
+    # about its normal, so detector rows no longer run along z (the general
+    # view path of the backprojection; mixed with upright views in one call)
     C, U, V, S = _cone(24)
+    if tilt != "none":
+        sel = slice(None) if tilt == "all" else slice(0, None, 2)
+        c, s = np.float32(np.cos(0.3)), np.float32(np.sin(0.3))
+        U2, V2 = c * U + s * V, c * V - s * U
+        U[sel], V[sel] = U2[sel], V2[sel]
     r = _arb_recon(24, 24, 12, 40, 16, 1.5, 1.5, 24)
     rng = np.random.default_rng(5)
     x = rng.uniform(0, 1, (1, 24, 24, 12, 1)).astype(np.float32)
