@@ -322,8 +322,8 @@ int rn_autotune_reset(void);
  * 3 DD FP, 4 DD adjoint, 5 DD BP, 7 DD FBP BP; 8 / 9 Siddon FP / BP,
  * 10 / 11 DD BP / FBP BP and 12 DD FP on the interleaved 4-slice layout used
  * for 4 or more slices; 13 the CNN's 32 -> 32 convolution, 14 the one-launch CNN forward, 15 the
- * helical backprojection, 16 the cone DD forward projector, 17 the pixel-driven BP) for every shape,
- * bypassing the
+ * helical backprojection, 16 the cone DD forward projector, 17 the pixel-driven BP, 18 the Siddon cone
+ * backprojection of upright detectors, 19 the cone DD adjoint) for every shape, bypassing the
  * timing; candidate < 0 removes the pin.  A launch of a kernel with fewer
  * candidates fails with an error.  Lets tests run every launch shape. */
 int rn_autotune_pin(int kernel, int candidate);

@@ -81,7 +81,8 @@ def check(status, what):
 # kernel ids of the tuned launches (TuneKernel in csrc/rn_tune.h)
 TUNED_KERNELS = {"siddon_fp": 1, "siddon_bp": 2, "dd_fp": 3, "dd_adjoint": 4, "dd_bp": 5, "sqs_update": 6,
                  "dd_fbp": 7, "siddon_fp_q": 8, "siddon_bp_q": 9, "dd_bp_q": 10, "dd_fbp_q": 11, "dd_fp_q": 12,
-                 "conv_mid": 13, "cnn_forward": 14, "helical_bp": 15, "dd_cone_fp": 16, "pixel_bp": 17}
+                 "conv_mid": 13, "cnn_forward": 14, "helical_bp": 15, "dd_cone_fp": 16, "pixel_bp": 17,
+                 "siddon_cone_bp": 18, "dd_cone_adj": 19}
 
 
 def autotune_pin(kernel, candidate):

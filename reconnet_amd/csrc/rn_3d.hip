@@ -256,202 +256,241 @@ __global__ void k_siddon_arb_views(ArbGeo a, Grid3K g, int parallel, const float
     vt[4 * view + 3] = make_float4((float)m, 0.0f, 0.0f, 0.0f);
 }
 
-template <bool ROWS_Z>
+// What one lane's voxels take from one cone view whose detector rows run
+// along z and whose columns lie in a horizontal plane (V = (0, 0, v),
+// U.z = 0).  N and Ud are then horizontal and Vd = (0, 0, 1/v): a voxel's
+// detector columns depend on (x, y) only, so the column range and each
+// column's in-plane ray direction and x/y slabs are the wave's (a wave holds
+// one (x, y)); the rows scale with z and each adds its z slab.  A lane holds
+// ZPL voxels stacked in z from vlz (bottom face) and vcz (centre of the
+// lowest, FP64); every row is loaded once for all of them.
+struct SidBpCol {
+    F3 vlo, vhi;          // the wave's x/y faces (z unused)
+    F3 glo, ghi, hw;      // grid box, voxel half widths
+    double vcx, vcy;      // the wave's voxel centre (x, y), FP64
+    float cu, cv, gdz;
+};
+
+template <int ZPL>
+__device__ __forceinline__ void siddon_rows_z_view(const ArbGeo& a, const SidBpCol& w, int view, const F3& Cv,
+                                                   const F3& Uv, const F3& Vv, const F3& Sv, const float4& t0v,
+                                                   const float4& t1v, const float4& t2v, float vlz, double vcz,
+                                                   const float* __restrict__ q, int nC, float (&acc)[ZPL])
+{
+    const F3 N{t0v.x, t0v.y, t0v.z}, Ud{t1v.x, t1v.y, t1v.z};
+    const float A = t0v.w, su = t1v.w, sv = t2v.w, vdz = t2v.z;
+    float umin = INFINITY, umax = -INFINITY, tmin = INFINITY, tmax = -INFINITY;
+    bool ok = true;
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float wxk = ((k & 1) ? w.vhi.x : w.vlo.x) - Sv.x, wyk = ((k & 2) ? w.vhi.y : w.vlo.y) - Sv.y;
+        const float den = fmaf(wxk, N.x, wyk * N.y);
+        ok = ok && (den * A > 0.f);
+        const float t = A * __builtin_amdgcn_rcpf(den);
+        const float u = fmaf(t, fmaf(wxk, Ud.x, wyk * Ud.y), su);
+        umin = fminf(umin, u); umax = fmaxf(umax, u);
+        tmin = fminf(tmin, t); tmax = fmaxf(tmax, t);
+    }
+    if (!ok) return;
+    // rows: the corners' v over the lane's stack of voxels
+    const float z0 = (vlz - Sv.z) * vdz, z1 = (vlz + ZPL * w.gdz - Sv.z) * vdz;
+    const float vmin = fminf(fminf(tmin * z0, tmax * z0), fminf(tmin * z1, tmax * z1)) + sv;
+    const float vmax = fmaxf(fmaxf(tmin * z0, tmax * z0), fmaxf(tmin * z1, tmax * z1)) + sv;
+    const float fu0 = umin / a.du + w.cu, fu1 = umax / a.du + w.cu;
+    const float fv0 = vmin / a.dv + w.cv, fv1 = vmax / a.dv + w.cv;
+    const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0), iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
+    const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0), iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
+    const double wx = wave_uniform(w.vcx - Sv.x), wy = wave_uniform(w.vcy - Sv.y), wz = vcz - Sv.z;
+    // rows in float: the lowest voxel's z offset from the source split in two
+    // floats, the detector centre's height above the source rounded once
+    const float wz_hi = (float)wz, wz_lo = (float)(wz - wz_hi), czs = (float)((double)Cv.z - Sv.z);
+    for (int iu = iu0; iu <= iu1; ++iu) {
+        const double uc = (double)(iu - w.cu) * a.du;
+        const double dxd = wave_uniform(fma((double)Uv.x, uc, (double)Cv.x) - Sv.x);
+        const double dyd = wave_uniform(fma((double)Uv.y, uc, (double)Cv.y) - Sv.y);
+        const float dx = wave_uniform((float)dxd), dy = wave_uniform((float)dyd);
+        const float dxy2 = wave_uniform(fmaf(dx, dx, dy * dy));
+        // tp: the column's in-plane closest approach to the voxel (any nearby
+        // value serves: r is exact for the tp used); the ray's points are
+        // S + (tp + s) d, at s d - r from the voxel centre
+        const float tp = wave_uniform((float)(wx * dxd + wy * dyd) * __builtin_amdgcn_rcpf(dxy2));
+        const float rx = wave_uniform((float)fma(-(double)tp, dxd, wx));
+        const float ry = wave_uniform((float)fma(-(double)tp, dyd, wy));
+        float xy0 = -INFINITY, xy1 = INFINITY;
+        if (dx != 0.f) {
+            const float inv = __builtin_amdgcn_rcpf(dx);
+            const float a0 = (rx - w.hw.x) * inv, a1 = (rx + w.hw.x) * inv;
+            xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
+        } else if (!(rx > -w.hw.x && rx <= w.hw.x)) {
+            xy1 = -INFINITY;
+        }
+        if (dy != 0.f) {
+            const float inv = __builtin_amdgcn_rcpf(dy);
+            const float a0 = (ry - w.hw.y) * inv, a1 = (ry + w.hw.y) * inv;
+            xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
+        } else if (!(ry > -w.hw.y && ry <= w.hw.y)) {
+            xy1 = -INFINITY;
+        }
+        xy0 = wave_uniform(fmaxf(xy0, -tp));          // t >= 0: from the source on
+        xy1 = wave_uniform(xy1);
+        if (!(xy1 > xy0)) continue;
+        const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
+        for (int iv = iv0; iv <= iv1; ++iv) {
+            const float dz = fmaf(Vv.z, (iv - w.cv) * a.dv, czs), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
+            const float len = sqrtf(fmaf(dz, dz, dxy2));
+            const float val = qc[(long)iv * nC];
+            const float inv = __builtin_amdgcn_rcpf(dz);
+            #pragma unroll
+            for (int k = 0; k < ZPL; ++k) {
+                const float rzk = fmaf((float)k, w.gdz, rz);
+                float s0 = xy0, s1 = xy1;
+                if (dz != 0.f) {
+                    const float a0 = (rzk - w.hw.z) * inv, a1 = (rzk + w.hw.z) * inv;
+                    s0 = fmaxf(s0, fminf(a0, a1)); s1 = fminf(s1, fmaxf(a0, a1));
+                } else if (!(rzk > -w.hw.z && rzk <= w.hw.z)) {
+                    continue;
+                }
+                if (!(s1 > s0)) continue;
+                if (s0 >= 1.f - tp) {
+                    // a voxel beyond the detector point: the ray counts only if
+                    // it entered the grid before that point
+                    float g0, g1;
+                    slab(Sv, F3{dx, dy, dz}, w.glo, w.ghi, g0, g1);
+                    if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
+                }
+                acc[k] = fmaf((s1 - s0) * len, val, acc[k]);
+            }
+        }
+    }
+}
+
+// ROWS_Z: the caller has found every view to be an upright cone view (the
+// kernel then has no general-view path: its FP64 ray set-up would double
+// the register count); ZPL voxels per lane stacked in z (ROWS_Z only).
+template <bool ROWS_Z, int ZPL>
 __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ prj,
                                                        float* __restrict__ img, const float* C, const float* U,
                                                        const float* V, const float* S, const float4* __restrict__ vt,
                                                        int nC)
 {
-    const int iz = blockIdx.x * 64 + (threadIdx.x & 63);
+    static_assert(ROWS_Z || ZPL == 1, "stacked voxels need upright cone views");
+    const int iz = (blockIdx.x * 64 + (threadIdx.x & 63)) * ZPL;
     const int iy = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (int)(threadIdx.x >> 6));
     const int ix = blockIdx.z % g.nx, bc = blockIdx.z / g.nx;
     if (iz >= g.nz || iy >= g.ny) return;
     const int cc = bc % nC, b = bc / nC;
     const float* q = prj + (long)b * a.nu * a.nview * a.nv * nC + cc;
     // wave-uniform values are kept in scalar registers (wave_uniform)
+    SidBpCol w;
     const float vlz = g.cz + (iz - g.nz / 2.0f) * g.dz;
-    const F3 vlo{wave_uniform(g.cx + (ix - g.nx / 2.0f) * g.dx), wave_uniform(g.cy + (iy - g.ny / 2.0f) * g.dy), vlz};
-    const F3 vhi{wave_uniform(vlo.x + g.dx), wave_uniform(vlo.y + g.dy), vlz + g.dz};
-    const F3 glo{wave_uniform(g.cx - g.nx / 2.0f * g.dx), wave_uniform(g.cy - g.ny / 2.0f * g.dy),
-                 wave_uniform(g.cz - g.nz / 2.0f * g.dz)};
-    const F3 ghi{wave_uniform(g.cx + g.nx / 2.0f * g.dx), wave_uniform(g.cy + g.ny / 2.0f * g.dy),
-                 wave_uniform(g.cz + g.nz / 2.0f * g.dz)};
+    w.vlo = F3{wave_uniform(g.cx + (ix - g.nx / 2.0f) * g.dx), wave_uniform(g.cy + (iy - g.ny / 2.0f) * g.dy), vlz};
+    w.vhi = F3{wave_uniform(w.vlo.x + g.dx), wave_uniform(w.vlo.y + g.dy), vlz + g.dz};
+    w.glo = F3{wave_uniform(g.cx - g.nx / 2.0f * g.dx), wave_uniform(g.cy - g.ny / 2.0f * g.dy),
+               wave_uniform(g.cz - g.nz / 2.0f * g.dz)};
+    w.ghi = F3{wave_uniform(g.cx + g.nx / 2.0f * g.dx), wave_uniform(g.cy + g.ny / 2.0f * g.dy),
+               wave_uniform(g.cz + g.nz / 2.0f * g.dz)};
     // the voxel centre in FP64 and its half widths: chords are measured from a
-    // point of the ray next to the voxel (slab_local), because parameters
-    // counted from the source (~600 mm away) lose ~5e-5 mm to float rounding
-    const double vcx = wave_uniform(g.cx + (ix + 0.5 - g.nx / 2.0) * g.dx);
-    const double vcy = wave_uniform(g.cy + (iy + 0.5 - g.ny / 2.0) * g.dy);
+    // point of the ray next to the voxel, because parameters counted from
+    // the source (~600 mm away) lose ~5e-5 mm to float rounding
+    w.vcx = wave_uniform(g.cx + (ix + 0.5 - g.nx / 2.0) * g.dx);
+    w.vcy = wave_uniform(g.cy + (iy + 0.5 - g.ny / 2.0) * g.dy);
     const double vcz = g.cz + (iz + 0.5 - g.nz / 2.0) * g.dz;
-    const F3 hw{wave_uniform(0.5f * g.dx), wave_uniform(0.5f * g.dy), wave_uniform(0.5f * g.dz)};
-    const float cu = wave_uniform((a.nu - 1) / 2.0f + a.off_u), cv = wave_uniform((a.nv - 1) / 2.0f + a.off_v);
+    w.hw = F3{wave_uniform(0.5f * g.dx), wave_uniform(0.5f * g.dy), wave_uniform(0.5f * g.dz)};
+    w.cu = wave_uniform((a.nu - 1) / 2.0f + a.off_u);
+    w.cv = wave_uniform((a.nv - 1) / 2.0f + a.off_v);
+    w.gdz = wave_uniform(g.dz);
     auto dot = [](const F3& p, const F3& r) { return p.x * r.x + p.y * r.y + p.z * r.z; };
-    float acc = 0.f;
+    float acc[ZPL];
+    #pragma unroll
+    for (int k = 0; k < ZPL; ++k) acc[k] = 0.f;
     for (int view = 0; view < a.nview; ++view) {
         const F3 Cv = ld3(C, view), Uv = ld3(U, view), Vv = ld3(V, view), Sv = ld3(S, view);
         const float4 t0v = vt[4 * view], t1v = vt[4 * view + 1], t2v = vt[4 * view + 2];
-        const F3 N{t0v.x, t0v.y, t0v.z}, Ud{t1v.x, t1v.y, t1v.z}, Vd{t2v.x, t2v.y, t2v.z};
-        // cone views whose detector rows run along z and columns lie in a
-        // horizontal plane (V = (0, 0, v), U.z = 0): N and Ud are horizontal
-        // and Vd = (0, 0, 1/v), so a voxel's detector columns depend on (x, y)
-        // only -- the column range and each column's in-plane ray direction
-        // are the wave's (the wave holds one (x, y)) -- and its rows scale
-        // with z: each row adds its z slab to the column's x and y slabs
         if (ROWS_Z || (!parallel && Vv.x == 0.f && Vv.y == 0.f && Uv.z == 0.f)) {
-            const float A = t0v.w, su = t1v.w, sv = t2v.w;
-            float umin = INFINITY, umax = -INFINITY, tmin = INFINITY, tmax = -INFINITY;
+            siddon_rows_z_view<ZPL>(a, w, view, Cv, Uv, Vv, Sv, t0v, t1v, t2v, vlz, vcz, q, nC, acc);
+            continue;
+        }
+        if constexpr (!ROWS_Z) {
+            // any other view: the voxel's eight corners on the detector (u, v in mm)
+            const F3 N{t0v.x, t0v.y, t0v.z}, Ud{t1v.x, t1v.y, t1v.z}, Vd{t2v.x, t2v.y, t2v.z};
+            const F3 vlo = w.vlo, vhi = w.vhi;
+            float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
             bool ok = true;
-            #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float wxk = ((k & 1) ? vhi.x : vlo.x) - Sv.x, wyk = ((k & 2) ? vhi.y : vlo.y) - Sv.y;
-                const float den = fmaf(wxk, N.x, wyk * N.y);
-                ok = ok && (den * A > 0.f);
-                const float t = A * __builtin_amdgcn_rcpf(den);
-                const float u = fmaf(t, fmaf(wxk, Ud.x, wyk * Ud.y), su);
-                umin = fminf(umin, u); umax = fmaxf(umax, u);
-                tmin = fminf(tmin, t); tmax = fmaxf(tmax, t);
+            float m = 0.f;
+            if (!parallel) {
+                const float A = t0v.w, su = t1v.w, sv = t2v.w;
+                #pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const F3 wk{((k & 1) ? vhi.x : vlo.x) - Sv.x, ((k & 2) ? vhi.y : vlo.y) - Sv.y,
+                                ((k & 4) ? vhi.z : vlo.z) - Sv.z};
+                    const float den = dot(wk, N);
+                    ok = ok && (den * A > 0.f);
+                    const float t = A * __builtin_amdgcn_rcpf(den);
+                    const float u = fmaf(t, dot(wk, Ud), su), v = fmaf(t, dot(wk, Vd), sv);
+                    umin = fminf(umin, u); umax = fmaxf(umax, u);
+                    vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+                }
+            } else {
+                const float sdn = t0v.w, sdu = t1v.w, sdv = t2v.w;
+                ok = sdn != 0.f;
+                const float isdn = -__builtin_amdgcn_rcpf(sdn);
+                #pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const F3 wk{((k & 1) ? vhi.x : vlo.x) - Cv.x, ((k & 2) ? vhi.y : vlo.y) - Cv.y,
+                                ((k & 4) ? vhi.z : vlo.z) - Cv.z};
+                    const float tau = dot(wk, N) * isdn;
+                    const float u = fmaf(tau, sdu, dot(wk, Ud)), v = fmaf(tau, sdv, dot(wk, Vd));
+                    umin = fminf(umin, u); umax = fmaxf(umax, u);
+                    vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+                }
+                m = vt[4 * view + 3].x;
             }
             if (!ok) continue;
-            const float z0 = (vlo.z - Sv.z) * Vd.z, z1 = (vhi.z - Sv.z) * Vd.z;
-            const float vmin = fminf(fminf(tmin * z0, tmax * z0), fminf(tmin * z1, tmax * z1)) + sv;
-            const float vmax = fmaxf(fmaxf(tmin * z0, tmax * z0), fmaxf(tmin * z1, tmax * z1)) + sv;
-            const float fu0 = umin / a.du + cu, fu1 = umax / a.du + cu, fv0 = vmin / a.dv + cv, fv1 = vmax / a.dv + cv;
+            // cells whose centre ray meets the corners' bounding box
+            const float fu0 = umin / a.du + w.cu, fu1 = umax / a.du + w.cu;
+            const float fv0 = vmin / a.dv + w.cv, fv1 = vmax / a.dv + w.cv;
             const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0);
             const int iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
             const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0);
             const int iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
-            const double wx = wave_uniform(vcx - Sv.x), wy = wave_uniform(vcy - Sv.y), wz = vcz - Sv.z;
-            // rows in float: the voxel's z offset from the source split in two
-            // floats, the detector centre's height above the source rounded once
-            const float wz_hi = (float)wz, wz_lo = (float)(wz - wz_hi), czs = (float)((double)Cv.z - Sv.z);
             for (int iu = iu0; iu <= iu1; ++iu) {
-                const double uc = (double)(iu - cu) * a.du;
-                const double dxd = wave_uniform(fma((double)Uv.x, uc, (double)Cv.x) - Sv.x);
-                const double dyd = wave_uniform(fma((double)Uv.y, uc, (double)Cv.y) - Sv.y);
-                const float dx = wave_uniform((float)dxd), dy = wave_uniform((float)dyd);
-                const float dxy2 = wave_uniform(fmaf(dx, dx, dy * dy));
-                // tp: the column's in-plane closest approach to the voxel (any
-                // nearby value serves: r is exact for the tp used)
-                const float tp = wave_uniform((float)(wx * dxd + wy * dyd) * __builtin_amdgcn_rcpf(dxy2));
-                const float rx = wave_uniform((float)fma(-(double)tp, dxd, wx));
-                const float ry = wave_uniform((float)fma(-(double)tp, dyd, wy));
-                float xy0 = -INFINITY, xy1 = INFINITY;
-                if (dx != 0.f) {
-                    const float inv = __builtin_amdgcn_rcpf(dx);
-                    const float a0 = (rx - hw.x) * inv, a1 = (rx + hw.x) * inv;
-                    xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
-                } else if (!(rx > -hw.x && rx <= hw.x)) {
-                    xy1 = -INFINITY;
-                }
-                if (dy != 0.f) {
-                    const float inv = __builtin_amdgcn_rcpf(dy);
-                    const float a0 = (ry - hw.y) * inv, a1 = (ry + hw.y) * inv;
-                    xy0 = fmaxf(xy0, fminf(a0, a1)); xy1 = fminf(xy1, fmaxf(a0, a1));
-                } else if (!(ry > -hw.y && ry <= hw.y)) {
-                    xy1 = -INFINITY;
-                }
-                xy0 = wave_uniform(xy0);
-                xy1 = wave_uniform(xy1);
-                if (!(xy1 > fmaxf(xy0, -tp))) continue;
+                const double uc = (double)(iu - w.cu) * a.du;
                 const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
                 for (int iv = iv0; iv <= iv1; ++iv) {
-                    const float dz = fmaf(Vv.z, (iv - cv) * a.dv, czs), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
-                    float s0 = xy0, s1 = xy1;
-                    if (dz != 0.f) {
-                        const float inv = __builtin_amdgcn_rcpf(dz);
-                        const float a0 = (rz - hw.z) * inv, a1 = (rz + hw.z) * inv;
-                        s0 = fmaxf(s0, fminf(a0, a1)); s1 = fminf(s1, fmaxf(a0, a1));
-                    } else if (!(rz > -hw.z && rz <= hw.z)) {
-                        continue;
-                    }
+                    const double vc = (double)(iv - w.cv) * a.dv;
+                    const double dst[3] = {Cv.x + Uv.x * uc + Vv.x * vc, Cv.y + Uv.y * uc + Vv.y * vc,
+                                           Cv.z + Uv.z * uc + Vv.z * vc};
+                    const double o[3] = {parallel ? dst[0] + (double)Sv.x * m : (double)Sv.x,
+                                         parallel ? dst[1] + (double)Sv.y * m : (double)Sv.y,
+                                         parallel ? dst[2] + (double)Sv.z * m : (double)Sv.z};
+                    const double dd[3] = {dst[0] - o[0], dst[1] - o[1], dst[2] - o[2]};
+                    const double wv[3] = {w.vcx - o[0], w.vcy - o[1], vcz - o[2]};
+                    const float tp = (float)((wv[0] * dd[0] + wv[1] * dd[1] + wv[2] * dd[2]) /
+                                             (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]));
+                    const F3 r{(float)fma(-(double)tp, dd[0], wv[0]), (float)fma(-(double)tp, dd[1], wv[1]),
+                               (float)fma(-(double)tp, dd[2], wv[2])};
+                    const F3 d{(float)dd[0], (float)dd[1], (float)dd[2]};
+                    float s0, s1;
+                    slab_local(r, d, w.hw, s0, s1);
                     s0 = fmaxf(s0, -tp);
                     if (!(s1 > s0)) continue;
                     if (s0 >= 1.f - tp) {
                         // a voxel beyond the detector point: the ray counts only
                         // if it entered the grid before that point
+                        const F3 of{(float)o[0], (float)o[1], (float)o[2]};
                         float g0, g1;
-                        slab(Sv, F3{dx, dy, dz}, glo, ghi, g0, g1);
+                        slab(of, d, w.glo, w.ghi, g0, g1);
                         if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
                     }
-                    acc = fmaf((s1 - s0) * sqrtf(fmaf(dz, dz, dxy2)), qc[(long)iv * nC], acc);
+                    acc[0] = fmaf((s1 - s0) * sqrtf(dot(d, d)), qc[(long)iv * nC], acc[0]);
                 }
-            }
-            continue;
-        }
-        // any other view (not instantiated when the caller has found every
-        // view to be of the kind above: the FP64 ray set-up below would
-        // double the register count): the voxel's eight corners on the
-        // detector (u, v in mm)
-        if constexpr (ROWS_Z) continue;
-        float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
-        bool ok = true;
-        float m = 0.f;
-        if (!parallel) {
-            const float A = t0v.w, su = t1v.w, sv = t2v.w;
-            #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const F3 w{((k & 1) ? vhi.x : vlo.x) - Sv.x, ((k & 2) ? vhi.y : vlo.y) - Sv.y,
-                           ((k & 4) ? vhi.z : vlo.z) - Sv.z};
-                const float den = dot(w, N);
-                ok = ok && (den * A > 0.f);
-                const float t = A * __builtin_amdgcn_rcpf(den);
-                const float u = fmaf(t, dot(w, Ud), su), v = fmaf(t, dot(w, Vd), sv);
-                umin = fminf(umin, u); umax = fmaxf(umax, u);
-                vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
-            }
-        } else {
-            const float sdn = t0v.w, sdu = t1v.w, sdv = t2v.w;
-            ok = sdn != 0.f;
-            const float isdn = -__builtin_amdgcn_rcpf(sdn);
-            #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const F3 w{((k & 1) ? vhi.x : vlo.x) - Cv.x, ((k & 2) ? vhi.y : vlo.y) - Cv.y,
-                           ((k & 4) ? vhi.z : vlo.z) - Cv.z};
-                const float tau = dot(w, N) * isdn;
-                const float u = fmaf(tau, sdu, dot(w, Ud)), v = fmaf(tau, sdv, dot(w, Vd));
-                umin = fminf(umin, u); umax = fmaxf(umax, u);
-                vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
-            }
-            m = vt[4 * view + 3].x;
-        }
-        if (!ok) continue;
-        // cells whose centre ray meets the corners' bounding box
-        const float fu0 = umin / a.du + cu, fu1 = umax / a.du + cu, fv0 = vmin / a.dv + cv, fv1 = vmax / a.dv + cv;
-        const int iu0 = max((int)ceilf(fminf(fu0, fu1) - 1e-3f), 0), iu1 = min((int)floorf(fmaxf(fu0, fu1) + 1e-3f), a.nu - 1);
-        const int iv0 = max((int)ceilf(fminf(fv0, fv1) - 1e-3f), 0), iv1 = min((int)floorf(fmaxf(fv0, fv1) + 1e-3f), a.nv - 1);
-        for (int iu = iu0; iu <= iu1; ++iu) {
-            const double uc = (double)(iu - cu) * a.du;
-            const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
-            for (int iv = iv0; iv <= iv1; ++iv) {
-                const double vc = (double)(iv - cv) * a.dv;
-                const double dst[3] = {Cv.x + Uv.x * uc + Vv.x * vc, Cv.y + Uv.y * uc + Vv.y * vc,
-                                       Cv.z + Uv.z * uc + Vv.z * vc};
-                const double o[3] = {parallel ? dst[0] + (double)Sv.x * m : (double)Sv.x,
-                                     parallel ? dst[1] + (double)Sv.y * m : (double)Sv.y,
-                                     parallel ? dst[2] + (double)Sv.z * m : (double)Sv.z};
-                const double dd[3] = {dst[0] - o[0], dst[1] - o[1], dst[2] - o[2]};
-                const double w[3] = {vcx - o[0], vcy - o[1], vcz - o[2]};
-                const float tp = (float)((w[0] * dd[0] + w[1] * dd[1] + w[2] * dd[2]) /
-                                         (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]));
-                const F3 r{(float)fma(-(double)tp, dd[0], w[0]), (float)fma(-(double)tp, dd[1], w[1]),
-                           (float)fma(-(double)tp, dd[2], w[2])};
-                const F3 d{(float)dd[0], (float)dd[1], (float)dd[2]};
-                float s0, s1;
-                slab_local(r, d, hw, s0, s1);
-                s0 = fmaxf(s0, -tp);
-                if (!(s1 > s0)) continue;
-                if (s0 >= 1.f - tp) {
-                    // a voxel beyond the detector point: the ray counts only if it
-                    // entered the grid before that point
-                    const F3 of{(float)o[0], (float)o[1], (float)o[2]};
-                    float g0, g1;
-                    slab(of, d, glo, ghi, g0, g1);
-                    if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
-                }
-                acc = fmaf((s1 - s0) * sqrtf(dot(d, d)), qc[(long)iv * nC], acc);
             }
         }
     }
-    img[((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc] = acc;
+    float* out = img + ((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc;
+    #pragma unroll
+    for (int k = 0; k < ZPL; ++k)
+        if (iz + k < g.nz) out[(long)k * nC] = acc[k];
 }
 
 // ---- DD tomosynthesis ----
@@ -692,12 +731,13 @@ __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const flo
 // z(v) = alpha + beta v (the forward pass's z-edge crossings), which gives
 // each lane its candidate rows and every row its footprint with a few FMAs.
 // Reference layouts.
+template <int ZPL>
 __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const float* __restrict__ prj,
                                                      float* __restrict__ img, const float* Cp, const float* Up,
                                                      const float* Vp, const float* Sp, const int* __restrict__ cls,
                                                      int nC)
 {
-    const int iz = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iz = (blockIdx.x * 64 + (threadIdx.x & 63)) * ZPL;
     const int iy = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (int)(threadIdx.x >> 6));
     const int ix = blockIdx.z % g.nx, bc = blockIdx.z / g.nx;
     if (iz >= g.nz || iy >= g.ny) return;
@@ -707,7 +747,9 @@ __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const f
     const float margin = 0.05f;
     const float fz = (float)iz;
     const float idu = 1.0f / a.du, idv = 1.0f / a.dv, idz = 1.0f / g.dz;
-    float acc = 0.f;
+    float acc[ZPL];
+    #pragma unroll
+    for (int k = 0; k < ZPL; ++k) acc[k] = 0.f;
     for (int view = 0; view < a.nview; ++view) {
         const bool xv = cls[view] == 0;
         const F3 c = ld3(Cp, view), U = ld3(Up, view), V = ld3(Vp, view), s = ld3(Sp, view);
@@ -762,7 +804,7 @@ __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const f
             const float beta = mp * V.z * idpc * idz;
             if (!(fabsf(beta) > 0.f)) continue;
             const float ib = __builtin_amdgcn_rcpf(beta);
-            const float va = (fz - alpha) * ib, vb = (fz + 1.0f - alpha) * ib;
+            const float va = (fz - alpha) * ib, vb = (fz + (float)ZPL - alpha) * ib;
             const float glo = fmaxf(fminf(va, vb) * idv + cv - 0.5f - margin, -1.0f);
             const float ghi = fminf(fmaxf(va, vb) * idv + cv + 0.5f + margin, (float)a.nv);
             if (!(ghi >= glo)) continue;
@@ -774,15 +816,20 @@ __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const f
                 const float vc = ((float)iv - cv) * a.dv;
                 const float za = fmaf(beta, vc - hb, alpha), zb = fmaf(beta, vc + hb, alpha);
                 const float z1 = fminf(za, zb), z2 = fmaxf(za, zb);
-                const float oz = fminf(z2, fz + 1.0f) - fmaxf(z1, fz);
-                if (!(oz > 0.f)) continue;
                 const float lz = fmaf(vc, V.z, zc0);
-                acc = fmaf(q[(((long)iu * a.nview + view) * a.nv + iv) * nC], wcol * oz * sqrtf(fmaf(lz, lz, lxy2)),
-                           acc);
+                const float wv = wcol * sqrtf(fmaf(lz, lz, lxy2)) * q[(((long)iu * a.nview + view) * a.nv + iv) * nC];
+                #pragma unroll
+                for (int k = 0; k < ZPL; ++k) {
+                    const float oz = fminf(z2, fz + (float)(k + 1)) - fmaxf(z1, fz + (float)k);
+                    if (oz > 0.f) acc[k] = fmaf(oz, wv, acc[k]);
+                }
             }
         }
     }
-    img[((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc] = acc;
+    float* out = img + ((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc;
+    #pragma unroll
+    for (int k = 0; k < ZPL; ++k)
+        if (iz + k < g.nz) out[(long)k * nC] = acc[k];
 }
 
 // ---- cone OS-SQS helpers ----
@@ -969,11 +1016,23 @@ void siddon_arb_bp(const ArbGeo& g, bool parallel, int nB, int nC, const float* 
     hipLaunchKernelGGL(k_siddon_arb_views, dim3(cdiv(g.nview, 256)), dim3(256), 0, st, g, grid_of(g),
                        parallel ? 1 : 0, detCenter, detU, detV, srcOrDir, vt);
     check_launch("siddon_arb_views");
-    dim3 blk(256), grd(cdiv(g.nz, 64), cdiv(g.ny, 4), g.nx * nB * nC);
-    hipLaunchKernelGGL(!parallel && g.rows_z ? k_siddon_arb_bp<true> : k_siddon_arb_bp<false>, grd, blk, 0, st, g,
-                       grid_of(g), parallel ? 1 : 0, prj, img, detCenter, detU,
-                       detV, srcOrDir, vt, nC);
-    check_launch("siddon_arb_bp");
+    if (parallel || !g.rows_z) {
+        dim3 grd(cdiv(g.nz, 64), cdiv(g.ny, 4), g.nx * nB * nC);
+        hipLaunchKernelGGL((k_siddon_arb_bp<false, 1>), grd, dim3(256), 0, st, g, grid_of(g), parallel ? 1 : 0, prj, img,
+                           detCenter, detU, detV, srcOrDir, vt, nC);
+        check_launch("siddon_arb_bp");
+        return;
+    }
+    // upright cone views: 1, 2 or 4 voxels per lane (the launch-shape tuner)
+    auto launch = [&](int c) {
+        const int zpl = 1 << c;
+        dim3 grd(cdiv(g.nz, 64 * zpl), cdiv(g.ny, 4), g.nx * nB * nC);
+        auto k = c == 0 ? k_siddon_arb_bp<true, 1> : c == 1 ? k_siddon_arb_bp<true, 2> : k_siddon_arb_bp<true, 4>;
+        hipLaunchKernelGGL(k, grd, dim3(256), 0, st, g, grid_of(g), 0, prj, img, detCenter, detU, detV, srcOrDir, vt, nC);
+        check_launch("siddon_arb_bp");
+    };
+    const int c = tune_choose({TK_SIDDON_CONE_BP, g.nx * g.ny, g.nz, g.nu * g.nv, g.nview, nB * nC}, 3, launch, st, true);
+    launch(c);
 }
 
 void dd_tomo_fp(const ArbGeo& g, int nB, int nC, const float* img, float* prj,
@@ -1072,10 +1131,17 @@ void ConePlan::dd_adj(int nB, int nC, const float* prj, float* img, hipStream_t 
     const ArbGeo& g = geo;
     RN_REQUIRE(nB > 0 && nC > 0 && (long)g.nx * nB * nC <= 65535, "cone batch too large for one launch");
     const float* v = vec_.as<float>();
-    dim3 blk(256), grd(cdiv(g.nz, 64), cdiv(g.ny, 4), g.nx * nB * nC);
-    hipLaunchKernelGGL(k_dd_cone_adj, grd, blk, 0, st, g, grid_of(g), prj, img, v, v + 3 * g.nview, v + 6 * g.nview,
-                       v + 9 * g.nview, cls_.as<int>(), nC);
-    check_launch("dd_cone_adj");
+    // 1, 2 or 4 voxels per lane (the launch-shape tuner)
+    auto launch = [&](int c) {
+        const int zpl = 1 << c;
+        dim3 grd(cdiv(g.nz, 64 * zpl), cdiv(g.ny, 4), g.nx * nB * nC);
+        auto k = c == 0 ? k_dd_cone_adj<1> : c == 1 ? k_dd_cone_adj<2> : k_dd_cone_adj<4>;
+        hipLaunchKernelGGL(k, grd, dim3(256), 0, st, g, grid_of(g), prj, img, v, v + 3 * g.nview, v + 6 * g.nview,
+                           v + 9 * g.nview, cls_.as<int>(), nC);
+        check_launch("dd_cone_adj");
+    };
+    const int c = tune_choose({TK_DD_CONE_ADJ, g.nx * g.ny, g.nz, g.nu * g.nv, g.nview, nB * nC}, 3, launch, st, true);
+    launch(c);
 }
 
 void ConePlan::siddon_fp(int nB, int nC, const float* img, float* prj, DevBuf& work, hipStream_t st) const

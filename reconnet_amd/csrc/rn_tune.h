@@ -46,6 +46,8 @@ enum TuneKernel {
     TK_HELICAL_BP = 15,
     TK_DD_CONE_FP = 16,
     TK_PIXEL_BP = 17,
+    TK_SIDDON_CONE_BP = 18,
+    TK_DD_CONE_ADJ = 19,
 };
 
 // Returns the candidate index to launch for `key`; runs the timing when the

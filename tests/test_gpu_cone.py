@@ -46,6 +46,39 @@ def test_dd_cone_matches_oracle(nC, off):
     assert rel_l2(b, orc.dd_cone_adj(p, *vecs, geo, (n, n, nz))) <= TOL
 
 
+@pytest.mark.parametrize("kind,kernel,ncand", [("dd", "dd_cone_fp", 2), ("dd", "dd_cone_adj", 3),
+                                               ("siddon", "siddon_cone_bp", 3)])
+def test_every_cone_launch_shape_matches_oracle(kind, kernel, ncand):
+    # each launch shape of the tuned cone kernels (XCD-ordered or plain grid;
+    # 1, 2 or 4 voxels per lane stacked in z, with a z extent that leaves the
+    # last lane's stack partly outside the volume), two channels and an
+    # off-centre grid, against the FP64 oracle
+    from reconnet_amd._lib import autotune_pin
+
+    n, nz, nu, nv, nview = 20, 13, 36, 18, 24
+    vecs = circ_cone(nview, 120.0, 200.0, th0=0.1)
+    geo = dict(dx=1.0, dy=1.0, dz=1.0, du=1.3, dv=1.1, off_u=0.2, off_v=-0.3, cx=0.4, cy=-0.3, cz=0.2)
+    plan = _plan(n, nz, nu, nv, 1.3, 1.1, vecs, off_u=0.2, off_v=-0.3, cx=0.4, cy=-0.3, cz=0.2)
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0, 1, (1, n, n, nz, 2)).astype(np.float32)
+    p = rng.uniform(0, 1, (1, nu, nview, nv, 2)).astype(np.float32)
+    if kernel == "dd_cone_fp":
+        ref, run = orc.dd_cone_fp(x, *vecs, geo, nu, nv), lambda: plan.forward(torch.from_numpy(x).cuda(), kind)
+    else:
+        bp = orc.dd_cone_adj if kind == "dd" else orc.siddon_cone_bp
+        ref, run = bp(p, *vecs, geo, (n, n, nz)), lambda: plan.backward(torch.from_numpy(p).cuda(), kind)
+    assert np.abs(ref).max() > 1.0
+    try:
+        for c in range(ncand):
+            autotune_pin(kernel, c)
+            assert rel_l2(run().cpu().numpy(), ref) <= TOL, c
+        autotune_pin(kernel, ncand)
+        with pytest.raises(RuntimeError, match="out of range"):
+            run()
+    finally:
+        autotune_pin(kernel, None)
+
+
 @pytest.mark.parametrize("kind", ["dd", "siddon"])
 def test_cone_full_size_views_match_oracle(kind):
     # the 256^2 x 128 volume at 1 mm (seeded ellipsoid phantom), 64 x 512
