@@ -682,13 +682,56 @@ __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const flo
     const int view = bz % a.nview, bc = bz / a.nview;
     if (iv >= a.nv) return;
     const bool xv = cls[view] == 0;
-    const ConeRay r = cone_ray(a, g, ld3(Cp, view), ld3(Up, view), ld3(Vp, view), ld3(Sp, view), xv, iu, iv);
+    ConeRay r = cone_ray(a, g, ld3(Cp, view), ld3(Up, view), ld3(Vp, view), ld3(Sp, view), xv, iu, iv);
     const int nm = xv ? g.nx : g.ny, nn = xv ? g.ny : g.nx;
     const long sP = xv ? (long)g.ny * g.nz : g.nz, sJ = xv ? g.nz : (long)g.ny * g.nz;
     const int c = bc % nC, b = bc / nC;
     const float* base = vol + (long)b * g.nx * g.ny * g.nz * nC + c;   // reference layout, channel innermost
     float acc = 0.f;
-    if (r.ok) {
+    // the in-plane edges do not depend on the row (detV = (0, 0, v)): the
+    // wave's, in scalar registers
+    #pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        r.a[e] = wave_uniform(r.a[e]);
+        r.b[e] = wave_uniform(r.b[e]);
+    }
+    // footprints at most one cell wide and high on every plane (the widths
+    // are affine in the plane index: the end planes bound them): at most 2 x
+    // 2 cells, taken without loops
+    const float wl = nm - 1.0f;
+    const bool small = fmaxf(fabsf(r.a[1] - r.a[0]), fabsf(fmaf(r.b[1] - r.b[0], wl, r.a[1] - r.a[0]))) <= 1.0f &&
+                       fmaxf(fabsf(r.a[3] - r.a[2]), fabsf(fmaf(r.b[3] - r.b[2], wl, r.a[3] - r.a[2]))) <= 1.0f;
+    if (r.ok && small) {
+        // the two edges of a pair never cross between planes (the footprint
+        // would vanish): order each pair once
+        const bool se = r.a[0] > r.a[1], sz = r.a[2] > r.a[3];
+        const float ea1 = se ? r.a[1] : r.a[0], eb1 = se ? r.b[1] : r.b[0];
+        const float ea2 = se ? r.a[0] : r.a[1], eb2 = se ? r.b[0] : r.b[1];
+        const float za1 = sz ? r.a[3] : r.a[2], zb1 = sz ? r.b[3] : r.b[2];
+        const float za2 = sz ? r.a[2] : r.a[3], zb2 = sz ? r.b[2] : r.b[3];
+        for (int p = 0; p < nm; ++p) {
+            const float fp = (float)p;
+            const float e1 = fmaf(eb1, fp, ea1), e2 = fmaf(eb2, fp, ea2);
+            const float le = fmaxf(e1, 0.f), he = fminf(e2, (float)nn);
+            if (!(he > le)) continue;
+            const float z1 = fmaf(zb1, fp, za1), z2 = fmaf(zb2, fp, za2);
+            const float lz = fmaxf(z1, 0.f), hz = fminf(z2, (float)g.nz);
+            const float area = (e2 - e1) * (z2 - z1);
+            if (!(hz > lz) || !(area > 0.f)) continue;
+            // cells j0, j0 + 1 and k0, k0 + 1; a second cell's share is 0 when
+            // the footprint ends in the first (its index is then clamped)
+            const int j0 = (int)le, k0 = (int)lz;
+            const float oe0 = fminf(he, (float)(j0 + 1)) - le, oe1 = fmaxf(he - (float)(j0 + 1), 0.f);
+            const float oz0 = fminf(hz, (float)(k0 + 1)) - lz, oz1 = fmaxf(hz - (float)(k0 + 1), 0.f);
+            const int j1 = min(j0 + 1, nn - 1), k1 = min(k0 + 1, g.nz - 1);
+            const float* P = base + p * sP * nC;
+            const float v00 = P[(j0 * sJ + k0) * nC], v01 = P[(j0 * sJ + k1) * nC];
+            const float v10 = P[(j1 * sJ + k0) * nC], v11 = P[(j1 * sJ + k1) * nC];
+            const float sum = fmaf(fmaf(v01, oz1, v00 * oz0), oe0, fmaf(v11, oz1, v10 * oz0) * oe1);
+            acc = fmaf(sum, __builtin_amdgcn_rcpf(area), acc);
+        }
+        acc *= r.L;
+    } else if (r.ok) {
         for (int p = 0; p < nm; ++p) {
             const float fp = (float)p;
             float e1 = fmaf(r.b[0], fp, r.a[0]), e2 = fmaf(r.b[1], fp, r.a[1]);

@@ -31,12 +31,16 @@ def _plan(n, nz, nu, nv, du, dv, vecs, **kw):
     return ConePlan(n, n, nz, 1.0, 1.0, 1.0, nu, nv, du, dv, *vecs, **kw)
 
 
-@pytest.mark.parametrize("nC,off", [(1, 0.0), (2, 0.3)])
-def test_dd_cone_matches_oracle(nC, off):
+@pytest.mark.parametrize("nC,off,du,dv", [(1, 0.0, 1.3, 1.1), (2, 0.3, 1.3, 1.1), (1, 0.2, 2.6, 2.3),
+                                          (2, -0.1, 1.3, 2.3)])
+def test_dd_cone_matches_oracle(nC, off, du, dv):
+    # cells of 1.3 x 1.1 project to footprints under one voxel on every plane
+    # (the forward's 2 x 2-cell path); 2.6 x 2.3 and 1.3 x 2.3 to larger ones
+    # (its general loop)
     n, nz, nu, nv, nview = 24, 12, 40, 20, 30
     vecs = circ_cone(nview, 120.0, 200.0, th0=0.2)
-    geo = dict(dx=1.0, dy=1.0, dz=1.0, du=1.3, dv=1.1, off_u=off, off_v=-off, cx=0.4, cy=-0.3, cz=0.2)
-    plan = _plan(n, nz, nu, nv, 1.3, 1.1, vecs, off_u=off, off_v=-off, cx=0.4, cy=-0.3, cz=0.2)
+    geo = dict(dx=1.0, dy=1.0, dz=1.0, du=du, dv=dv, off_u=off, off_v=-off, cx=0.4, cy=-0.3, cz=0.2)
+    plan = _plan(n, nz, nu, nv, du, dv, vecs, off_u=off, off_v=-off, cx=0.4, cy=-0.3, cz=0.2)
     rng = np.random.default_rng(3)
     x = rng.uniform(0, 1, (1, n, n, nz, nC)).astype(np.float32)
     y = plan.forward(torch.from_numpy(x).cuda(), "dd").cpu().numpy()
