@@ -114,53 +114,6 @@ __global__ __launch_bounds__(256) void k_conv_in(const float* __restrict__ x, co
     }
 }
 
-// The three column-shifted B fragments (pixels xpix - 1, xpix, xpix + 1 of
-// row yy, 8 channels per lane) of a 16-pixel column segment.  Only the centre
-// fragment is loaded by every lane; the shifted ones are the same registers
-// moved one lane along each 16-lane DPP row (a row is one channel group), and
-// the one pixel each shift brings in from outside the segment is loaded by the
-// 4 edge lanes alone.  Pixels outside the image are zero.
-__device__ __forceinline__ uint32_t dpp_from_lower(uint32_t src, uint32_t edge)
-{
-    // lane i takes lane i - 1 of its row; lane 0 of the row keeps `edge`
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)src, 0x111, 0xf, 0xf, false);
-}
-__device__ __forceinline__ uint32_t dpp_from_upper(uint32_t src, uint32_t edge)
-{
-    // lane i takes lane i + 1 of its row; lane 15 of the row keeps `edge`
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)src, 0x101, 0xf, 0xf, false);
-}
-
-// raw loads of one input row: the centre fragment and the two edge pixels
-struct RowLoad {
-    uint4 c, el, er;
-};
-
-__device__ __forceinline__ RowLoad row_load(const uint16_t* __restrict__ in, long img, int yy, int H, int W, int xpix,
-                                            int r16, int g4)
-{
-    RowLoad r;
-    r.c = r.el = r.er = make_uint4(0, 0, 0, 0);
-    if (yy >= 0 && yy < H) {
-        const uint16_t* row = in + (img + (long)yy * W) * 32 + 8 * g4;
-        if (xpix < W) r.c = *reinterpret_cast<const uint4*>(row + (long)xpix * 32);
-        if (r16 == 0 && xpix >= 1 && xpix - 1 < W) r.el = *reinterpret_cast<const uint4*>(row + (long)(xpix - 1) * 32);
-        if (r16 == 15 && xpix + 1 < W) r.er = *reinterpret_cast<const uint4*>(row + (long)(xpix + 1) * 32);
-    }
-    return r;
-}
-
-__device__ __forceinline__ void row_fragments(const RowLoad& r, bf16x8 (&f)[3])
-{
-    const uint4 lo = make_uint4(dpp_from_lower(r.c.x, r.el.x), dpp_from_lower(r.c.y, r.el.y),
-                                dpp_from_lower(r.c.z, r.el.z), dpp_from_lower(r.c.w, r.el.w));
-    const uint4 hi = make_uint4(dpp_from_upper(r.c.x, r.er.x), dpp_from_upper(r.c.y, r.er.y),
-                                dpp_from_upper(r.c.z, r.er.z), dpp_from_upper(r.c.w, r.er.w));
-    f[0] = as_bf16x8(lo);
-    f[1] = as_bf16x8(r.c);
-    f[2] = as_bf16x8(hi);
-}
-
 // ---------------------------------------------------------------------------
 // 32 -> 32 layer on MFMA: v_mfma_f32_32x32x16_bf16, which on gfx950 moves
 // twice the flops per cycle of the 16x16x32 form (measured 2.17 against 1.22
