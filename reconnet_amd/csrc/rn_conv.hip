@@ -422,11 +422,11 @@ __global__ __launch_bounds__(256) void k_conv_out(const uint16_t* __restrict__ a
 //
 // A workgroup computes a strip of 16 T computed columns (T tiles of 16
 // pixels) over L output rows of one image, walking down the rows.  Each of
-// its 4 waves owns layers: wave 0 the 1 -> 32 layer (VALU, 8 channels of a
-// pixel per lane) and the 32 -> 1 output layer (MFMA), wave k = 1, 2, 3 the 32 -> 32
-// layer k + 1 (MFMA, its 72 VGPRs of weight fragments resident).  They run
-// as a row pipeline: at step s wave 0 writes a1 row y0 - 4 + s and wave k
-// writes row y0 - 4 - 2k + s of its layer, each consuming the row its
+// its 4 waves owns layers: wave 0 the x rows and the 32 -> 1 output layer
+// (MFMA), wave k = 1, 2, 3 the 32 -> 32 layer k + 1 (MFMA, its 72 VGPRs of
+// weight fragments resident); wave 0 also runs the 1 -> 32 layer, on FP32
+// MFMA.  They run as a row pipeline: at step s wave 0 writes a1 row
+// y0 - 4 + s and wave k writes row y0 - 4 - 2k + s of its layer, each consuming the row its
 // producer wrote at step s - 1 from a double-buffered LDS row (bf16,
 // [channel group][pixel], 16 bytes per entry; pitch = 4 (mod 16) entries
 // keeps the fragment reads and the row writes free of bank conflicts); one
@@ -446,42 +446,97 @@ struct FusedCnn {
     static constexpr int WO = 16 * T - 8;                         // output columns of a strip
 };
 
-template <int T, int L>
+template <int T>
 __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x, const float* __restrict__ w1,
                                                    const float* __restrict__ b1, const uint16_t* __restrict__ w2,
                                                    const float* __restrict__ b2, const uint16_t* __restrict__ w3,
                                                    const float* __restrict__ b3, const uint16_t* __restrict__ w4,
                                                    const float* __restrict__ b4, const float* __restrict__ w5, float b5,
-                                                   float* __restrict__ y, int H, int W)
+                                                   float* __restrict__ y, int H, int W, int nstrips, long total,
+                                                   long per)
 {
     using F = FusedCnn<T>;
     static_assert(T % 2 == 0, "32-pixel tiles of the 32 -> 32 layers");
     constexpr int NPP = F::NPP, NP = F::NP;
+    constexpr int KP = (16 * T + 63) / 64;   // layer-1 pixels per lane
     __shared__ uint4 act[4][2][4 * NPP];     // rows of a1 .. a4
-    __shared__ float xr[3][NP];              // x rows y - 1, y, y + 1 of the a1 row (wave 0)
+    __shared__ float xr[4][NP];              // x ring: rows y - 1, y, y + 1 of the a1 row and the next
     __shared__ float ur[3][NP * 18];         // U rows of the output layer (wave 0)
     const int t = threadIdx.x;
     const int lane = t & 63;
-    const int wv = t >> 6;
-    const int r16 = lane & 15, g4 = lane >> 4;
-    const int c0 = blockIdx.x * F::WO - 4;   // first computed column
-    const int y0 = blockIdx.y * L;
-    const long img = (long)blockIdx.z * H * W;
-    for (int i = t; i < 4 * 2 * 4 * NPP; i += 256) (&act[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    // The workgroup's rows: [blockIdx.x per, + per) of the (image, strip, row)
+    // sequence, walked as segments of consecutive rows of one strip (a new
+    // segment refills the pipeline).  Every wave runs the same segments.
+    long i0 = (long)blockIdx.x * per;
+    const long i1 = i0 + per < total ? i0 + per : total;
+    int c0 = 0, y0 = 0, L = 0;               // first computed column, first row, rows of the segment
+    long img = 0;
+    auto next_segment = [&]() {
+        if (i0 >= i1) return false;
+        const long si = i0 / H;              // image * nstrips + strip
+        y0 = (int)(i0 - si * H);
+        L = (int)(i1 - i0 < (long)(H - y0) ? i1 - i0 : (long)(H - y0));
+        c0 = (int)(si % nstrips) * F::WO - 4;
+        img = (si / nstrips) * (long)H * W;
+        i0 += L;
+        __syncthreads();                     // the previous segment's last reads are done
+        for (int i = t; i < 4 * 2 * 4 * NPP; i += 256) (&act[0][0][0])[i] = make_uint4(0, 0, 0, 0);
+        return true;
+    };
 
+    // The 1 -> 32 layer on wave 0's matrix core, whose SIMD the 32 -> 32 layers
+    // leave idle: per 32-pixel tile five v_mfma_f32_32x32x2f32 add taps 2 i
+    // and 2 i + 1 to an accumulator that starts at the bias (tap 9 has weight
+    // 0).  This is k_conv_in's arithmetic (z = b; z = fma(w[t], v[t], z) for
+    // t = 0..8): the FP32 MFMA adds its two products in order with one
+    // rounding each, the same bits as the FMA chain.  A lane holds w1[tap
+    // 2 i + l / 32][co = l % 32] and the x value of that tap at pixel l % 32;
+    // D is laid out as the 32 -> 32 layers' (channel 8 (r / 4) + 4 (l / 32) +
+    // r % 4 of pixel l % 32 in register r).  At step s the a1 row y0 - 4 + s
+    // uses x rows y0 - 5 + s .. y0 - 3 + s, in ring slots s % 4 .. (s + 2) % 4;
+    // x row y0 - 2 + s goes into slot (s + 3) % 4 for the next step.
     if (wv == 0) {
-        // ---- 1 -> 32 (VALU) and 32 -> 1 (1x1 MFMA + 3x3 sum, as k_conv_out) ----
+        // ---- x rows, the 1 -> 32 layer and 32 -> 1 (1x1 MFMA + 3x3 sum, as k_conv_out) ----
+        float wa1[5], ba1[16];
+        {
+            const int co = lane & 31, hk = lane >> 5;
+            #pragma unroll
+            for (int i = 0; i < 5; ++i) wa1[i] = 2 * i + hk < 9 ? w1[(2 * i + hk) * 32 + co] : 0.0f;
+            #pragma unroll
+            for (int r = 0; r < 16; ++r) ba1[r] = b1[8 * (r >> 2) + 4 * hk + (r & 3)];
+        }
+        auto conv_in_row = [&](int s) {
+            const int r1 = y0 - 4 + s;
+            uint2* a1w = reinterpret_cast<uint2*>(act[0][s & 1]);
+            const int n = lane & 31, hk = lane >> 5;
+            #pragma unroll
+            for (int j = 0; j < T / 2; ++j) {
+                const int pp = 32 * j + n;
+                f32x16 z;
+                #pragma unroll
+                for (int r = 0; r < 16; ++r) z[r] = ba1[r];
+                #pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    const int tp = 2 * i + hk;           // tap 3 dy + dx
+                    const float v = tp < 9 ? xr[(s + tp / 3) & 3][pp + tp % 3] : 0.0f;
+                    z = __builtin_amdgcn_mfma_f32_32x32x2f32(wa1[i], v, z, 0, 0, 0);
+                }
+                const int c = c0 + pp;
+                const bool ok = r1 >= 0 && r1 < H && c >= 0 && c < W;
+                #pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint2 ov = make_uint2(0, 0);
+                    if (ok) {
+                        ov.x = pack_bf16(fmaxf(z[4 * q], 0.0f), fmaxf(z[4 * q + 1], 0.0f));
+                        ov.y = pack_bf16(fmaxf(z[4 * q + 2], 0.0f), fmaxf(z[4 * q + 3], 0.0f));
+                    }
+                    a1w[(q * NPP + pp + 1) * 2 + hk] = ov;
+                }
+            }
+        };
         bf16x8 wa5[2];
         load_w5(w5, lane, wa5);
-        for (int i = lane; i < 3 * NP * 18; i += 64) (&ur[0][0])[i] = 0.0f;
-        // layer-1 weights and biases of this lane's 8 channels
-        float ww1[9][8], bb1[8];
-        #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            bb1[e] = b1[8 * g4 + e];
-            #pragma unroll
-            for (int tp = 0; tp < 9; ++tp) ww1[tp][e] = w1[tp * 32 + 8 * g4 + e];
-        }
         // x rows: lane l holds columns c0 - 1 + l + 64 k of a row (zero outside the
         // image); each row is loaded one step before it enters the LDS ring, and
         // the residual values of the y row (columns c0 + l + 64 k) one step before
@@ -510,47 +565,24 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
             }
         };
         float xn[KX], xres[KY];
+        while (next_segment()) {
+        for (int i = lane; i < 3 * NP * 18; i += 64) (&ur[0][0])[i] = 0.0f;
         fetch_x(y0 - 5, xn);
         put_x(0, xn);
         fetch_x(y0 - 4, xn);
         put_x(1, xn);
         fetch_x(y0 - 3, xn);
+        put_x(2, xn);
+        fetch_x(y0 - 2, xn);
         fetch_res(y0 - 12, xres);
         __syncthreads();
-        // one pipeline step (x row y0 - 5 + s + dy in ring slot (s + dy) % 3, U row
-        // y0 - 13 + s + dy of the output layer in slot (s + 1 + dy) % 3; PH = s % 3)
+        // one pipeline step (U row y0 - 13 + s + dy of the output layer in ring
+        // slot (s + 1 + dy) % 3; PH = s % 3)
         auto step = [&](auto ph, int s) {
             constexpr int PH = decltype(ph)::value;
-            put_x((PH + 2) % 3, xn);                    // x row y0 - 3 + s
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            fetch_x(y0 - 2 + s, xn);                    // for the next step
-            // a1 row r1 (k_conv_in's arithmetic: z = b; z = fma(w[t], v[t], z) for t = 0..8);
-            // lane (r16, g4): pixel 16 i + r16 of tile i, channels 8 g4 .. 8 g4 + 7
-            const int r1 = y0 - 4 + s;
-            uint4* a1w = act[0][s & 1];
-            #pragma unroll 2
-            for (int i = 0; i < T; ++i) {
-                const int pp = 16 * i + r16;
-                const int c = c0 + pp;
-                float v[9];
-                #pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-                    #pragma unroll
-                    for (int dx = 0; dx < 3; ++dx) v[3 * dy + dx] = xr[(PH + dy) % 3][pp + dx];
-                const bool ok = r1 >= 0 && r1 < H && c >= 0 && c < W;
-                uint32_t pk[4];
-                #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float z0 = bb1[2 * e], z1 = bb1[2 * e + 1];
-                    #pragma unroll
-                    for (int tp = 0; tp < 9; ++tp) {
-                        z0 = fmaf(ww1[tp][2 * e], v[tp], z0);
-                        z1 = fmaf(ww1[tp][2 * e + 1], v[tp], z1);
-                    }
-                    pk[e] = ok ? pack_bf16(fmaxf(z0, 0.0f), fmaxf(z1, 0.0f)) : 0u;
-                }
-                a1w[g4 * NPP + pp + 1] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-            }
+            put_x((s + 3) & 3, xn);                     // x row y0 - 2 + s, for step s + 1
+            fetch_x(y0 - 1 + s, xn);                    // for the step after
+            conv_in_row(s);
             // output layer: U row of the a4 row y0 - 11 + s written at step s - 1,
             // then the 3x3 sum of y row y0 - 12 + s
             const uint4* a4r = act[3][(s + 1) & 1];
@@ -581,8 +613,9 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
             if (s + 1 < L + 12) step(std::integral_constant<int, 1>{}, s + 1);
             if (s + 2 < L + 12) step(std::integral_constant<int, 2>{}, s + 2);
         }
+        }
     } else {
-        // ---- 32 -> 32 layer wv + 1: relu(conv(a_wv) + b), T / 2 tiles of 32 pixels ----
+        // ---- the 32 -> 32 layer wv + 1: relu(conv(a_wv) + b), T / 2 tiles of 32 pixels ----
         constexpr int T2 = T / 2;
         bf16x8 wa[9][2];
         load_w32(wv == 1 ? w2 : (wv == 2 ? w3 : w4), lane, wa);
@@ -592,6 +625,7 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
         for (int i = 0; i < 16; ++i) bv[i] = bias[8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)];
         const int n = lane & 31, h = lane >> 5;
         f32x16 m0[T2], m1[T2], m2[T2];
+        while (next_segment()) {
         #pragma unroll
         for (int j = 0; j < T2; ++j) m0[j] = m1[j] = m2[j] = zero16();
         __syncthreads();
@@ -601,12 +635,16 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
             uint2* ow = reinterpret_cast<uint2*>(act[wv][s & 1]);
             const int ro = y0 - 4 - 2 * wv + s;            // the row this step completes
             const bool rok = ro >= 0 && ro < H;
+            // every tile's MFMAs first, then the completed rows
             #pragma unroll
             for (int j = 0; j < T2; ++j) {
                 bf16x8 f[3][2];
                 frags32<NPP>(in, 32 * j + n, lane, f);
                 Cm[j] = zero16();
                 mid32_row(wa, f, A[j], Bm[j], Cm[j]);
+            }
+            #pragma unroll
+            for (int j = 0; j < T2; ++j) {
                 // this lane holds channels 8 q + 4 h .. + 3 of pixel n of the tile
                 const int c = c0 + 32 * j + n;
                 const bool ok = rok && c >= 0 && c < W;
@@ -626,6 +664,7 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
             step(s, m0, m1, m2);
             if (s + 1 < L + 12) step(s + 1, m1, m2, m0);
             if (s + 2 < L + 12) step(s + 2, m2, m0, m1);
+        }
         }
     }
 }
@@ -907,17 +946,27 @@ void cnn_forward(const float* x, const float* w1, const float* b1, const uint16_
                  const uint16_t* w3, const float* b3, const uint16_t* w4, const float* b4, const float* w5, float b5,
                  float* y, int B, int H, int W, hipStream_t st)
 {
-    RN_REQUIRE(B > 0 && H > 0 && W > 0 && B <= 65535, "cnn_forward: bad batch or image size");
-    // candidates: strips of T = 4 or 6 tiles of 16 columns, 32, 64 or 128 rows per workgroup
+    RN_REQUIRE(B > 0 && H > 0 && W > 0, "cnn_forward: bad batch or image size");
+    int dev = 0, ncu = 0;
+    RN_HIP_CHECK(hipGetDevice(&dev));
+    RN_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // candidates: strips of T = 4 or 6 tiles of 16 columns; the (image, strip,
+    // row) sequence cut into equal shares for 2 workgroups per CU (one round),
+    // or into shares of 128 or 64 rows
     auto launch = [&](int c) {
-        auto go = [&](auto kern, int wo, int rows) {
-            const dim3 grd((unsigned)cdiv(W, wo), (unsigned)cdiv(H, rows), (unsigned)B);
-            hipLaunchKernelGGL(kern, grd, dim3(256), 0, st, x, w1, b1, w2, b2, w3, b3, w4, b4, w5, b5, y, H, W);
+        auto go = [&](auto kern, int wo, long per_rows) {
+            const int nstrips = cdiv(W, wo);
+            const long total = (long)B * nstrips * H;
+            const long per = per_rows > 0 ? per_rows : (total + 2L * ncu - 1) / (2L * ncu);
+            const long nwg = (total + per - 1) / per;
+            RN_REQUIRE(nwg <= 0x7fffffffL, "cnn_forward: grid too large");
+            hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), 0, st, x, w1, b1, w2, b2, w3, b3, w4, b4, w5, b5,
+                               y, H, W, nstrips, total, per);
         };
-        if (c == 0) go(k_cnn_fwd<4, 64>, FusedCnn<4>::WO, 64);
-        else if (c == 1) go(k_cnn_fwd<6, 64>, FusedCnn<6>::WO, 64);
-        else if (c == 2) go(k_cnn_fwd<4, 128>, FusedCnn<4>::WO, 128);
-        else go(k_cnn_fwd<4, 32>, FusedCnn<4>::WO, 32);
+        if (c == 0) go(k_cnn_fwd<4>, FusedCnn<4>::WO, 0);
+        else if (c == 1) go(k_cnn_fwd<6>, FusedCnn<6>::WO, 0);
+        else if (c == 2) go(k_cnn_fwd<4>, FusedCnn<4>::WO, 128);
+        else go(k_cnn_fwd<4>, FusedCnn<4>::WO, 64);
         check_launch("cnn_forward");
     };
     const int c = tune_choose({TK_CNN_FWD, B, H, W, 0, 0}, 4, launch, st, true);
