@@ -334,24 +334,34 @@ __device__ __forceinline__ void siddon_rows_z_view(const ArbGeo& a, const SidBpC
         xy0 = wave_uniform(fmaxf(xy0, -tp));          // t >= 0: from the source on
         xy1 = wave_uniform(xy1);
         if (!(xy1 > xy0)) continue;
+        // a voxel can lie beyond the detector point only if the column's rays
+        // are still inside the voxel column there
+        const bool far = xy1 > 1.f - tp;
         const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
         for (int iv = iv0; iv <= iv1; ++iv) {
             const float dz = fmaf(Vv.z, (iv - w.cv) * a.dv, czs), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
             const float len = sqrtf(fmaf(dz, dz, dxy2));
             const float val = qc[(long)iv * nC];
-            const float inv = __builtin_amdgcn_rcpf(dz);
+            // the lowest voxel's z slab; voxel k's is shifted by k dz_vox / dz
+            float zl, zh, st;
+            if (dz != 0.f) {
+                const float inv = __builtin_amdgcn_rcpf(dz);
+                const float a0 = (rz - w.hw.z) * inv, a1 = (rz + w.hw.z) * inv;
+                zl = fminf(a0, a1); zh = fmaxf(a0, a1); st = w.gdz * inv;
+            } else {
+                // a ray at a voxel's height for all t: voxel k holds it if
+                // -h < rz + k dz_vox <= h
+                zl = -INFINITY; zh = INFINITY; st = 0.f;
+            }
             #pragma unroll
             for (int k = 0; k < ZPL; ++k) {
-                const float rzk = fmaf((float)k, w.gdz, rz);
-                float s0 = xy0, s1 = xy1;
-                if (dz != 0.f) {
-                    const float a0 = (rzk - w.hw.z) * inv, a1 = (rzk + w.hw.z) * inv;
-                    s0 = fmaxf(s0, fminf(a0, a1)); s1 = fminf(s1, fmaxf(a0, a1));
-                } else if (!(rzk > -w.hw.z && rzk <= w.hw.z)) {
-                    continue;
+                if (dz == 0.f) {
+                    const float rzk = fmaf((float)k, w.gdz, rz);
+                    if (!(rzk > -w.hw.z && rzk <= w.hw.z)) continue;
                 }
+                const float s0 = fmaxf(xy0, fmaf((float)k, st, zl)), s1 = fminf(xy1, fmaf((float)k, st, zh));
                 if (!(s1 > s0)) continue;
-                if (s0 >= 1.f - tp) {
+                if (far && s0 >= 1.f - tp) {
                     // a voxel beyond the detector point: the ray counts only if
                     // it entered the grid before that point
                     float g0, g1;
