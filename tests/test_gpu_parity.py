@@ -483,6 +483,44 @@ def test_dd_tomo_parity():
     assert rel_l2(r.cDDTomoBackprojection4d(y, C, S), orc.dd_tomo_bp(y, C, S, g, (32, 28, 10))) <= TOL
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_norm_images_match_oracle(masked):
+    # cDDTomoNormImg4d and cSiddonConeNormImgAbitrary4d (ReconNet_no_tf.py:281-288,
+    # 551-558): A^T(A(1 * mask)) * mask, against the same composition of the
+    # FP64 oracle's projector pair (two channels; the mask a disc that leaves
+    # the corner voxels out)
+    nview = 9
+    th = np.linspace(-0.35, 0.35, nview)
+    S = np.stack([250 * np.sin(th), 10 * np.cos(th), 300 * np.cos(th)], 1).astype(np.float32)
+    C = np.tile([0.0, 0.0, -60.0], (nview, 1)).astype(np.float32)
+    r = _arb_recon(32, 28, 10, 48, 40, 1.2, 1.0, nview)
+    g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.2, dv=1.0, off_u=0.3, off_v=-0.2)
+    shp = (1, 32, 28, 10, 2)
+    yy, xx = np.meshgrid(np.arange(28) - 13.5, np.arange(32) - 15.5)
+    disc = ((xx ** 2 + yy ** 2) < 14.0 ** 2).astype(np.float32)[None, :, :, None, None]
+    mask = np.broadcast_to(disc, shp).astype(np.float32) if masked else None
+    m = mask if masked else np.ones(shp, np.float32)
+    prj = np.zeros((1, 48, nview, 40, 2), np.float32)
+    out = r.cDDTomoNormImg4d(prj, C, S, mask=mask)
+    ref = orc.dd_tomo_bp(orc.dd_tomo_fp(m, C, S, g, 48, 40), C, S, g, (32, 28, 10)) * m
+    assert np.abs(ref).max() > 1.0
+    assert rel_l2(out, ref) <= TOL
+
+    Cc, U, V, Sc = _cone(24)
+    rc = _arb_recon(24, 24, 12, 40, 16, 1.5, 1.5, 24)
+    gc = dict(dx=1.0, dy=1.0, dz=1.0, du=1.5, dv=1.5, off_u=0.3, off_v=-0.2)
+    shp = (1, 24, 24, 12, 2)
+    yy, xx = np.meshgrid(np.arange(24) - 11.5, np.arange(24) - 11.5)
+    disc = ((xx ** 2 + yy ** 2) < 10.5 ** 2).astype(np.float32)[None, :, :, None, None]
+    mask = np.broadcast_to(disc, shp).astype(np.float32) if masked else None
+    m = mask if masked else np.ones(shp, np.float32)
+    prj = np.zeros((1, 40, 24, 16, 2), np.float32)
+    out = rc.cSiddonConeNormImgAbitrary4d(prj, Cc, U, V, Sc, mask=mask)
+    ref = orc.siddon_cone_bp(orc.siddon_cone_fp(m, Cc, U, V, Sc, gc, 40, 16), Cc, U, V, Sc, gc, (24, 24, 12)) * m
+    assert np.abs(ref).max() > 1.0
+    assert rel_l2(out, ref) <= TOL
+
+
 def test_cone_ossqs_parity():
     C, U, V, S = _cone(16)
     r = _arb_recon(16, 16, 8, 32, 12, 1.5, 1.5, 16)
