@@ -864,13 +864,17 @@ __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const f
             const int v0 = max((int)ceilf(glo), 0), v1 = min((int)floorf(ghi), a.nv - 1);
             // oe / (e2 - e1) / (|beta| dv) * dm / |dpc|: every factor of the weight but oz and the length
             const float wcol = oe * __builtin_amdgcn_rcpf((e2 - e1) * fabsf(beta) * a.dv) * dm * fabsf(idpc);
-            const float hb = 0.5f * a.dv;
+            // the row's z edges and the ray's z run are affine in the row
+            // index: z1 = z1_0 + beta dv iv, z2 = z1 + |beta| dv, lz = lz_0 + V.z dv iv
+            const float bdv = beta * a.dv, hb = 0.5f * a.dv;
+            const float z1_0 = fmaf(beta, -cv * a.dv - hb, alpha) + fminf(bdv, 0.f), zh = fabsf(bdv);
+            const float lzs = V.z * a.dv, lz_0 = fmaf(-cv, lzs, zc0);
+            const float* qcol = q + ((long)iu * a.nview + view) * a.nv * nC;
             for (int iv = v0; iv <= v1; ++iv) {
-                const float vc = ((float)iv - cv) * a.dv;
-                const float za = fmaf(beta, vc - hb, alpha), zb = fmaf(beta, vc + hb, alpha);
-                const float z1 = fminf(za, zb), z2 = fmaxf(za, zb);
-                const float lz = fmaf(vc, V.z, zc0);
-                const float wv = wcol * sqrtf(fmaf(lz, lz, lxy2)) * q[(((long)iu * a.nview + view) * a.nv + iv) * nC];
+                const float fiv = (float)iv;
+                const float z1 = fmaf(bdv, fiv, z1_0), z2 = z1 + zh;
+                const float lz = fmaf(lzs, fiv, lz_0);
+                const float wv = wcol * sqrtf(fmaf(lz, lz, lxy2)) * qcol[(long)iv * nC];
                 #pragma unroll
                 for (int k = 0; k < ZPL; ++k) {
                     const float oz = fminf(z2, fz + (float)(k + 1)) - fmaxf(z1, fz + (float)k);
