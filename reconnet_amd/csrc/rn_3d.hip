@@ -338,8 +338,10 @@ __device__ __forceinline__ void siddon_rows_z_view(const ArbGeo& a, const SidBpC
         // are still inside the voxel column there
         const bool far = xy1 > 1.f - tp;
         const float* qc = q + ((long)iu * a.nview + view) * a.nv * nC;
+        // the row's ray height is affine in the row index
+        const float dzs = Vv.z * a.dv, dz_0 = fmaf(-w.cv, dzs, czs);
         for (int iv = iv0; iv <= iv1; ++iv) {
-            const float dz = fmaf(Vv.z, (iv - w.cv) * a.dv, czs), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
+            const float dz = fmaf(dzs, (float)iv, dz_0), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
             const float len = sqrtf(fmaf(dz, dz, dxy2));
             const float val = qc[(long)iv * nC];
             // the lowest voxel's z slab; voxel k's is shifted by k dz_vox / dz
