@@ -925,6 +925,20 @@ void dd_tomo_bp(R* imgOut, const float* prj, const float* detCenter, const float
 // ---------------------------------------------------------------------------
 // Helical BP from rebinned parallel-cone data: Recon3D/HelicalBP.cu:10-83.
 // ---------------------------------------------------------------------------
+// The voxel's pi-segment start as the reference computes it, in float
+// (HelicalBP.cu:26-29: z = iz dz + volZ0; int((z / zrot 2 PI - PI / 4)
+// nviewPerPI / PI)), with the multiply-adds contracted to FMAs as nvcc does by
+// default.  The value is often an integer in exact arithmetic (for instance
+// 8 z - 12 at zrot = 12, 48 views per pi), so its truncation follows the last
+// bit of the float computation: computed in FP64 it differs by one view at
+// such heights.
+inline int helical_istart(int iz, float dz, float volZ0, float zrot, int nviewPerPI)
+{
+    const float PIf = (float)kPI;
+    const float z = std::fma((float)iz, dz, volZ0);
+    return (int)(std::fma(z / zrot * 2.0f, PIf, -PIf / 4.0f) * (float)nviewPerPI / PIf);
+}
+
 template <class R>
 void helical_bp(R* imgOut, const float* prj, int nB, int nC, int nx, int ny, int nz, float dx, float dy, float dz,
                 int nu, int nview, int nv, float du, float dv, float off_u, float off_v, float dsd, float dso,
@@ -940,7 +954,7 @@ void helical_bp(R* imgOut, const float* prj, int nB, int nC, int nx, int ny, int
         for (int iz = 0; iz < nz; iz++) {
             R x = (ix - (nx - 1) / R(2)) * (R)dx, y = (iy - (ny - 1) / R(2)) * (R)dy, z = iz * (R)dz + (R)volZ0;
             const R PI = (R)kPI;
-            int iStart = (int)((z / (R)zrot * 2 * PI - PI / 4) * nviewPerPI / PI);
+            const int iStart = helical_istart(iz, dz, volZ0, zrot, nviewPerPI);
             R imgVal = 0;
             for (int i = iStart; i < iStart + nviewPerPI; i++) {
                 R totalW = 0, val = 0;

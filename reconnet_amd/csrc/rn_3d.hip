@@ -938,14 +938,34 @@ __global__ void k_cone_update(float* __restrict__ x, const float* __restrict__ b
 // ascending), the window start in the reference's float arithmetic.
 // ---------------------------------------------------------------------------
 // prj_k: [slice][nview][nv][nu]
-// per view k: (sin t_k, cos t_k, zrot k / (2 nviewPerPI)), t_k = theta0 + k PI / nviewPerPI, in FP64
+// per view k: (sin t_k, cos t_k, zk = zrot k / (2 nviewPerPI) as a float pair
+// hi, lo), t_k = theta0 + k PI / nviewPerPI, in FP64
 __global__ void k_helical_views(HelicalGeo h, float4* __restrict__ vt)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= h.nview) return;
     const double PI = (double)3.141592657f;
     const double t = (double)h.theta0 + k * PI / h.nviewPerPI;
-    vt[k] = make_float4((float)sin(t), (float)cos(t), (float)((double)h.zrot * k / (2.0 * h.nviewPerPI)), 0.0f);
+    const double zk = (double)h.zrot * k / (2.0 * h.nviewPerPI);
+    const float zhi = (float)zk;
+    vt[k] = make_float4((float)sin(t), (float)cos(t), zhi, (float)(zk - zhi));
+}
+
+// asin by its Maclaurin series, sum_k C(2k, k) / (4^k (2k + 1)) x^(2k + 1),
+// through x^15 for |x| <= 0.35 (truncation below 3e-10), asinf beyond
+__device__ __forceinline__ float asin_series(float x)
+{
+    if (!(fabsf(x) <= 0.35f)) return asinf(x);
+    const float x2 = x * x;
+    float p = 1.396484375e-02f;
+    p = fmaf(p, x2, 1.735276442e-02f);
+    p = fmaf(p, x2, 2.237215909e-02f);
+    p = fmaf(p, x2, 3.038194444e-02f);
+    p = fmaf(p, x2, 4.464285714e-02f);
+    p = fmaf(p, x2, 7.500000000e-02f);
+    p = fmaf(p, x2, 1.666666667e-01f);
+    p = fmaf(p, x2, 1.000000000e+00f);
+    return x * p;
 }
 
 template <int ZC>
@@ -967,8 +987,11 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
     int iLo = 0x7fffffff, iHi = -0x7fffffff;
     #pragma unroll
     for (int j = 0; j < ZC; ++j) {
-        zc[j] = (z0 + j) * h.dz + h.volZ0;
-        is[j] = (int)((zc[j] / h.zrot * 2 * PI - PI / 4) * h.nviewPerPI / PI);
+        // the reference's float arithmetic, multiply-adds as FMAs (nvcc's
+        // default contraction): the start is often an integer in exact
+        // arithmetic, so its truncation follows these roundings
+        zc[j] = fmaf((float)(z0 + j), h.dz, h.volZ0);
+        is[j] = (int)(fmaf(zc[j] / h.zrot * 2.0f, PI, -PI / 4.0f) * (float)h.nviewPerPI / PI);
         if (z0 + j < h.nz) {
             iLo = min(iLo, is[j]);
             iHi = max(iHi, is[j] + h.nviewPerPI);
@@ -981,6 +1004,11 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
     const float zr2pi = h.zrot / (2 * PI);
     const float qs = 2.0f / (h.dv * h.nv);
     const float cu = (h.nu - 1.0f) / 2.0f + h.off_u, cv = (h.nv - 1.0f) / 2.0f + h.off_v;
+    const float idso = 1.0f / h.dso, idu = 1.0f / h.du, idv = 1.0f / h.dv;
+    // the taper's phase in revolutions: cos((q - Q) / (1 - Q) pi / 2) = cos(2 pi r),
+    // r = (q - Q) / (4 (1 - Q)), by the hardware cosine (taper weights enter as
+    // sum w P / sum w; its error is far below the parity bound)
+    const float tq = 0.25f / (1.0f - h.Q);
     float acc[ZC];
     #pragma unroll
     for (int j = 0; j < ZC; ++j) acc[j] = 0.0f;
@@ -994,9 +1022,11 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
             const float4 tv = vt[k];                          // k is wave-uniform: scalar loads
             const float st = tv.x, ct = tv.y;
             const float u = x * st - y * ct;
-            const float mag = h.dsd / (sqrtf(h.dso * h.dso - u * u) - (x * ct + y * st));
-            const float zs = tv.z - zr2pi * asinf(u / h.dso);
-            const float pu = u / h.du + cu;
+            const float mag = h.dsd * __builtin_amdgcn_rcpf(sqrtf(h.dso * h.dso - u * u) - (x * ct + y * st));
+            // source height zk - zr2pi asin(u / dso) as tv.z + zsl (tv.z exact in
+            // float, so that z - tv.z loses nothing to the source's turn height)
+            const float zsl = fmaf(-zr2pi, asin_series(u * idso), tv.w);
+            const float pu = fmaf(u, idu, cu);
             const float fu = floorf(pu);
             const int u0 = (int)fu;
             const float au = pu - fu;
@@ -1007,13 +1037,13 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
             #pragma unroll
             for (int j = 0; j < ZC; ++j) {
                 if (i < is[j] || i >= is[j] + h.nviewPerPI) continue;
-                const float v = (zc[j] - zs) * mag;
+                const float v = ((zc[j] - tv.z) - zsl) * mag;
                 const float q = fabsf(v * qs);
                 float w;
                 if (q <= h.Q) w = 1.0f;
-                else if (q < 1.0f) { w = cosf((q - h.Q) / (1 - h.Q) * PI / 2); w *= w; }
+                else if (q < 1.0f) { w = __builtin_amdgcn_cosf((q - h.Q) * tq); w *= w; }
                 else w = 0.0f;
-                const float pv = v / h.dv + cv;
+                const float pv = fmaf(v, idv, cv);
                 const float fv = floorf(pv);
                 const int v0 = (int)fv;
                 const float av = pv - fv;

@@ -564,6 +564,25 @@ def test_helical_bp_parity(cand, m_pi, Q, nC):
     assert rel_l2(out, ref) <= TOL
 
 
+def test_helical_bp_parity_wide_fan():
+    # SID 40 mm around a 32 x 30 mm volume: |u| / SID reaches 0.55, so the
+    # voxels take both branches of the source-height term (the series for
+    # |u| / SID <= 0.35, asinf beyond)
+    from reconnet_amd.recon import ReconNet
+
+    npp, turns, nu, nv = 48, 3, 96, 24
+    r = ReconNet()
+    r.nx, r.ny, r.nz, r.dx, r.dy, r.dz = 32, 30, 13, 1.0, 1.0, 1.0
+    r.du, r.dv, r.off_u, r.off_v, r.dsd, r.dso = 0.5, 1.0, 0.25, -0.5, 80.0, 40.0
+    rng = np.random.default_rng(10)
+    prj = rng.uniform(0, 1, (1, nu, 2 * npp * turns, nv, 1)).astype(np.float32)
+    out = r.cHelicalBP4d(prj, npp, 0.1, 8.0, 12.0, mPI=1, Q=0.5)
+    g = dict(dx=1.0, dy=1.0, dz=1.0, du=0.5, dv=1.0, off_u=0.25, off_v=-0.5, dsd=80.0, dso=40.0)
+    ref = orc.helical_bp(prj, g, (32, 30, 13), npp, 0.1, 8.0, 12.0, m_pi=1, Q=0.5)
+    assert np.abs(ref).max() > 0.1
+    assert rel_l2(out, ref) <= TOL
+
+
 def test_helical_bp_device_entry_equals_host_abi():
     import torch
 
