@@ -9,7 +9,8 @@
 // to independent per-slice launches.  Consecutive slices of one geometry and
 // angle set share a descriptor, up to kRagS of them: a lane then carries the
 // ray or pixel through all of them, its geometry work done once (each slice
-// with the arithmetic of a one-slice call).  Descriptors are ordered by
+// with the arithmetic of a one-slice call; the forward projection takes a
+// group in chunks of kRagFp).  Descriptors are ordered by
 // decreasing cost so the heaviest start first.  Every wave leaves the loop once
 // the counter passes the item total.
 
@@ -26,7 +27,10 @@ namespace {
 
 using namespace fan;
 
-constexpr int kRagS = 4;   // slices of one geometry per descriptor
+constexpr int kRagS = 8;    // slices of one geometry per backprojection descriptor
+constexpr int kRagFp = 4;   // per forward descriptor (a group's chunks): the ray
+                            // loops carry fewer registers and the launch more items
+constexpr int kRagSidBp = 4;   // per Siddon backprojection descriptor (the same)
 
 struct RagSlice {
     FanK g;
@@ -63,22 +67,25 @@ __global__ __launch_bounds__(256) void k_ragged_fp(const RagSlice* __restrict__ 
         const bool active = iu < d.g.nu;
         const long plane = (long)d.g.nx * d.g.ny, sino = (long)d.g.nu * d.g.nview;
         // every lane takes part (wave-uniform line loops)
-        float acc[kRagS];
+        float acc[kRagFp];
         if (siddon)
-            siddon_fp_ray_slices<float, kRagS>(d.g, d.pa, d.pb, plane, d.nsl, view, active ? iu : d.g.nu - 1, active,
-                                               acc);
+            siddon_fp_ray_slices<float, kRagFp>(d.g, d.pa, d.pb, plane, d.nsl, view, active ? iu : d.g.nu - 1, active,
+                                                acc);
         else
-            dd_fp_ray_slices<float, kRagS>(d.g, d.pa, d.pb, plane, d.nsl, view, active ? iu : d.g.nu - 1, active, acc);
+            dd_fp_ray_slices<float, kRagFp>(d.g, d.pa, d.pb, plane, d.nsl, view, active ? iu : d.g.nu - 1, active,
+                                            acc);
         if (active) {
             #pragma unroll
-            for (int q = 0; q < kRagS; ++q)
+            for (int q = 0; q < kRagFp; ++q)
                 if (q < d.nsl) d.out[q * sino + (long)iu * d.g.nview + view] = acc[q];
         }
     }
 }
 
+// S: the descriptors' slice bound (kRagSidBp for Siddon, kRagS for the DD adjoint)
+template <bool SIDDON, int S>
 __global__ __launch_bounds__(256) void k_ragged_bp(const RagSlice* __restrict__ sl, int nsl, int total,
-                                                   int* __restrict__ counter, int siddon)
+                                                   int* __restrict__ counter)
 {
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -89,15 +96,15 @@ __global__ __launch_bounds__(256) void k_ragged_bp(const RagSlice* __restrict__ 
         const int ix = (local / d.tiles) * 8 + (lane >> 3), iy = (local % d.tiles) * 8 + (lane & 7);
         if (ix < d.g.nx && iy < d.g.ny) {
             const long plane = (long)d.g.nx * d.g.ny, sino = (long)d.g.nu * d.g.nview;
-            float acc[kRagS];
-            if (siddon)
-                siddon_bp_pixel_slices<float, 4, kRagS>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
+            float acc[S];
+            if (SIDDON)
+                siddon_bp_pixel_slices<float, 4, S>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
             else if (d.g.unit_angle)
-                dd_adj_pixel_slices<true, kRagS>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
+                dd_adj_pixel_slices<true, S>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
             else
-                dd_adj_pixel_slices<false, kRagS>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
+                dd_adj_pixel_slices<false, S>(d.g, d.pa, sino, d.nsl, 0, d.g.nview, ix, iy, acc);
             #pragma unroll
-            for (int q = 0; q < kRagS; ++q)
+            for (int q = 0; q < S; ++q)
                 if (q < d.nsl) d.out[q * plane + (long)ix * d.g.ny + iy] = acc[q];
         }
     }
@@ -178,7 +185,7 @@ void RaggedBatch::setup(const std::vector<FanGeo>& geos, const std::vector<const
         const FanGeo& gb = geos[p_->first[b]];
         return (double)ga.nx * ga.ny * ga.nview > (double)gb.nx * gb.ny * gb.nview;
     });
-    p_->desc.alloc(nd * sizeof(RagSlice));
+    p_->desc.alloc(nd * ((kRagS + kRagFp - 1) / kRagFp) * sizeof(RagSlice));
     p_->counter.alloc(sizeof(int));
     p_->wa.alloc(io * sizeof(float));
     p_->wb.alloc(std::max(io, so) * sizeof(float));
@@ -194,7 +201,7 @@ void RaggedBatch::forward(bool siddon, const float* img, float* prj, hipStream_t
     const int nd = (int)p_->first.size();
     float* A = p_->wa.as<float>();
     float* Bb = p_->wb.as<float>();
-    p_->host.assign(nd, RagSlice{});
+    p_->host.clear();
     int item = 0;
     for (int k = 0; k < nd; k++) {
         const int s = p_->first[p_->order[k]], cnt = p_->count[p_->order[k]];
@@ -202,31 +209,41 @@ void RaggedBatch::forward(bool siddon, const float* img, float* prj, hipStream_t
         const float* P = img + p_->img_off[s];
         float* a = A + p_->img_off[s];
         float* b = Bb + p_->img_off[s];
-        RagSlice& d = p_->host[k];
-        d.g = p_->tables[s].k;
-        d.nsl = cnt;
+        const float* pa;
+        const float* pb;
         if (siddon) {
             launch_transpose(P, a, g.nx, g.ny, cnt, st);    // rows contiguous
-            d.pa = a;
-            d.pb = P;
+            pa = a;
+            pb = P;
         } else {
             // RY[x][y] = (P[x-1][y] + P[x][y]) / 2 ;  RX = transpose((P[x][y-1] + P[x][y]) / 2)
             float* c = p_->wc.as<float>() + p_->img_off[s];
             launch_avg_prev(P, b, g.nx, g.ny, cnt, 0, st);
             launch_avg_prev(P, c, g.nx, g.ny, cnt, 1, st);
             launch_transpose(c, a, g.nx, g.ny, cnt, st);
-            d.pa = a;
-            d.pb = b;
+            pa = a;
+            pb = b;
         }
-        d.out = prj + p_->sino_off[s];
-        d.tiles = cdiv(g.nu, 64);
-        d.nitems = d.tiles * g.nview;
-        d.item0 = item;
-        item += d.nitems;
+        // the group's slices in chunks of kRagFp, consecutive in both layouts
+        const long plane = (long)g.nx * g.ny, sino = (long)g.nu * g.nview;
+        for (int c0 = 0; c0 < cnt; c0 += kRagFp) {
+            RagSlice d{};
+            d.g = p_->tables[s].k;
+            d.nsl = std::min(kRagFp, cnt - c0);
+            d.pa = pa + c0 * plane;
+            d.pb = pb + c0 * plane;
+            d.out = prj + p_->sino_off[s] + c0 * sino;
+            d.tiles = cdiv(g.nu, 64);
+            d.nitems = d.tiles * g.nview;
+            d.item0 = item;
+            item += d.nitems;
+            p_->host.push_back(d);
+        }
     }
-    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), nd * sizeof(RagSlice), hipMemcpyHostToDevice, st));
+    const int nfd = (int)p_->host.size();
+    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), nfd * sizeof(RagSlice), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p_->counter.as<int>());
-    hipLaunchKernelGGL(k_ragged_fp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), nd, item,
+    hipLaunchKernelGGL(k_ragged_fp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), nfd, item,
                        p_->counter.as<int>(), siddon ? 1 : 0);
     check_launch("ragged_fp");
     RN_HIP_CHECK(hipStreamSynchronize(st));
@@ -236,28 +253,38 @@ void RaggedBatch::backward(bool siddon, const float* prj, float* img, hipStream_
 {
     const int nd = (int)p_->first.size();
     float* V = p_->wb.as<float>();   // view-major copies of the sinograms
-    p_->host.assign(nd, RagSlice{});
+    p_->host.clear();
+    const int chunk = siddon ? kRagSidBp : kRagS;
     int item = 0;
     for (int k = 0; k < nd; k++) {
         const int s = p_->first[p_->order[k]], cnt = p_->count[p_->order[k]];
         const FanGeo& g = p_->tables[s].geo;
         float* v = V + p_->sino_off[s];
         launch_transpose(prj + p_->sino_off[s], v, g.nu, g.nview, cnt, st);
-        RagSlice& d = p_->host[k];
-        d.g = p_->tables[s].k;
-        d.nsl = cnt;
-        d.pa = v;
-        d.pb = nullptr;
-        d.out = img + p_->img_off[s];
-        d.tiles = cdiv(g.ny, 8);
-        d.nitems = d.tiles * cdiv(g.nx, 8);
-        d.item0 = item;
-        item += d.nitems;
+        const long plane = (long)g.nx * g.ny, sino = (long)g.nu * g.nview;
+        for (int c0 = 0; c0 < cnt; c0 += chunk) {
+            RagSlice d{};
+            d.g = p_->tables[s].k;
+            d.nsl = std::min(chunk, cnt - c0);
+            d.pa = v + c0 * sino;
+            d.pb = nullptr;
+            d.out = img + p_->img_off[s] + c0 * plane;
+            d.tiles = cdiv(g.ny, 8);
+            d.nitems = d.tiles * cdiv(g.nx, 8);
+            d.item0 = item;
+            item += d.nitems;
+            p_->host.push_back(d);
+        }
     }
-    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), nd * sizeof(RagSlice), hipMemcpyHostToDevice, st));
+    const int nbd = (int)p_->host.size();
+    RN_HIP_CHECK(hipMemcpyAsync(p_->desc.get(), p_->host.data(), nbd * sizeof(RagSlice), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, p_->counter.as<int>());
-    hipLaunchKernelGGL(k_ragged_bp, dim3(persistent_blocks()), dim3(256), 0, st, p_->desc.as<RagSlice>(), nd, item,
-                       p_->counter.as<int>(), siddon ? 1 : 0);
+    if (siddon)
+        hipLaunchKernelGGL((k_ragged_bp<true, kRagSidBp>), dim3(persistent_blocks()), dim3(256), 0, st,
+                           p_->desc.as<RagSlice>(), nbd, item, p_->counter.as<int>());
+    else
+        hipLaunchKernelGGL((k_ragged_bp<false, kRagS>), dim3(persistent_blocks()), dim3(256), 0, st,
+                           p_->desc.as<RagSlice>(), nbd, item, p_->counter.as<int>());
     check_launch("ragged_bp");
     RN_HIP_CHECK(hipStreamSynchronize(st));
 }
