@@ -458,7 +458,6 @@ __global__ __launch_bounds__(256, 2) void k_cnn_fwd(const float* __restrict__ x,
     using F = FusedCnn<T>;
     static_assert(T % 2 == 0, "32-pixel tiles of the 32 -> 32 layers");
     constexpr int NPP = F::NPP, NP = F::NP;
-    constexpr int KP = (16 * T + 63) / 64;   // layer-1 pixels per lane
     __shared__ uint4 act[4][2][4 * NPP];     // rows of a1 .. a4
     __shared__ float xr[4][NP];              // x ring: rows y - 1, y, y + 1 of the a1 row and the next
     __shared__ float ur[3][NP * 18];         // U rows of the output layer (wave 0)
