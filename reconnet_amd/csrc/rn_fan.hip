@@ -481,7 +481,6 @@ __device__ __forceinline__ void dd_bp_pixel(const FanK& g, const T* const (&rows
     const float x = (ix - 0.5f * (g.nx - 1)) * g.dx, y = (iy - 0.5f * (g.ny - 1)) * g.dy;
     const float inv_da = 1.0f / g.da;
     const float boff = 0.5f * g.nu - g.off_a;
-    T val[S];
     #pragma unroll
     for (int q = 0; q < S; ++q) acc[q] = vzero<T>();
     for (int view = 0; view < g.nview; ++view) {
@@ -492,15 +491,13 @@ __device__ __forceinline__ void dd_bp_pixel(const FanK& g, const T* const (&rows
         else                   { ex = 0.5f * g.dy * st; ey = 0.5f * g.dy * ct; }
         const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
         const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
-        seg_avg_rows<T, S>(rows, (unsigned)(view * g.nu) * (unsigned)sizeof(T), g.nu, fminf(b1, b2),
-                           fmaxf(b1, b2), val);
         float fw = 1.0f;
         if (fbp) {
             const float d = g.dso + ry;
             fw = g.dso * g.dso / (rx * rx + d * d);
         }
-        #pragma unroll
-        for (int q = 0; q < S; ++q) vadd(acc[q], fbp ? vscale(val[q], fw) : val[q]);
+        seg_acc_rows<T, S>(rows, (unsigned)(view * g.nu) * (unsigned)sizeof(T), g.nu, fminf(b1, b2),
+                           fmaxf(b1, b2), fw, acc);
     }
 }
 
@@ -651,16 +648,13 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_dd_fan_bp_ql(FanK g
             if (nhi >= nlo && nhi - nlo < CAP) issue(view + 1, nlo);
         }
         if (any) {
-            float4 val[G];
+            const float sc = fbp ? fw : 1.0f;
             if (staged) {
                 // lanes outside the image read nothing: their interval is made empty
-                seg_avg_staged<G, CAP, NB>(st, wlo, g.nu, inside ? lo : -2.0f, inside ? hi : -1.0f, val);
-            } else {
-                if (inside) seg_avg_rows<float4, G>(rows, (unsigned)(view * g.nu) * 16u, g.nu, lo, hi, val);
-                else for (int q = 0; q < G; ++q) val[q] = vzero<float4>();
+                seg_acc_staged<G, CAP, NB>(st, wlo, g.nu, inside ? lo : -2.0f, inside ? hi : -1.0f, sc, acc);
+            } else if (inside) {
+                seg_acc_rows<float4, G>(rows, (unsigned)(view * g.nu) * 16u, g.nu, lo, hi, sc, acc);
             }
-            #pragma unroll
-            for (int q = 0; q < G; ++q) vadd(acc[q], fbp ? vscale(val[q], fw) : val[q]);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         lo = nlo_f; hi = nhi_f; fw = nfw; wlo = nlo; whi = nhi;

@@ -65,71 +65,43 @@ __device__ __forceinline__ void yline(const FanK& g, float ct, float st, float s
     B = (g.dx / g.dy) * cu * inv;
 }
 
-// Average over [lo, hi] of a piecewise-constant line of n cells (zero outside
-// [0, n)).  For hi == lo the value at lo is returned (a ray exactly along an
-// image line).  Numerator and denominator use the same rounded end points, so
-// a constant line is reproduced exactly.
-template <class T>
-__device__ __forceinline__ T seg_avg(const T* __restrict__ line, int n, float lo, float hi)
-{
-    const float w = hi - lo;
-    const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
-    if (!(h > l)) {
-        if (w > 0.0f) return T(0);
-        return (lo >= 0.0f && lo < (float)n) ? line[(int)lo] : T(0);
-    }
-    const int m0 = (int)l;
-    const int m1 = min((int)h, n - 1);
-    T s = T(0);
-    for (int m = m0; m <= m1; ++m) {
-        const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
-        s += line[m] * (T)(ph - pl);
-    }
-    return s / (T)w;
-}
-
-// seg_avg of S rows (row q `rboff` bytes after the wave-uniform base rows[q])
-// over one interval: the cells, overlaps and the reciprocal of the width are
-// computed once for the S rows.
+// acc[q] += scale * the average of row q (`rboff` bytes after the
+// wave-uniform base rows[q]) over one interval [lo, hi] of its n
+// piecewise-constant cells (zero outside [0, n)): the cells, their overlaps and the
+// combined weight overlap * scale / width are computed once for the S rows,
+// and each row adds one FMA per cell (a point interval, width 0, adds its
+// cell times scale).
 template <class T, int S>
-__device__ __forceinline__ void seg_avg_rows(const T* const (&rows)[S], unsigned rboff, int n, float lo, float hi,
-                                             T (&val)[S])
+__device__ __forceinline__ void seg_acc_rows(const T* const (&rows)[S], unsigned rboff, int n, float lo, float hi,
+                                             float scale, T (&acc)[S])
 {
     const float w = hi - lo;
     const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
-    #pragma unroll
-    for (int q = 0; q < S; ++q) val[q] = vzero<T>();
     if (!(h > l)) {
         if (w > 0.0f || !(lo >= 0.0f && lo < (float)n)) return;
         const unsigned boff = rboff + (unsigned)(int)lo * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) val[q] = ld_at(rows[q], boff);
+        for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(rows[q], boff), scale);
         return;
     }
+    const float k = scale / w;
     const int m0 = (int)l;
     const int m1 = min((int)h, n - 1);
     for (int m = m0; m <= m1; ++m) {
         const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
         const unsigned boff = rboff + (unsigned)m * (unsigned)sizeof(T);
         #pragma unroll
-        for (int q = 0; q < S; ++q) vmadd(val[q], ld_at(rows[q], boff), ph - pl);
+        for (int q = 0; q < S; ++q) vmadd(acc[q], ld_at(rows[q], boff), (ph - pl) * k);
     }
-    // one correctly rounded reciprocal for all S rows
-    const float inv = 1.0f / w;
-    #pragma unroll
-    for (int q = 0; q < S; ++q) val[q] = vscale(val[q], inv);
 }
 
-// seg_avg_rows for G rows staged in LDS: row q of `st` holds bins wlo ..
-// wlo + CAP - 1 of group q's row (every slot written: the caller skips views
-// whose window is empty, and the window covers every bin the interval
-// reads).  The same sums in the same order, written without branches around
-// the sums: every lane takes NB bins from its first bin on (bins past its
-// last one re-read its last bin, or slot 0 if it reads none, with weight 0;
-// the degenerate zero-width interval takes its bin with weight 1 and no
-// scaling), the rest in a loop.
+// seg_acc_rows for G rows staged in LDS: row q of `st` holds bins wlo ..
+// wlo + CAP - 1 (CAP float4 entries).  The first NB bins of the interval are
+// taken without branches (a bin past its end with weight 0, its index
+// clamped into the interval), further bins in a loop.
 template <int G, int CAP, int NB>
-__device__ __forceinline__ void seg_avg_staged(const float4* st, int wlo, int n, float lo, float hi, float4 (&val)[G])
+__device__ __forceinline__ void seg_acc_staged(const float4* st, int wlo, int n, float lo, float hi, float scale,
+                                               float4 (&acc)[G])
 {
     const float w = hi - lo;
     const float l = fmaxf(lo, 0.0f), h = fminf(hi, (float)n);
@@ -140,32 +112,26 @@ __device__ __forceinline__ void seg_avg_staged(const float4* st, int wlo, int n,
     const int m0 = normal ? (int)l : (point ? (int)lo : wlo);
     const int m1 = normal ? min((int)h, n - 1) : (point ? m0 : m0 - 1);
     const int t0 = m0 - wlo, tl = max(m1 - wlo, 0);
+    const float k = normal ? scale / w : scale;
     #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const int m = m0 + j;
         const bool on = m <= m1;
         const int t = on ? t0 + j : tl;
         const float ov = normal ? fminf(h, (float)(m + 1)) - fmaxf(l, (float)m) : 1.0f;
-        const float wt = on ? ov : 0.0f;
+        const float c = on ? ov * k : 0.0f;
         #pragma unroll
-        for (int q = 0; q < G; ++q) {
-            if (j == 0) val[q] = vscale(st[q * CAP + t], wt);
-            else vmadd(val[q], st[q * CAP + t], wt);
-        }
+        for (int q = 0; q < G; ++q) vmadd(acc[q], st[q * CAP + t], c);
     }
     for (int m = m0 + NB; m <= m1; ++m) {
         const float pl = fmaxf(l, (float)m), ph = fminf(h, (float)(m + 1));
+        const float c = (ph - pl) * k;
         #pragma unroll
-        for (int q = 0; q < G; ++q) vmadd(val[q], st[q * CAP + m - wlo], ph - pl);
-    }
-    if (normal) {
-        const float inv = 1.0f / w;
-        #pragma unroll
-        for (int q = 0; q < G; ++q) val[q] = vscale(val[q], inv);
+        for (int q = 0; q < G; ++q) vmadd(acc[q], st[q * CAP + m - wlo], c);
     }
 }
 
-// The bins seg_avg_rows reads for the interval [lo, hi] of a row of n bins:
+// The bins seg_acc_rows reads for the interval [lo, hi] of a row of n bins:
 // m0 .. m1, or m0 = n, m1 = -1 when it reads none.
 __device__ __forceinline__ void seg_avg_bins(int n, float lo, float hi, int& m0, int& m1)
 {
@@ -180,15 +146,6 @@ __device__ __forceinline__ void seg_avg_bins(int n, float lo, float hi, int& m0,
         m0 = n;
         m1 = -1;
     }
-}
-
-// Weight of cell m in seg_avg(line, n, lo, hi) (the transpose of seg_avg).
-__device__ __forceinline__ float seg_weight(int m, float lo, float hi)
-{
-    const float w = hi - lo;
-    const float pl = fmaxf(lo, (float)m), ph = fminf(hi, (float)(m + 1));
-    if (w > 0.0f) return ph > pl ? (ph - pl) / w : 0.0f;
-    return (lo >= (float)m && lo < (float)(m + 1)) ? 1.0f : 0.0f;
 }
 
 // conservative half width (in bins) of the set of rays through a disc of
