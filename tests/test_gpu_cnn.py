@@ -236,7 +236,9 @@ def test_training_step_reduces_loss():
 @pytest.mark.parametrize("shape", [(1, 9, 70), (2, 17, 33), (1, 64, 64), (3, 130, 200), (1, 64, 89), (1, 200, 97)])
 def test_fused_forward_equals_layer_by_layer_bitwise(shape):
     # one launch of the whole network against conv_in, 3 x conv_mid, conv_out:
-    # strips of 88 output columns overlapping by 8, 64-row segments, borders
+    # strips of 56 or 88 output columns overlapping by 8, each workgroup's
+    # share of (image, strip, row) walked as segments (a share crosses strips
+    # and images at these sizes), borders; every launch shape
     from reconnet_amd.cnn import PriorCNN, init_params
 
     B, H, W = shape
