@@ -274,7 +274,9 @@ int rn_tv_iso_sqs(const float* img, float* s1, float* s2, float* var, int nb, in
  * axis (x iff |C.x - S.x| >= |C.y - S.y|) the rectangle footprints of the
  * detector cells; its backward is the exact adjoint.  The Siddon kind is the
  * SiddonCone pair (SiddonCone.cu:117,196).  Device pointers, reference
- * layouts, caller stream. */
+ * layouts, caller stream.  A plan keeps one scratch buffer (the DD
+ * backward's FP64 row sums, 8·nBatches·nChannels·nu·nview·(nv + 1) bytes):
+ * calls on one plan must not overlap. */
 enum { RN_CONE_DD = 0, RN_CONE_SIDDON = 1 };
 void* rn_cone_plan_create(int nx, int ny, int nz, float dx, float dy, float dz, float cx, float cy, float cz,
         int nu, int nview, int nv, float du, float dv, float off_u, float off_v,

@@ -51,12 +51,14 @@ public:
     void setup(const ArbGeo& g, const float* C, const float* U, const float* V, const float* S, hipStream_t st);
     // reference layouts: img [B][nx][ny][nz][C], prj [B][nu][nview][nv][C] (device)
     void dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t st) const;
-    void dd_adj(int nB, int nC, const float* prj, float* img, hipStream_t st) const;
+    void dd_adj(int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st) const;
     void siddon_fp(int nB, int nC, const float* img, float* prj, DevBuf& work, hipStream_t st) const;
     void siddon_bp(int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st) const;
     ArbGeo geo{};
 private:
-    DevBuf vec_, cls_;
+    // vec_: C, U, V, S per view; cls_: main axis per view; views_, cols_:
+    // per-view and per-(view, column) constants of the table-driven adjoint
+    DevBuf vec_, cls_, views_, cols_;
 };
 
 struct HelicalGeo {

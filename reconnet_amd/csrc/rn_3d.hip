@@ -774,6 +774,73 @@ __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const flo
     prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c] = acc;
 }
 
+// Tables of the table-driven cone DD adjoint (k_dd_cone_adj_tab).
+// Row sums: per (slice, view, detector column) the nv + 1 FP64 sums
+// G[r] = sum over rows below r of len(row) q(row), len the length of the ray
+// through the cell centre (the adjoint's row weight).
+// ConeView: per view the constants of the plane-crossing geometry, resolved
+// for the view's main axis m (x or y; n the other in-plane axis, p the plane
+// index along m and j the voxel index along n).
+struct alignas(16) ConeView {
+    int xv;             // 1: main axis x
+    float mp0, dm;      // plane p's main-axis offset from the source: mp0 + p dm
+    float qn0, dn;      // n edge j relative to the source: qn0 + j dn
+    float A, Cc, Um, Un;// detector centre minus source (n, m) and detU (m, n)
+    float o;            // n index of the point on the plane at the source's n
+    float a0;           // z index of the source
+    float pad;
+};
+// ConeColumn: per (view, detector column) the ray constants that do not
+// depend on the plane: the n index of the column's two u-edge rays on plane
+// mp is mp k_e + o; the z index of row 0's lower edge on plane mp is
+// mp A1 + a0; a row spans mp / T3 in z index; W the column weight dm /
+// |main-axis run to the detector|, signed with the rows' z direction.
+// Columns that cannot be hit (an edge ray parallel to the planes) hold
+// k0 = k1 = 0 (an empty footprint) and W = 0.
+struct alignas(32) ConeColumn { float k0, k1, A1, T3, W, pad[3]; };
+
+// Row sums: one wave per (slice, view, column), in table order; lanes take
+// 64 rows at a time, an FP64 inclusive scan across the wave, then a carry.
+__global__ __launch_bounds__(256) void k_dd_cone_rowsum(ArbGeo a, const float* __restrict__ prj, const float* Cp,
+                                                        const float* Up, const float* Vp, const float* Sp, int nC,
+                                                        long ncols, double* __restrict__ T)
+{
+    const long col = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (col >= ncols) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int iu = (int)(col % a.nu);
+    const long rest = col / a.nu;
+    const int view = (int)(rest % a.nview);
+    const long bc = rest / a.nview;
+    const int cc = (int)(bc % nC), b = (int)(bc / nC);
+    const F3 c = ld3(Cp, view), U = ld3(Up, view), V = ld3(Vp, view), s = ld3(Sp, view);
+    const float cu = (a.nu - 1) / 2.0f + a.off_u, cv = (a.nv - 1) / 2.0f + a.off_v;
+    const float uc = ((float)iu - cu) * a.du;
+    const float lx = fmaf(uc, U.x, c.x) - s.x, ly = fmaf(uc, U.y, c.y) - s.y;
+    const float lxy2 = fmaf(lx, lx, ly * ly);
+    const float zc0 = fmaf(uc, U.z, c.z) - s.z;
+    const float lzs = V.z * a.dv, lz_0 = fmaf(-cv, lzs, zc0);
+    const float* qcol = prj + (((long)b * a.nu + iu) * a.nview + view) * a.nv * nC + cc;
+    double* G = T + col * (a.nv + 1);
+    if (lane == 0) G[0] = 0.0;
+    double carry = 0.0;
+    for (int r0 = 0; r0 < a.nv; r0 += 64) {
+        const int iv = r0 + lane;
+        double f = 0.0;
+        if (iv < a.nv) {
+            const float lz = fmaf(lzs, (float)iv, lz_0);
+            f = (double)(sqrtf(fmaf(lz, lz, lxy2)) * qcol[(long)iv * nC]);
+        }
+        #pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(f, d);
+            if (lane >= d) f += y;
+        }
+        if (iv < a.nv) G[iv + 1] = carry + f;
+        carry += __shfl(f, 63);
+    }
+}
+
 // Exact transpose of k_dd_cone_fp: voxel (ix, iy, iz) gathers, per view, the
 // cells whose footprint on its plane overlaps it, with the forward pass's
 // weight overlap / area * L.  A wave holds one (ix, iy) column of 64 voxels
@@ -882,6 +949,92 @@ __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const f
                     const float oz = fminf(z2, fz + (float)(k + 1)) - fmaxf(z1, fz + (float)k);
                     if (oz > 0.f) acc[k] = fmaf(oz, wv, acc[k]);
                 }
+            }
+        }
+    }
+    float* out = img + ((((long)b * g.nx + ix) * g.ny + iy) * g.nz + iz) * nC + cc;
+    #pragma unroll
+    for (int k = 0; k < ZPL; ++k)
+        if (iz + k < g.nz) out[(long)k * nC] = acc[k];
+}
+
+// k_dd_cone_adj driven by tables: the rows' z footprints on a column's
+// plane tile z without gaps (row r covers [zA + r bdv, zA + (r + 1) bdv],
+// bdv = beta dv), so the sum over rows of the z overlap times len q is bdv
+// times the difference of the row sums, interpolated linearly within a row,
+// at the voxel's two z edges.  ZPL + 1 lookups of one 16-byte pair
+// (G[r], G[r + 1]) give ZPL voxels; the per-view and per-column geometry
+// comes from the ConeView / ConeColumn tables (wave-uniform loads).  Same
+// weights as the forward pass (the transpose holds to rounding).  A NaN or
+// infinity in a row reaches every voxel above that row on the column's
+// plane, not only the voxels the row overlaps.
+template <int ZPL>
+__global__ __launch_bounds__(256) void k_dd_cone_adj_tab(ArbGeo a, Grid3K g, const double* __restrict__ T,
+                                                         float* __restrict__ img, const ConeView* __restrict__ views,
+                                                         const ConeColumn* __restrict__ cols, int nC)
+{
+    const int iz = (blockIdx.x * 64 + (threadIdx.x & 63)) * ZPL;
+    const int iy = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (int)(threadIdx.x >> 6));
+    const int ix = blockIdx.z % g.nx, bc = blockIdx.z / g.nx;
+    if (iz >= g.nz || iy >= g.ny) return;
+    const int cc = bc % nC, b = bc / nC;
+    const float cu = (a.nu - 1) / 2.0f + a.off_u;
+    const float margin = 0.05f;
+    const float fz = (float)iz, fnv = (float)a.nv;
+    const float idu = 1.0f / a.du;
+    const double* Tb = T + (long)bc * a.nview * a.nu * (a.nv + 1);
+    float acc[ZPL];
+    #pragma unroll
+    for (int k = 0; k < ZPL; ++k) acc[k] = 0.f;
+    for (int view = 0; view < a.nview; ++view) {
+        const ConeView vw = views[view];
+        const double* Tv = Tb + (long)view * a.nu * (a.nv + 1);
+        const int p = vw.xv ? ix : iy, j = vw.xv ? iy : ix;
+        const float mp = fmaf((float)p, vw.dm, vw.mp0);
+        const float imp = __builtin_amdgcn_rcpf(mp);
+        // u of the rays through the voxel's two in-plane edges
+        float ulo = 3.0e38f, uhi = -3.0e38f;
+        #pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const float qe = fmaf((float)(j + e), vw.dn, vw.qn0) * imp;
+            const float u = fmaf(-qe, vw.Cc, vw.A) * __builtin_amdgcn_rcpf(fmaf(qe, vw.Um, -vw.Un));
+            ulo = fminf(ulo, u);
+            uhi = fmaxf(uhi, u);
+        }
+        const float flo = fmaxf(ulo * idu + cu - 0.5f - margin, -1.0f);
+        const float fhi = fminf(uhi * idu + cu + 0.5f + margin, (float)a.nu);
+        if (!(fhi >= flo)) continue;                // also rejects NaN
+        const int u0 = max((int)ceilf(flo), 0), u1 = min((int)floorf(fhi), a.nu - 1);
+        const float smp = mp > 0.f ? 1.f : -1.f;
+        for (int iu = u0; iu <= u1; ++iu) {
+            const ConeColumn col = cols[(long)view * a.nu + iu];
+            // in-plane footprint of the column on plane p (index coordinates)
+            const float ea = fmaf(mp, col.k0, vw.o), eb = fmaf(mp, col.k1, vw.o);
+            const float e1 = fminf(ea, eb), e2 = fmaxf(ea, eb);
+            const float oe = fminf(e2, (float)(j + 1)) - fmaxf(e1, (float)j);
+            if (!(oe > 0.f) || !(e2 > e1)) continue;
+            // row coordinate of z index zi: s = (zi - zA) / bdv = zi ibdv + s0
+            const float ibdv = imp * col.T3;
+            if (!(fabsf(ibdv) < 1e20f)) continue;
+            const float s0 = -fmaf(mp, col.A1, vw.a0) * ibdv;
+            const float sb = fmaf(fz, ibdv, s0), st = fmaf(fz + (float)ZPL, ibdv, s0);
+            if (!(fmaxf(sb, st) > 0.f && fminf(sb, st) < fnv)) continue;
+            // oe / (e2 - e1) / |bdv| * dm / |dpc| times bdv
+            const float wcol = oe * __builtin_amdgcn_rcpf(e2 - e1) * col.W * smp;
+            // one 16-byte load per edge: G[r] and G[r + 1] (the address
+            // processing of these loads bounds the kernel, so lanes whose
+            // voxels miss the rows load nothing)
+            const double* Gc = Tv + (long)iu * (a.nv + 1);
+            double hp = 0.0;
+            #pragma unroll
+            for (int k = 0; k <= ZPL; ++k) {
+                const float sk = __builtin_amdgcn_fmed3f(fmaf(fz + (float)k, ibdv, s0), 0.f, fnv);
+                const unsigned r = min((unsigned)sk, (unsigned)(a.nv - 1));
+                const double* gp = Gc + r;
+                const double g0 = gp[0], g1 = gp[1];
+                const double h = fma((double)(sk - (float)r), g1 - g0, g0);
+                if (k > 0) acc[k - 1] = fmaf(wcol, (float)(h - hp), acc[k - 1]);
+                hp = h;
             }
         }
     }
@@ -1187,6 +1340,52 @@ void ConePlan::setup(const ArbGeo& g, const float* C, const float* U, const floa
     cls_.alloc(cls.size() * sizeof(int));
     RN_HIP_CHECK(hipMemcpyAsync(vec_.get(), h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
     RN_HIP_CHECK(hipMemcpyAsync(cls_.get(), cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    // view and column tables of the table-driven DD adjoint, in double
+    std::vector<ConeView> vt(n);
+    std::vector<ConeColumn> ct((size_t)n * g.nu);
+    const double cu = (g.nu - 1) / 2.0 + g.off_u, cv = (g.nv - 1) / 2.0 + g.off_v;
+    for (int v = 0; v < n; v++) {
+        const int xv = cls[v] == 0;
+        const double Cm = C[3 * v + (xv ? 0 : 1)], Cn = C[3 * v + (xv ? 1 : 0)], Cz = C[3 * v + 2];
+        const double Sm = S[3 * v + (xv ? 0 : 1)], Sn = S[3 * v + (xv ? 1 : 0)], Sz = S[3 * v + 2];
+        const double Um = U[3 * v + (xv ? 0 : 1)], Un = U[3 * v + (xv ? 1 : 0)], Uz = U[3 * v + 2];
+        const double Vz = V[3 * v + 2];
+        const int nm = xv ? g.nx : g.ny, nn = xv ? g.ny : g.nx;
+        const double dm = xv ? g.dx : g.dy, dn = xv ? g.dy : g.dx;
+        const double gm = xv ? g.cx : g.cy, gn = xv ? g.cy : g.cx;
+        ConeView& w = vt[v];
+        w.xv = xv;
+        w.mp0 = (float)((0.5 - 0.5 * nm) * dm + gm - Sm);
+        w.dm = (float)dm;
+        w.qn0 = (float)(gn - 0.5 * nn * dn - Sn);
+        w.dn = (float)dn;
+        w.A = (float)(Cn - Sn);
+        w.Cc = (float)(Cm - Sm);
+        w.Um = (float)Um;
+        w.Un = (float)Un;
+        w.o = (float)((Sn - gn) / dn + 0.5 * nn);
+        w.a0 = (float)((Sz - g.cz) / g.dz + 0.5 * g.nz);
+        w.pad = 0.f;
+        for (int iu = 0; iu < g.nu; iu++) {
+            ConeColumn& k = ct[(size_t)v * g.nu + iu];
+            k = ConeColumn{0.f, 0.f, 0.f, 0.f, 0.f, {0.f, 0.f, 0.f}};
+            const double uc = (iu - cu) * g.du;
+            const double e0m = (uc - 0.5 * g.du) * Um + Cm - Sm, e0n = (uc - 0.5 * g.du) * Un + Cn - Sn;
+            const double e1m = (uc + 0.5 * g.du) * Um + Cm - Sm, e1n = (uc + 0.5 * g.du) * Un + Cn - Sn;
+            const double dpc = uc * Um + Cm - Sm;
+            if (!(std::fabs(e0m) > 1e-6) || !(std::fabs(e1m) > 1e-6) || !(std::fabs(dpc) > 1e-6)) continue;
+            const double zr0 = uc * Uz + Cz - Sz - (cv + 0.5) * g.dv * Vz;   // row 0's lower edge, from the source
+            k.k0 = (float)(e0n / e0m / dn);
+            k.k1 = (float)(e1n / e1m / dn);
+            k.A1 = (float)(zr0 / dpc / g.dz);
+            k.T3 = (float)(dpc * g.dz / (Vz * g.dv));
+            k.W = (float)(dm / std::fabs(dpc) * (Vz / dpc > 0 ? 1.0 : -1.0));
+        }
+    }
+    views_.alloc(vt.size() * sizeof(ConeView));
+    cols_.alloc(ct.size() * sizeof(ConeColumn));
+    RN_HIP_CHECK(hipMemcpyAsync(views_.get(), vt.data(), vt.size() * sizeof(ConeView), hipMemcpyHostToDevice, st));
+    RN_HIP_CHECK(hipMemcpyAsync(cols_.get(), ct.data(), ct.size() * sizeof(ConeColumn), hipMemcpyHostToDevice, st));
     RN_HIP_CHECK(hipStreamSynchronize(st));
     geo = g;
     geo.rows_z = arb_rows_along_z(U, V, n) ? 1 : 0;
@@ -1215,21 +1414,35 @@ void ConePlan::dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t s
     launch(c);
 }
 
-void ConePlan::dd_adj(int nB, int nC, const float* prj, float* img, hipStream_t st) const
+void ConePlan::dd_adj(int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st) const
 {
     const ArbGeo& g = geo;
     RN_REQUIRE(nB > 0 && nC > 0 && (long)g.nx * nB * nC <= 65535, "cone batch too large for one launch");
     const float* v = vec_.as<float>();
-    // 1, 2 or 4 voxels per lane (the launch-shape tuner)
+    // candidates 0-2: 1, 2 or 4 voxels per lane summing the rows one by one;
+    // 3-5: 1, 2 or 4 voxels per lane from the row-sum table
+    const long ncols = (long)nB * nC * g.nu * g.nview;
+    RN_REQUIRE((ncols + 3) / 4 < 0x7fffffffL, "dd_cone_adj: too many detector columns");
     auto launch = [&](int c) {
-        const int zpl = 1 << c;
+        const int zpl = 1 << (c % 3);
         dim3 grd(cdiv(g.nz, 64 * zpl), cdiv(g.ny, 4), g.nx * nB * nC);
-        auto k = c == 0 ? k_dd_cone_adj<1> : c == 1 ? k_dd_cone_adj<2> : k_dd_cone_adj<4>;
-        hipLaunchKernelGGL(k, grd, dim3(256), 0, st, g, grid_of(g), prj, img, v, v + 3 * g.nview, v + 6 * g.nview,
-                           v + 9 * g.nview, cls_.as<int>(), nC);
+        if (c < 3) {
+            auto k = c == 0 ? k_dd_cone_adj<1> : c == 1 ? k_dd_cone_adj<2> : k_dd_cone_adj<4>;
+            hipLaunchKernelGGL(k, grd, dim3(256), 0, st, g, grid_of(g), prj, img, v, v + 3 * g.nview,
+                               v + 6 * g.nview, v + 9 * g.nview, cls_.as<int>(), nC);
+        } else {
+            work.ensure((size_t)ncols * (g.nv + 1) * sizeof(double));
+            double* T = work.as<double>();
+            hipLaunchKernelGGL(k_dd_cone_rowsum, dim3((unsigned)cdiv(ncols, 4)), dim3(256), 0, st, g, prj, v,
+                               v + 3 * g.nview, v + 6 * g.nview, v + 9 * g.nview, nC, ncols, T);
+            check_launch("dd_cone_rowsum");
+            auto k = c == 3 ? k_dd_cone_adj_tab<1> : c == 4 ? k_dd_cone_adj_tab<2> : k_dd_cone_adj_tab<4>;
+            hipLaunchKernelGGL(k, grd, dim3(256), 0, st, g, grid_of(g), T, img, views_.as<ConeView>(),
+                               cols_.as<ConeColumn>(), nC);
+        }
         check_launch("dd_cone_adj");
     };
-    const int c = tune_choose({TK_DD_CONE_ADJ, g.nx * g.ny, g.nz, g.nu * g.nv, g.nview, nB * nC}, 3, launch, st, true);
+    const int c = tune_choose({TK_DD_CONE_ADJ, g.nx * g.ny, g.nz, g.nu * g.nv, g.nview, nB * nC}, 6, launch, st, true);
     launch(c);
 }
 

@@ -679,7 +679,7 @@ extern "C" int rn_cone_backward(void* plan, int kind, const float* prj, float* i
         RN_REQUIRE(nBatches > 0 && nChannels > 0, "nBatches and nChannels must be positive");
         ConeHandle* p = static_cast<ConeHandle*>(plan);
         const hipStream_t st = static_cast<hipStream_t>(stream);
-        if (kind == RN_CONE_DD) p->plan.dd_adj(nBatches, nChannels, prj, img, st);
+        if (kind == RN_CONE_DD) p->plan.dd_adj(nBatches, nChannels, prj, img, p->work, st);
         else if (kind == RN_CONE_SIDDON) p->plan.siddon_bp(nBatches, nChannels, prj, img, p->work, st);
         else throw Error("unknown cone backprojector kind");
     });

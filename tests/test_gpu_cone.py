@@ -50,7 +50,7 @@ def test_dd_cone_matches_oracle(nC, off, du, dv):
     assert rel_l2(b, orc.dd_cone_adj(p, *vecs, geo, (n, n, nz))) <= TOL
 
 
-@pytest.mark.parametrize("kind,kernel,ncand", [("dd", "dd_cone_fp", 2), ("dd", "dd_cone_adj", 3),
+@pytest.mark.parametrize("kind,kernel,ncand", [("dd", "dd_cone_fp", 2), ("dd", "dd_cone_adj", 6),
                                                ("siddon", "siddon_cone_bp", 3)])
 def test_every_cone_launch_shape_matches_oracle(kind, kernel, ncand):
     # each launch shape of the tuned cone kernels (XCD-ordered or plain grid;
