@@ -1121,6 +1121,9 @@ __device__ __forceinline__ float asin_series(float x)
     return x * p;
 }
 
+// two adjacent floats at a 4-byte-aligned address (one 8-byte load)
+typedef float PairF __attribute__((ext_vector_type(2), aligned(4)));
+
 template <int ZC>
 __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* __restrict__ prj,
                                                     const float4* __restrict__ vt, float* __restrict__ img, int nC,
@@ -1185,32 +1188,49 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
             const float au = pu - fu;
             const float wu0 = (u0 >= 0 && u0 < h.nu) ? 1.0f - au : 0.0f;
             const float wu1 = (u0 + 1 >= 0 && u0 + 1 < h.nu) ? au : 0.0f;
-            const int ua = min(max(u0, 0), h.nu - 1), ub = min(max(u0 + 1, 0), h.nu - 1);
+            // the two columns as one 8-byte load of the pair (ua, ua + 1),
+            // ua = u0 clamped to [0, nu - 2]; the weights follow the columns
+            // (u0 and u0 + 1 inside the pair where they fall in it)
+            const int ua = min(max(u0, 0), h.nu - 2);
+            const float wlo = (u0 == ua ? wu0 : 0.0f) + (u0 + 1 == ua ? wu1 : 0.0f);
+            const float whi = (u0 == ua + 1 ? wu0 : 0.0f) + (u0 + 1 == ua + 1 ? wu1 : 0.0f);
+            // the view's rows from a wave-uniform base and 32-bit lane
+            // offsets; every voxel's two rows are loaded (rows clamped into
+            // the detector) before any is used, and rows outside it are
+            // dropped by selects
             const float* pk = pb + (long)k * h.nv * h.nu;
+            float v0w[ZC], v1w[ZC], w[ZC];
+            bool in0[ZC], in1[ZC];
+            PairF r0[ZC], r1[ZC];
             #pragma unroll
             for (int j = 0; j < ZC; ++j) {
-                if (i < is[j] || i >= is[j] + h.nviewPerPI) continue;
                 const float v = ((zc[j] - tv.z) - zsl) * mag;
                 const float q = fabsf(v * qs);
-                float w;
-                if (q <= h.Q) w = 1.0f;
-                else if (q < 1.0f) { w = __builtin_amdgcn_cosf((q - h.Q) * tq); w *= w; }
-                else w = 0.0f;
+                float wj;
+                if (q <= h.Q) wj = 1.0f;
+                else if (q < 1.0f) { wj = __builtin_amdgcn_cosf((q - h.Q) * tq); wj *= wj; }
+                else wj = 0.0f;
+                w[j] = wj;
                 const float pv = fmaf(v, idv, cv);
                 const float fv = floorf(pv);
                 const int v0 = (int)fv;
                 const float av = pv - fv;
+                in0[j] = v0 >= 0 && v0 < h.nv;
+                in1[j] = v0 + 1 >= 0 && v0 + 1 < h.nv;
+                v0w[j] = 1.0f - av;
+                v1w[j] = av;
+                const int ra = min(max(v0, 0), h.nv - 1), rb = min(max(v0 + 1, 0), h.nv - 1);
+                r0[j] = *reinterpret_cast<const PairF*>(pk + (__umul24(ra, h.nu) + (unsigned)ua));
+                r1[j] = *reinterpret_cast<const PairF*>(pk + (__umul24(rb, h.nu) + (unsigned)ua));
+            }
+            #pragma unroll
+            for (int j = 0; j < ZC; ++j) {
+                if (i < is[j] || i >= is[j] + h.nviewPerPI) continue;     // wave-uniform
                 float smp = 0.0f;
-                if (v0 >= 0 && v0 < h.nv) {
-                    const float* r = pk + (long)v0 * h.nu;
-                    smp += (1.0f - av) * (wu0 * r[ua] + wu1 * r[ub]);
-                }
-                if (v0 + 1 >= 0 && v0 + 1 < h.nv) {
-                    const float* r = pk + (long)(v0 + 1) * h.nu;
-                    smp += av * (wu0 * r[ua] + wu1 * r[ub]);
-                }
-                val[j] += smp * w;
-                tw[j] += w;
+                if (in0[j]) smp += v0w[j] * (wlo * r0[j].x + whi * r0[j].y);
+                if (in1[j]) smp += v1w[j] * (wlo * r1[j].x + whi * r1[j].y);
+                val[j] += smp * w[j];
+                tw[j] += w[j];
             }
         }
         #pragma unroll
@@ -1568,6 +1588,9 @@ void helical_rebin(const RebinGeo& r, int nB, int nC, const float* fan, float* p
 
 void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* img, DevBuf& work, hipStream_t st)
 {
+    RN_REQUIRE(h.nu >= 2, "the helical backprojection needs at least 2 detector columns");
+    RN_REQUIRE(h.nu < (1 << 24) && h.nv < (1 << 24) && (long)h.nu * h.nv < (1L << 31),
+               "helical backprojection: detector too large");
     const size_t per = (size_t)h.nu * h.nview * h.nv;
     const size_t vt_off = (2 * (size_t)nB * nC * per + 3) / 4 * 4;          // 16-byte aligned view table
     work.ensure((vt_off + 4 * (size_t)h.nview) * sizeof(float));
@@ -1584,7 +1607,7 @@ void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* im
         base = t2;
     }
     launch_transpose(base, t1, h.nu, h.nview * h.nv, nB * nC, st);
-    // candidates: columns of 4, 8 or 16 voxels per lane
+    // candidates: columns of 4, 8 or 2 voxels per lane
     auto launch = [&](int cand) {
         auto go = [&](auto kern, int zc) {
             const int nzc = cdiv(h.nz, zc);
@@ -1594,7 +1617,7 @@ void helical_bp(const HelicalGeo& h, int nB, int nC, const float* prj, float* im
         };
         if (cand == 0) go(k_helical_bp<4>, 4);
         else if (cand == 1) go(k_helical_bp<8>, 8);
-        else go(k_helical_bp<16>, 16);
+        else go(k_helical_bp<2>, 2);
         check_launch("helical_bp");
     };
     const int c = tune_choose({TK_HELICAL_BP, h.nx * h.ny, h.nz, h.nu * h.nv, h.nview, nB * nC}, 3, launch, st, true);

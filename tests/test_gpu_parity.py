@@ -601,6 +601,18 @@ def test_helical_bp_device_entry_equals_host_abi():
     assert np.array_equal(dev.cpu().numpy(), host)
 
 
+def test_helical_bp_rejects_a_single_detector_column():
+    # the gather reads each row's two interpolation columns as one pair
+    import torch
+
+    from reconnet_amd.helical import helical_backproject
+
+    par = torch.zeros((1, 1, 96, 8, 1), device="cuda")
+    with pytest.raises(RuntimeError, match="at least 2 detector columns"):
+        helical_backproject(par, 48, 0.1, 8.0, 12.0, nx=8, ny=8, nz=4, dx=1.0, dy=1.0, dz=1.0, du=1.0, dv=1.0,
+                            dsd=1000.0, dso=600.0)
+
+
 def test_helical_bp_full_size_slices_match_oracle():
     # 256^2 x 128 volume at 1 mm, 64 rows x 512 columns (1.0 x 1.2 mm), SID 595 mm,
     # SDD 1085.6 mm, 720 views per turn, 3 turns, pitch 1 (table feed per turn =
