@@ -342,7 +342,7 @@ __device__ __forceinline__ void siddon_rows_z_view(const ArbGeo& a, const SidBpC
         const float dzs = Vv.z * a.dv, dz_0 = fmaf(-w.cv, dzs, czs);
         for (int iv = iv0; iv <= iv1; ++iv) {
             const float dz = fmaf(dzs, (float)iv, dz_0), rz = fmaf(-tp, dz, wz_hi) + wz_lo;
-            const float len = sqrtf(fmaf(dz, dz, dxy2));
+            const float len = __builtin_amdgcn_sqrtf(fmaf(dz, dz, dxy2));      // hardware sqrt, 1 ulp
             const float val = qc[(long)iv * nC];
             // the lowest voxel's z slab; voxel k's is shifted by k dz_vox / dz
             float zl, zh, st;
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256) void k_siddon_arb_bp(ArbGeo a, Grid3K g, int p
                         slab(of, d, w.glo, w.ghi, g0, g1);
                         if (!(fminf(g1, 1.f) > fmaxf(g0, 0.f))) continue;
                     }
-                    acc[0] = fmaf((s1 - s0) * sqrtf(dot(d, d)), qc[(long)iv * nC], acc[0]);
+                    acc[0] = fmaf((s1 - s0) * __builtin_amdgcn_sqrtf(dot(d, d)), qc[(long)iv * nC], acc[0]);
                 }
             }
         }
@@ -943,7 +943,7 @@ __global__ __launch_bounds__(256) void k_dd_cone_adj(ArbGeo a, Grid3K g, const f
                 const float fiv = (float)iv;
                 const float z1 = fmaf(bdv, fiv, z1_0), z2 = z1 + zh;
                 const float lz = fmaf(lzs, fiv, lz_0);
-                const float wv = wcol * sqrtf(fmaf(lz, lz, lxy2)) * qcol[(long)iv * nC];
+                const float wv = wcol * __builtin_amdgcn_sqrtf(fmaf(lz, lz, lxy2)) * qcol[(long)iv * nC];
                 #pragma unroll
                 for (int k = 0; k < ZPL; ++k) {
                     const float oz = fminf(z2, fz + (float)(k + 1)) - fmaxf(z1, fz + (float)k);
@@ -1178,7 +1178,8 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
             const float4 tv = vt[k];                          // k is wave-uniform: scalar loads
             const float st = tv.x, ct = tv.y;
             const float u = x * st - y * ct;
-            const float mag = h.dsd * __builtin_amdgcn_rcpf(sqrtf(h.dso * h.dso - u * u) - (x * ct + y * st));
+            const float mag = h.dsd * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(h.dso * h.dso - u * u) -
+                                                            (x * ct + y * st));
             // source height zk - zr2pi asin(u / dso) as tv.z + zsl (tv.z exact in
             // float, so that z - tv.z loses nothing to the source's turn height)
             const float zsl = fmaf(-zr2pi, asin_series(u * idso), tv.w);
