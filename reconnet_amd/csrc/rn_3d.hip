@@ -1236,7 +1236,8 @@ __global__ __launch_bounds__(256) void k_helical_bp(HelicalGeo h, const float* _
         }
         #pragma unroll
         for (int j = 0; j < ZC; ++j)
-            if (i >= is[j] && i < is[j] + h.nviewPerPI) acc[j] += val[j] / (tw[j] + 1e-6f);
+            if (i >= is[j] && i < is[j] + h.nviewPerPI)
+                acc[j] = fmaf(val[j], __builtin_amdgcn_rcpf(tw[j] + 1e-6f), acc[j]);   // 1-ulp reciprocal
     }
     #pragma unroll
     for (int j = 0; j < ZC; ++j)

@@ -968,8 +968,9 @@ namespace rn {
 // cropped to [lb, ub] when crop is set.  dx0, dx1, dy0, dy1 are the differences
 // to the 4-neighbours (the pixel value itself at the image border); gTV =
 // sum d / psi / 2 and nTV = sum 1 / psi with psi = sqrt(d^2 + eps), here with
-// reciprocal square roots (about 1 ulp each).  g: the backprojected weighted
-// residual, d: the normaliser.
+// reciprocal square roots (about 1 ulp each), and the quotient as a product
+// with a 1-ulp reciprocal.  g: the backprojected weighted residual, d: the
+// normaliser.
 __device__ __forceinline__ float sqs_update_core(float x, float dx0, float dx1, float dy0, float dy1, float g,
                                                  float d, float step, float beta2, float eps, float lb, float ub,
                                                  int crop)
@@ -984,7 +985,7 @@ __device__ __forceinline__ float sqs_update_core(float x, float dx0, float dx1, 
         num = fmaf(beta2, gt, num);
         den = fmaf(beta2, nt, den);
     }
-    float xn = x - num / (den + 1e-6f);
+    float xn = x - num * __builtin_amdgcn_rcpf(den + 1e-6f);     // 1-ulp reciprocal
     if (crop) xn = xn < lb ? lb : (xn > ub ? ub : xn);
     return xn;
 }
