@@ -470,7 +470,8 @@ __global__ void k_dd_preweight(FanK g, const float* __restrict__ prj, float* __r
 // Reference-parity DD BP (DDBPFanKernelX/Y): pixel edges projected through
 // atan, average of the (pre-weighted) detector row over the footprint.  One
 // lane per pixel of S consecutive slices (grid z counts groups of S): the
-// footprint and the FBP weight are computed once for the S slices.
+// footprint and the FBP weight are computed once for the S slices (their
+// quotients through 1-ulp reciprocals).
 // The pixel's S slices (value type T: float, or float4 for 4 interleaved
 // slices) over every view: rows[q] are the wave-uniform bases of the S
 // pre-weighted [view][nu] sinograms.
@@ -489,12 +490,12 @@ __device__ __forceinline__ void dd_bp_pixel(const FanK& g, const T* const (&rows
         float ex, ey;   // edge offset (rotated frame) of +-half a pixel
         if (g.vcls[view] == 0) { ex = 0.5f * g.dx * ct; ey = -0.5f * g.dx * st; }
         else                   { ex = 0.5f * g.dy * st; ey = 0.5f * g.dy * ct; }
-        const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
-        const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
+        const float b1 = boff - atanf((rx - ex) * __builtin_amdgcn_rcpf(ry - ey + g.dso)) * inv_da;
+        const float b2 = boff - atanf((rx + ex) * __builtin_amdgcn_rcpf(ry + ey + g.dso)) * inv_da;
         float fw = 1.0f;
         if (fbp) {
             const float d = g.dso + ry;
-            fw = g.dso * g.dso / (rx * rx + d * d);
+            fw = g.dso * g.dso * __builtin_amdgcn_rcpf(fmaf(rx, rx, d * d));
         }
         seg_acc_rows<T, S>(rows, (unsigned)(view * g.nu) * (unsigned)sizeof(T), g.nu, fminf(b1, b2),
                            fmaxf(b1, b2), fw, acc);
@@ -609,12 +610,12 @@ __global__ __launch_bounds__(BX * BY, G >= 8 ? 4 : 1) void k_dd_fan_bp_ql(FanK g
         float ex, ey;
         if (g.vcls[view] == 0) { ex = 0.5f * g.dx * ct; ey = -0.5f * g.dx * sn; }
         else                   { ex = 0.5f * g.dy * sn; ey = 0.5f * g.dy * ct; }
-        const float b1 = boff - atanf((rx - ex) / (ry - ey + g.dso)) * inv_da;
-        const float b2 = boff - atanf((rx + ex) / (ry + ey + g.dso)) * inv_da;
+        const float b1 = boff - atanf((rx - ex) * __builtin_amdgcn_rcpf(ry - ey + g.dso)) * inv_da;
+        const float b2 = boff - atanf((rx + ex) * __builtin_amdgcn_rcpf(ry + ey + g.dso)) * inv_da;
         lo = fminf(b1, b2);
         hi = fmaxf(b1, b2);
         const float d = g.dso + ry;
-        fw = g.dso * g.dso / (rx * rx + d * d);
+        fw = g.dso * g.dso * __builtin_amdgcn_rcpf(fmaf(rx, rx, d * d));
         seg_avg_bins(g.nu, lo, hi, m0, m1);
         if (!inside) { m0 = g.nu; m1 = -1; }
     };
@@ -697,11 +698,11 @@ __global__ __launch_bounds__(BX * BY) void k_pixel_fan_bp(FanK g, const float* _
         const float ct = g.vcos[view], st = g.vsin[view];
         const float rx = x * ct + y * st, ry = -x * st + y * ct;
         const float d = ry + g.dso;
-        const float pu = g.cu - g.off_a - atanf(rx / d) * inv_da;
+        const float pu = g.cu - g.off_a - atanf(rx * __builtin_amdgcn_rcpf(d)) * inv_da;
         const float f0 = floorf(pu);
         const int i0 = (int)f0;
         const float f = pu - f0;
-        const float w = g.dso * g.dso / (rx * rx + d * d);
+        const float w = g.dso * g.dso * __builtin_amdgcn_rcpf(fmaf(rx, rx, d * d));
         const bool in0 = i0 >= 0 && i0 < g.nu, in1 = i0 + 1 >= 0 && i0 + 1 < g.nu;
         const float* p = prj + ((long)s0 * g.nview + view) * g.nu;
         #pragma unroll
