@@ -55,8 +55,9 @@ def test_dd_cone_matches_oracle(nC, off, du, dv):
 def test_every_cone_launch_shape_matches_oracle(kind, kernel, ncand):
     # each launch shape of the tuned cone kernels (XCD-ordered or plain grid;
     # 1, 2 or 4 voxels per lane stacked in z, with a z extent that leaves the
-    # last lane's stack partly outside the volume), two channels and an
-    # off-centre grid, against the FP64 oracle
+    # last lane's stack partly outside the volume; the DD adjoint's row loop
+    # and row-sum table), two slices of two channels and an off-centre grid,
+    # against the FP64 oracle
     from reconnet_amd._lib import autotune_pin
 
     n, nz, nu, nv, nview = 20, 13, 36, 18, 24
@@ -64,8 +65,8 @@ def test_every_cone_launch_shape_matches_oracle(kind, kernel, ncand):
     geo = dict(dx=1.0, dy=1.0, dz=1.0, du=1.3, dv=1.1, off_u=0.2, off_v=-0.3, cx=0.4, cy=-0.3, cz=0.2)
     plan = _plan(n, nz, nu, nv, 1.3, 1.1, vecs, off_u=0.2, off_v=-0.3, cx=0.4, cy=-0.3, cz=0.2)
     rng = np.random.default_rng(21)
-    x = rng.uniform(0, 1, (1, n, n, nz, 2)).astype(np.float32)
-    p = rng.uniform(0, 1, (1, nu, nview, nv, 2)).astype(np.float32)
+    x = rng.uniform(0, 1, (2, n, n, nz, 2)).astype(np.float32)
+    p = rng.uniform(0, 1, (2, nu, nview, nv, 2)).astype(np.float32)
     if kernel == "dd_cone_fp":
         ref, run = orc.dd_cone_fp(x, *vecs, geo, nu, nv), lambda: plan.forward(torch.from_numpy(x).cuda(), kind)
     else:

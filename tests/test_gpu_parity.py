@@ -583,6 +583,34 @@ def test_helical_bp_parity_wide_fan():
     assert rel_l2(out, ref) <= TOL
 
 
+@pytest.mark.parametrize("cand", [0, 1, 2])
+def test_helical_bp_truncated_detector_matches_oracle(cand):
+    # 24 columns of 1 mm against a 32 x 30 mm volume: samples fall off both
+    # ends of the detector, through the columns -1 / 0 and nu - 1 / nu whose
+    # interpolation pair is clamped into the detector
+    from reconnet_amd._lib import autotune_pin
+    from reconnet_amd.recon import ReconNet
+
+    npp, turns, nu, nv = 48, 3, 24, 24
+    r = ReconNet()
+    r.nx, r.ny, r.nz, r.dx, r.dy, r.dz = 32, 30, 13, 1.0, 1.0, 1.0
+    r.du, r.dv, r.off_u, r.off_v, r.dsd, r.dso = 1.0, 1.0, 0.3, -0.5, 1000.0, 600.0
+    prj = np.random.default_rng(23).uniform(0, 1, (1, nu, 2 * npp * turns, nv, 1)).astype(np.float32)
+    autotune_pin("helical_bp", cand)
+    try:
+        out = r.cHelicalBP4d(prj, npp, 0.1, 8.0, 12.0, mPI=1, Q=0.5)
+    finally:
+        autotune_pin("helical_bp", None)
+    g = dict(dx=1.0, dy=1.0, dz=1.0, du=1.0, dv=1.0, off_u=0.3, off_v=-0.5, dsd=1000.0, dso=600.0)
+    ref = orc.helical_bp(prj, g, (32, 30, 13), npp, 0.1, 8.0, 12.0, m_pi=1, Q=0.5)
+    assert np.abs(ref).max() > 0.1
+    # voxels beyond the detector's 12 mm half-width miss it in some views
+    xx, yy = np.meshgrid(np.arange(32) - 15.5, np.arange(30) - 14.5, indexing="ij")
+    rad = np.hypot(xx, yy)
+    assert ref[0, rad > 15, :, 0].mean() < 0.8 * ref[0, rad < 8, :, 0].mean()
+    assert rel_l2(out, ref) <= TOL
+
+
 def test_helical_bp_device_entry_equals_host_abi():
     import torch
 
