@@ -136,7 +136,8 @@ void cOSSQSConeAbitrary(float* res, const float* img, const float* prj,
 /* TotalVariation.cu:70.  The returned cost is reduced in FP64 (the reference sums FP32 on the host). */
 float cTVSQS2D(float* grad, float* norm, const float* img, int nx, int ny);
 
-/* Recon3D/HelicalBP.cu:132-135 (exported by the reference's separate libRecon3D) */
+/* Recon3D/HelicalBP.cu:132-135 (exported by the reference's separate libRecon3D).
+ * Needs nu >= 2 (each row's two interpolation columns are read as a pair). */
 void cHelicalBPFromParallelConebeamBackprojection(float* pImg, const float* pPrj,
         int nBatches, int nChannels, int nx, int ny, int nz, float dx, float dy, float dz,
         int nu, int nview, int nv, float du, float dv, float off_u, float off_v, float dsd, float dso,
@@ -287,9 +288,9 @@ int rn_cone_backward(void* plan, int kind, const float* prj, float* img, int nBa
 
 /* cHelicalBPFromParallelConebeamBackprojection (Recon3D/HelicalBP.cu:132) on
  * device pointers and a caller stream: prj [B][nu][nview][nv][C] parallel-cone
- * data, img [B][nx][ny][nz][C].  Its scratch (the projection transposes) is
- * kept per host thread: calls from one thread on different streams must not
- * overlap. */
+ * data, img [B][nx][ny][nz][C], nu >= 2.  Its scratch (the projection
+ * transposes) is kept per host thread: calls from one thread on different
+ * streams must not overlap. */
 int rn_helical_bp(const float* prj, float* img, int nBatches, int nChannels, int nx, int ny, int nz,
         float dx, float dy, float dz, int nu, int nview, int nv, float du, float dv, float off_u, float off_v,
         float dsd, float dso, int nviewPerPI, float theta0, float volZ0, float zrot, int mPI, float Q, void* stream);
