@@ -670,7 +670,10 @@ __device__ __forceinline__ ConeRay cone_ray(const ArbGeo& a, const Grid3K& g, co
 // XCD > 0: 1-D grid (padded to a multiple of XCD) renumbered so that each XCD
 // walks a contiguous range of (row block, column, view): the adjacent detector
 // columns of a view, whose footprints share plane cells, meet in one L2.
-template <int XCD>
+// OFF32: the 2 x 2 path addresses a plane's cells as a wave-uniform plane
+// base plus 32-bit byte offsets from 24-bit multiplies (byte strides below
+// 2^24, one slice's volume below 4 GiB: checked on the host).
+template <int XCD, bool OFF32>
 __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const float* __restrict__ vol,
                                                    float* __restrict__ prj, const float* Cp, const float* Up,
                                                    const float* Vp, const float* Sp, const int* __restrict__ cls,
@@ -737,8 +740,22 @@ __global__ __launch_bounds__(64) void k_dd_cone_fp(ArbGeo a, Grid3K g, const flo
             const float oz0 = fminf(hz, (float)(k0 + 1)) - lz, oz1 = fmaxf(hz - (float)(k0 + 1), 0.f);
             const int j1 = min(j0 + 1, nn - 1), k1 = min(k0 + 1, g.nz - 1);
             const float* P = base + p * sP * nC;
-            const float v00 = P[(j0 * sJ + k0) * nC], v01 = P[(j0 * sJ + k1) * nC];
-            const float v10 = P[(j1 * sJ + k0) * nC], v11 = P[(j1 * sJ + k1) * nC];
+            float v00, v01, v10, v11;
+            if (OFF32) {
+                const unsigned sj4 = (unsigned)(sJ * nC * 4), sk4 = (unsigned)(nC * 4);
+                const unsigned oj0 = __umul24((unsigned)j0, sj4), oj1 = __umul24((unsigned)j1, sj4);
+                const unsigned ok0 = __umul24((unsigned)k0, sk4), ok1 = __umul24((unsigned)k1, sk4);
+                const char* Pc = reinterpret_cast<const char*>(P);
+                v00 = *reinterpret_cast<const float*>(Pc + (oj0 + ok0));
+                v01 = *reinterpret_cast<const float*>(Pc + (oj0 + ok1));
+                v10 = *reinterpret_cast<const float*>(Pc + (oj1 + ok0));
+                v11 = *reinterpret_cast<const float*>(Pc + (oj1 + ok1));
+            } else {
+                v00 = P[(j0 * sJ + k0) * nC];
+                v01 = P[(j0 * sJ + k1) * nC];
+                v10 = P[(j1 * sJ + k0) * nC];
+                v11 = P[(j1 * sJ + k1) * nC];
+            }
             const float sum = fmaf(fmaf(v01, oz1, v00 * oz0), oe0, fmaf(v11, oz1, v10 * oz0) * oe1);
             acc = fmaf(sum, __builtin_amdgcn_rcpf(area), acc);
         }
@@ -1420,15 +1437,21 @@ void ConePlan::dd_fp(int nB, int nC, const float* img, float* prj, hipStream_t s
     const float* v = vec_.as<float>();
     const long nblocks = (long)cdiv(g.nv, 64) * g.nu * g.nview * nB * nC;
     RN_REQUIRE(nblocks + 8 < 0x7fffffffL, "dd_cone_fp: grid too large");
+    // 32-bit cell offsets when every byte stride fits 24 bits and a slice's
+    // volume 4 GiB
+    const bool off32 = (long)std::max(g.ny, g.nx) * g.nz * nC * 4 < (1L << 24) &&
+                       (long)g.nx * g.ny * g.nz * nC * 4 < (1L << 32);
     auto launch = [&](int c) {
         if (c == 0) {
             const long n8 = (nblocks + 7) / 8 * 8;
-            hipLaunchKernelGGL(k_dd_cone_fp<8>, dim3((unsigned)n8), dim3(64), 0, st, g, grid_of(g), img, prj, v,
-                               v + 3 * g.nview, v + 6 * g.nview, v + 9 * g.nview, cls_.as<int>(), nC, (int)nblocks);
+            auto k = off32 ? k_dd_cone_fp<8, true> : k_dd_cone_fp<8, false>;
+            hipLaunchKernelGGL(k, dim3((unsigned)n8), dim3(64), 0, st, g, grid_of(g), img, prj, v, v + 3 * g.nview,
+                               v + 6 * g.nview, v + 9 * g.nview, cls_.as<int>(), nC, (int)nblocks);
         } else {
             dim3 grd(cdiv(g.nv, 64), g.nu, g.nview * nB * nC);
-            hipLaunchKernelGGL(k_dd_cone_fp<0>, grd, dim3(64), 0, st, g, grid_of(g), img, prj, v, v + 3 * g.nview,
-                               v + 6 * g.nview, v + 9 * g.nview, cls_.as<int>(), nC, (int)nblocks);
+            auto k = off32 ? k_dd_cone_fp<0, true> : k_dd_cone_fp<0, false>;
+            hipLaunchKernelGGL(k, grd, dim3(64), 0, st, g, grid_of(g), img, prj, v, v + 3 * g.nview, v + 6 * g.nview,
+                               v + 9 * g.nview, cls_.as<int>(), nC, (int)nblocks);
         }
         check_launch("dd_cone_fp");
     };
