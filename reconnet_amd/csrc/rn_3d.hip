@@ -70,9 +70,12 @@ __device__ __forceinline__ void arb_ray(const ArbGeo& a, const Grid3K& g, bool p
     move_near_grid(src, dst, g);
 }
 
-// Siddon walk (Siddon.cu:90-238).  f(voxel linear index x*ny*nz + y*nz + z, length)
-template <class F>
-__device__ void siddon_walk(F3 src, F3 dst, const Grid3K& g, F f)
+// Siddon walk (Siddon.cu:90-238).  f(voxel, length), voxel the linear index
+// x*ny*nz + y*nz + z, or with OFF32 that index times 4 nC as a 32-bit byte
+// offset, kept by adding the axis stride at each step (a slice's volume below
+// 4 GiB: checked on the host).
+template <bool OFF32, class F>
+__device__ void siddon_walk(F3 src, F3 dst, const Grid3K& g, int nC, F f)
 {
     const float eps = 1e-6f;
     const float dist = sqrtf((src.x - dst.x) * (src.x - dst.x) + (src.y - dst.y) * (src.y - dst.y) +
@@ -106,24 +109,34 @@ __device__ void siddon_walk(F3 src, F3 dst, const Grid3K& g, F f)
     // with no motion along an axis the reference keeps alpha = 2*dist + k*0
     float prev = amin * dist;
     const int maxit = g.nx + g.ny + g.nz + 4;
+    // byte offset of the current voxel and the signed steps along each axis
+    // (unsigned arithmetic: a step out of the grid ends the walk unused)
+    const unsigned oz = 4u * (unsigned)nC, oy = oz * (unsigned)g.nz, ox = oy * (unsigned)g.ny;
+    unsigned off = OFF32 ? ((unsigned)gx * (unsigned)g.ny + (unsigned)gy) * oy + (unsigned)gz * oz : 0u;
+    const unsigned dox = px ? ox : 0u - ox, doy = py ? oy : 0u - oy, doz = pz ? oz : 0u - oz;
     for (int it = 0; it < maxit; ++it) {
         const int cx = gx, cy = gy, cz = gz;
+        const unsigned coff = off;
         float now;
         bool done = false;
         if (ax < ay && ax < az) {
             now = ax;
             bx += stx; ax = mx ? (bx - sx) * ix : ax;
             gx += px ? 1 : -1; done = (gx >= g.nx || gx < 0);
+            off += dox;
         } else if (ay < az) {
             now = ay;
             by += sty; ay = my ? (by - sy) * iy : ay;
             gy += py ? 1 : -1; done = (gy >= g.ny || gy < 0);
+            off += doy;
         } else {
             now = az;
             bz += stz; az = mz ? (bz - sz) * iz : az;
             gz += pz ? 1 : -1; done = (gz >= g.nz || gz < 0);
+            off += doz;
         }
-        f(((long)cx * g.ny + cy) * g.nz + cz, now - prev);
+        if (OFF32) f((long)coff, now - prev);
+        else f(((long)cx * g.ny + cy) * g.nz + cz, now - prev);
         prev = now;
         if (done) break;
     }
@@ -133,6 +146,7 @@ __device__ void siddon_walk(F3 src, F3 dst, const Grid3K& g, F f)
 // its rays share their in-plane path when the rows run along z and read
 // adjacent z voxels (z is the innermost image index), and its outputs are
 // adjacent; 4 columns per workgroup.
+template <bool OFF32>
 __global__ __launch_bounds__(256) void k_siddon_arb_fp(ArbGeo a, Grid3K g, int parallel, const float* __restrict__ img,
                                                        float* __restrict__ prj, const float* C, const float* U,
                                                        const float* V, const float* S, int nC)
@@ -146,7 +160,14 @@ __global__ __launch_bounds__(256) void k_siddon_arb_fp(ArbGeo a, Grid3K g, int p
     arb_ray(a, g, parallel != 0, C, U, V, S, view, iu, iv, s, t);
     const float* base = img + (long)b * g.nx * g.ny * g.nz * nC + c;
     float val = 0.f;
-    siddon_walk(s, t, g, [&](long vox, float len) { val += base[vox * nC] * len; });
+    if (OFF32) {
+        const char* bb = reinterpret_cast<const char*>(base);
+        siddon_walk<true>(s, t, g, nC, [&](long off, float len) {
+            val += *reinterpret_cast<const float*>(bb + (unsigned)off) * len;
+        });
+    } else {
+        siddon_walk<false>(s, t, g, nC, [&](long vox, float len) { val += base[vox * nC] * len; });
+    }
     prj[((((long)b * a.nu + iu) * a.nview + view) * a.nv + iv) * nC + c] = val;
 }
 
@@ -1281,8 +1302,10 @@ void siddon_arb_fp(const ArbGeo& g, bool parallel, int nB, int nC, const float* 
     RN_REQUIRE(detCenter && detU && detV && srcOrDir, "null geometry array");
     RN_REQUIRE((long)g.nview * nB * nC <= 65535 && cdiv(g.nu, 4) <= 65535, "siddon_arb_fp: grid too large");
     dim3 blk(256), grd(cdiv(g.nv, 64), cdiv(g.nu, 4), g.nview * nB * nC);
-    hipLaunchKernelGGL(k_siddon_arb_fp, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, img, prj, detCenter, detU,
-                       detV, srcOrDir, nC);
+    // 32-bit voxel offsets when one slice's volume stays below 4 GiB
+    auto k = (long)g.nx * g.ny * g.nz * nC * 4 < (1L << 32) ? k_siddon_arb_fp<true> : k_siddon_arb_fp<false>;
+    hipLaunchKernelGGL(k, grd, blk, 0, st, g, grid_of(g), parallel ? 1 : 0, img, prj, detCenter, detU, detV, srcOrDir,
+                       nC);
     check_launch("siddon_arb_fp");
 }
 
