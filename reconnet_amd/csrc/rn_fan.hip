@@ -1104,15 +1104,14 @@ void go_dd_bp_ql(const FanK& g, const float4* wprj, float* img, int ns, int fbp,
                        dim3(BX, BY), 0, st, g, wprj, img, ns, fbp);
 }
 using DdBpQFn = void (*)(const FanK&, const float4*, float*, int, int, hipStream_t);
-// (the tuner takes the first candidate within 3 % of the fastest; the direct
-// 4-group kernel is listed before the staged 4- and 2-group ones: at 32
-// slices of 256^2 the three time alike in isolation, but after the
-// preweighting and packing launches of a backward call the staged
-// 4-group kernel ran 0.496 against 0.410 ms per call)
+// LDS-staged shapes only with 8 slice groups per lane: staged 4- and 2-group
+// kernels timed within a few per cent of the direct ones back to back, so the
+// tuner took them at times, but after the preweighting and packing launches
+// of a backward call they ran 0.496 and 0.639 ms against 0.410 ms (32 slices
+// of 256^2) and 3.4 and 5.3 ms against 2.4 ms (32 slices of 512^2)
 const DdBpQFn kDdBpQ[] = {go_dd_bp_ql<8, 8, 8, 24>, go_dd_bp_ql<8, 8, 8, 32>, go_dd_bp_ql<8, 32, 8, 32>,
-                          go_dd_bp_ql<8, 32, 8, 24>, go_dd_bp_q<8, 8, 4>,       go_dd_bp_ql<8, 8, 4, 32>,
-                          go_dd_bp_ql<8, 8, 2, 32>,  go_dd_bp_q<16, 16, 2>,     go_dd_bp_q<8, 8, 2>,
-                          go_dd_bp_q<16, 16, 1>,     go_dd_bp_q<8, 8, 1>};
+                          go_dd_bp_ql<8, 32, 8, 24>, go_dd_bp_q<8, 8, 4>,       go_dd_bp_q<16, 16, 2>,
+                          go_dd_bp_q<8, 8, 2>,       go_dd_bp_q<16, 16, 1>,     go_dd_bp_q<8, 8, 1>};
 constexpr int kNDdBpQ = sizeof(kDdBpQ) / sizeof(kDdBpQ[0]);
 
 // The Siddon and DD gather kernels address one slice's image plane, ray table
